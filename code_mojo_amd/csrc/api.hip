@@ -1,0 +1,559 @@
+// api.hip -- the C ABI of libbfly_hip.so (declared in include/bfly_hip.h).
+//
+// Ownership: a bfly_state owns two HBM arrays (re, im) and one HIP stream at
+// default priority; every call on a state is ordered on that stream.  A
+// bfly_plan owns its host schedule and, after the first execute, a device
+// copy of its sweep / instruction / unitary records.  No call allocates on
+// the device except bfly_state_create and a plan's first execute.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bfly_plan.h"
+#include "kernels.h"
+
+using namespace bfly;
+
+struct bfly_state {
+    int32_t n = 0;
+    int32_t dtype = BFLY_C128;
+    int device = 0;
+    void* re = nullptr;
+    void* im = nullptr;
+    double* scratch = nullptr;   // reduction partials
+    hipStream_t stream = nullptr;
+};
+
+struct bfly_plan {
+    HostPlan hp;
+    int device = -1;
+    TileSweep* d_sweeps = nullptr;
+    TileInstr* d_instrs = nullptr;
+    double* d_udata = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(BFLY_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                              \
+    do {                                                 \
+        hipError_t _e = (expr);                          \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+size_t elem_size(int dtype) { return dtype == BFLY_C64 ? sizeof(float) : sizeof(double); }
+
+// ---- profiling ------------------------------------------------------------
+enum Family { F_TILE = 0, F_GATE, F_BITREV, F_SWAP, F_UNITARY, F_REDUCE, F_NFAM };
+const char* kFamilyNames[F_NFAM] = {"tile_pass_kernel", "gate_kernel", "bitrev_kernel", "swap_kernel",
+                                    "unitary_kernel", "reduce_kernel"};
+std::mutex g_prof_mu;
+int64_t g_prof_launches[F_NFAM] = {0};
+double g_prof_ms[F_NFAM] = {0};
+
+struct ProfScope {
+    bool on;
+    hipStream_t st;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    explicit ProfScope(bool enabled, hipStream_t s) : on(enabled), st(s) {}
+    void begin(int fam) {
+        if (!on) return;
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return;
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+        (void)hipEventRecord(a, st);
+        ev.push_back({fam, {a, b}});
+    }
+    void end() {
+        if (!on || ev.empty()) return;
+        (void)hipEventRecord(ev.back().second.second, st);
+    }
+    void collect() {
+        if (!on) return;
+        for (auto& e : ev) {
+            float ms = 0;
+            if (hipEventSynchronize(e.second.second) == hipSuccess &&
+                hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+                std::lock_guard<std::mutex> lk(g_prof_mu);
+                g_prof_launches[e.first] += 1;
+                g_prof_ms[e.first] += ms;
+            }
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        ev.clear();
+    }
+};
+
+int check_state(const bfly_state* s) {
+    if (!s || !s->re || !s->im) return fail(BFLY_EINVAL, "null or destroyed state");
+    return BFLY_OK;
+}
+
+int upload_plan(bfly_plan* p, int device) {
+    if (p->device == device) return BFLY_OK;
+    HostPlan& hp = p->hp;
+    if (!hp.sweeps.empty()) {
+        HIP_TRY(hipMalloc(&p->d_sweeps, hp.sweeps.size() * sizeof(TileSweep)), "hipMalloc(plan sweeps)");
+        HIP_TRY(hipMemcpy(p->d_sweeps, hp.sweeps.data(), hp.sweeps.size() * sizeof(TileSweep), hipMemcpyHostToDevice),
+                "hipMemcpy(plan sweeps)");
+    }
+    if (!hp.instrs.empty()) {
+        HIP_TRY(hipMalloc(&p->d_instrs, hp.instrs.size() * sizeof(TileInstr)), "hipMalloc(plan instrs)");
+        HIP_TRY(hipMemcpy(p->d_instrs, hp.instrs.data(), hp.instrs.size() * sizeof(TileInstr), hipMemcpyHostToDevice),
+                "hipMemcpy(plan instrs)");
+    }
+    if (!hp.udata.empty()) {
+        HIP_TRY(hipMalloc(&p->d_udata, hp.udata.size() * sizeof(double)), "hipMalloc(plan unitaries)");
+        HIP_TRY(hipMemcpy(p->d_udata, hp.udata.data(), hp.udata.size() * sizeof(double), hipMemcpyHostToDevice),
+                "hipMemcpy(plan unitaries)");
+    }
+    p->device = device;
+    return BFLY_OK;
+}
+
+int run_reduce(bfly_state* a, bfly_state* b, int mode, uint64_t mask, uint64_t value, double* out, int nout) {
+    int rc = check_state(a);
+    if (rc) return rc;
+    if (b) {
+        if ((rc = check_state(b))) return rc;
+        if (a->n != b->n || a->dtype != b->dtype) return fail(BFLY_EINVAL, "state shape mismatch");
+        if (b->stream != a->stream) {
+            // order b's pending work before the reduction on a's stream
+            HIP_TRY(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        }
+    }
+    HIP_TRY(hipSetDevice(a->device), "hipSetDevice");
+    const uint64_t size = 1ull << a->n;
+    ProfScope prof(false, a->stream);
+    HIP_TRY(launch_reduce(a->dtype, mode, a->re, a->im, b ? b->re : nullptr, b ? b->im : nullptr, size, mask, value,
+                          a->scratch, a->stream),
+            "reduce launch");
+    HIP_TRY(hipMemcpyAsync(out, a->scratch, sizeof(double) * nout, hipMemcpyDeviceToHost, a->stream), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(a->stream), "hipStreamSynchronize");
+    return BFLY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bfly_last_error(void) { return g_last_error.c_str(); }
+
+int bfly_version(void) { return 100; }
+
+int bfly_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int bfly_state_create(bfly_state** out, int32_t num_qubits, int32_t dtype) {
+    if (!out) return fail(BFLY_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (num_qubits < 1 || num_qubits > kMaxQubits) return fail(BFLY_EINVAL, "qubit count out of range: " + std::to_string(num_qubits));
+    if (dtype != BFLY_C128 && dtype != BFLY_C64) return fail(BFLY_EINVAL, "unknown dtype");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice (no GPU?)");
+    auto* s = new bfly_state();
+    s->n = num_qubits;
+    s->dtype = dtype;
+    s->device = dev;
+    const size_t bytes = (1ull << num_qubits) * elem_size(dtype);
+    hipError_t e = hipMalloc(&s->re, bytes);
+    if (e == hipSuccess) e = hipMalloc(&s->im, bytes);
+    if (e == hipSuccess) e = hipMalloc(&s->scratch, reduce_scratch_doubles() * sizeof(double));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = launch_reset(dtype, s->re, s->im, 1ull << num_qubits, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        const bool oom = e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation;
+        if (s->re) (void)hipFree(s->re);
+        if (s->im) (void)hipFree(s->im);
+        if (s->scratch) (void)hipFree(s->scratch);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        (void)hipGetLastError();
+        return fail(oom ? BFLY_ENOMEM : BFLY_EHIP,
+                    std::string("state allocation failed: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return BFLY_OK;
+}
+
+int bfly_state_destroy(bfly_state* s) {
+    if (!s) return BFLY_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->re) (void)hipFree(s->re);
+    if (s->im) (void)hipFree(s->im);
+    if (s->scratch) (void)hipFree(s->scratch);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->re = s->im = nullptr;
+    delete s;
+    return BFLY_OK;
+}
+
+int bfly_state_reset(bfly_state* s) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(launch_reset(s->dtype, s->re, s->im, 1ull << s->n, s->stream), "reset launch");
+    return BFLY_OK;
+}
+
+int bfly_state_upload(bfly_state* s, const void* re, const void* im, uint64_t len) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (!re || !im) return fail(BFLY_EINVAL, "null host array");
+    if (len != (1ull << s->n)) return fail(BFLY_EINVAL, "length mismatch: expected " + std::to_string(1ull << s->n));
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    const size_t bytes = len * elem_size(s->dtype);
+    HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    HIP_TRY(hipMemcpy(s->re, re, bytes, hipMemcpyHostToDevice), "hipMemcpy(upload re)");
+    HIP_TRY(hipMemcpy(s->im, im, bytes, hipMemcpyHostToDevice), "hipMemcpy(upload im)");
+    return BFLY_OK;
+}
+
+int bfly_state_download(bfly_state* s, void* re, void* im, uint64_t len) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (!re || !im) return fail(BFLY_EINVAL, "null host array");
+    if (len != (1ull << s->n)) return fail(BFLY_EINVAL, "length mismatch: expected " + std::to_string(1ull << s->n));
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    const size_t bytes = len * elem_size(s->dtype);
+    HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    HIP_TRY(hipMemcpy(re, s->re, bytes, hipMemcpyDeviceToHost), "hipMemcpy(download re)");
+    HIP_TRY(hipMemcpy(im, s->im, bytes, hipMemcpyDeviceToHost), "hipMemcpy(download im)");
+    return BFLY_OK;
+}
+
+int bfly_state_copy(bfly_state* dst, const bfly_state* src) {
+    int rc = check_state(dst);
+    if (rc) return rc;
+    if ((rc = check_state(src))) return rc;
+    if (dst->n != src->n || dst->dtype != src->dtype) return fail(BFLY_EINVAL, "state shape mismatch");
+    HIP_TRY(hipSetDevice(dst->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(src->stream), "hipStreamSynchronize");
+    const size_t bytes = (1ull << dst->n) * elem_size(dst->dtype);
+    HIP_TRY(hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy re)");
+    HIP_TRY(hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy im)");
+    return BFLY_OK;
+}
+
+int bfly_state_info(const bfly_state* s, int32_t* num_qubits, int32_t* dtype) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (num_qubits) *num_qubits = s->n;
+    if (dtype) *dtype = s->dtype;
+    return BFLY_OK;
+}
+
+int bfly_state_device_ptrs(bfly_state* s, void** re, void** im) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (re) *re = s->re;
+    if (im) *im = s->im;
+    return BFLY_OK;
+}
+
+int bfly_state_synchronize(bfly_state* s) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    return BFLY_OK;
+}
+
+int bfly_plan_create(bfly_plan** out, int32_t num_qubits, int32_t dtype, const bfly_op* ops, uint64_t nops,
+                     const double* udata, uint64_t udata_len, const bfly_exec_opts* opts) {
+    if (!out) return fail(BFLY_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (num_qubits < 1 || num_qubits > kMaxQubits) return fail(BFLY_EINVAL, "qubit count out of range");
+    if (dtype != BFLY_C128 && dtype != BFLY_C64) return fail(BFLY_EINVAL, "unknown dtype");
+    if (nops && !ops) return fail(BFLY_EINVAL, "null op array");
+    bfly_exec_opts o{};
+    if (opts) o = *opts;
+    if (o.mode != BFLY_MODE_FUSED && o.mode != BFLY_MODE_UNFUSED && o.mode != BFLY_MODE_BLOCK)
+        return fail(BFLY_EINVAL, "unknown execution mode");
+    auto* p = new bfly_plan();
+    std::string err;
+    int rc = build_plan(p->hp, num_qubits, dtype, ops, nops, udata, udata_len, o, err);
+    if (rc) {
+        delete p;
+        return fail(rc, err);
+    }
+    *out = p;
+    return BFLY_OK;
+}
+
+int bfly_plan_destroy(bfly_plan* p) {
+    if (!p) return BFLY_OK;
+    if (p->device >= 0) (void)hipSetDevice(p->device);
+    if (p->d_sweeps) (void)hipFree(p->d_sweeps);
+    if (p->d_instrs) (void)hipFree(p->d_instrs);
+    if (p->d_udata) (void)hipFree(p->d_udata);
+    delete p;
+    return BFLY_OK;
+}
+
+int bfly_plan_stats(const bfly_plan* p, int64_t* passes, int64_t* launches, int64_t* gates) {
+    if (!p) return fail(BFLY_EINVAL, "null plan");
+    if (passes) *passes = p->hp.passes;
+    if (launches) *launches = static_cast<int64_t>(p->hp.steps.size());
+    if (gates) *gates = p->hp.gates;
+    return BFLY_OK;
+}
+
+int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed) {
+    if (!p) return fail(BFLY_EINVAL, "null plan");
+    std::string out;
+    char line[512];
+    auto put = [&](const char* fmt, auto... args) {
+        std::snprintf(line, sizeof line, fmt, args...);
+        out += line;
+    };
+    auto put_m = [&](const double* m) {
+        out += " m=";
+        for (int i = 0; i < 8; ++i) {
+            std::snprintf(line, sizeof line, i ? ",%a" : "%a", m[i]);
+            out += line;
+        }
+        out += "\n";
+    };
+    const HostPlan& hp = p->hp;
+    put("plan n=%d dtype=%d steps=%zu passes=%lld gates=%lld\n", hp.n, hp.dtype, hp.steps.size(),
+        static_cast<long long>(hp.passes), static_cast<long long>(hp.gates));
+    for (const Step& st : hp.steps) {
+        switch (st.kind) {
+            case S_TILE: {
+                const TilePass& tp = st.tile;
+                put("tile k=%d sweeps=%d qpos=", tp.k, tp.nsweeps);
+                for (int i = 0; i < tp.k; ++i) put(i ? ",%d" : "%d", tp.qpos[i]);
+                out += "\n";
+                for (int s = 0; s < tp.nsweeps; ++s) {
+                    const TileSweep& sw = hp.sweeps[tp.first_sweep + s];
+                    put("sweep s=%d,%d,%d ninstr=%d\n", sw.s[0], sw.s[1], sw.s[2], sw.ninstr);
+                    for (int q = 0; q < sw.ninstr; ++q) {
+                        const TileInstr& in = hp.instrs[sw.first + q];
+                        put("instr kind=%d slot=%d lmask=%u omask=%llu obit=%llu", in.kind, in.slot, in.lmask,
+                            static_cast<unsigned long long>(in.omask), static_cast<unsigned long long>(in.obit));
+                        put_m(in.m);
+                    }
+                }
+                break;
+            }
+            case S_GATE:
+                put("gate kind=%d target=%d cmask=%llu", st.gate.kind, st.gate.target,
+                    static_cast<unsigned long long>(st.gate.cmask));
+                put_m(st.gate.m);
+                break;
+            case S_BITREV: put("bitrev mask=%llu\n", static_cast<unsigned long long>(st.mask)); break;
+            case S_SWAP: put("swap a=%d b=%d\n", st.a, st.b); break;
+            case S_UNITARY:
+                put("unitary target=%d m=%d controls=%llu uoff=%llu\n", st.a, st.b,
+                    static_cast<unsigned long long>(st.controls), static_cast<unsigned long long>(st.uoff));
+                break;
+        }
+    }
+    if (needed) *needed = out.size() + 1;
+    if (buf && len) {
+        const size_t n = out.size() + 1 <= len ? out.size() + 1 : len;
+        std::memcpy(buf, out.c_str(), n - 1);
+        buf[n - 1] = 0;
+    }
+    return BFLY_OK;
+}
+
+int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
+    if (!p) return fail(BFLY_EINVAL, "null plan");
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (p->hp.n != s->n) return fail(BFLY_EINVAL, "Circuit qubits mismatch: circuit has " + std::to_string(p->hp.n) +
+                                                      ", state has " + std::to_string(s->n));
+    if (p->hp.dtype != s->dtype) return fail(BFLY_EINVAL, "plan / state dtype mismatch");
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    if ((rc = upload_plan(p, s->device))) return rc;
+    ProfScope prof((p->hp.opts.flags & BFLY_FLAG_PROFILE) != 0, s->stream);
+    for (const Step& st : p->hp.steps) {
+        hipError_t e = hipSuccess;
+        switch (st.kind) {
+            case S_TILE:
+                prof.begin(F_TILE);
+                e = launch_tile_pass(s->dtype, s->re, s->im, st.tile, p->d_sweeps, p->d_instrs, s->stream);
+                break;
+            case S_GATE:
+                prof.begin(F_GATE);
+                e = launch_gate(s->dtype, s->re, s->im, st.gate, s->stream);
+                break;
+            case S_BITREV:
+                prof.begin(F_BITREV);
+                e = launch_bitrev(s->dtype, s->re, s->im, s->n, st.mask, s->stream);
+                break;
+            case S_SWAP:
+                prof.begin(F_SWAP);
+                e = launch_swap(s->dtype, s->re, s->im, s->n, st.a, st.b, s->stream);
+                break;
+            case S_UNITARY:
+                prof.begin(F_UNITARY);
+                e = launch_unitary(s->dtype, s->re, s->im, s->n, st.a, st.b, st.controls, p->d_udata + 2 * st.uoff,
+                                   s->stream);
+                break;
+        }
+        prof.end();
+        if (e != hipSuccess) {
+            prof.collect();
+            return hip_fail(e, "kernel launch");
+        }
+    }
+    prof.collect();
+    if (p->hp.opts.flags & BFLY_FLAG_SYNC) HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    return BFLY_OK;
+}
+
+int bfly_apply(bfly_state* s, const bfly_op* ops, uint64_t nops, const double* udata, uint64_t udata_len,
+               const bfly_exec_opts* opts) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    bfly_plan* p = nullptr;
+    if ((rc = bfly_plan_create(&p, s->n, s->dtype, ops, nops, udata, udata_len, opts))) return rc;
+    rc = bfly_plan_execute(p, s);
+    if (rc == BFLY_OK) {
+        // the plan's device records must outlive the queued kernels
+        hipError_t e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize");
+    }
+    bfly_plan_destroy(p);
+    return rc;
+}
+
+int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uint64_t len, const bfly_op* ops,
+                      uint64_t nops, const double* udata, uint64_t udata_len, const bfly_exec_opts* opts) {
+    bfly_state* s = nullptr;
+    int rc = bfly_state_create(&s, num_qubits, dtype);
+    if (rc) return rc;
+    rc = bfly_state_upload(s, re, im, len);
+    if (!rc) rc = bfly_apply(s, ops, nops, udata, udata_len, opts);
+    if (!rc) rc = bfly_state_download(s, re, im, len);
+    std::string keep = rc ? g_last_error : std::string();
+    bfly_state_destroy(s);
+    if (rc) g_last_error = keep;
+    return rc;
+}
+
+int bfly_bitrev(bfly_state* s, uint64_t target_mask) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    const uint64_t all = (1ull << s->n) - 1;
+    if (target_mask & ~all) return fail(BFLY_EINVAL, "Target out of bounds");
+    if (target_mask == all) target_mask = 0;
+    if (target_mask && __builtin_popcountll(target_mask) <= 1) return BFLY_OK;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(launch_bitrev(s->dtype, s->re, s->im, s->n, target_mask, s->stream), "bitrev launch");
+    return BFLY_OK;
+}
+
+int bfly_swap_qubits(bfly_state* s, int32_t a, int32_t b) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (a < 0 || a >= s->n || b < 0 || b >= s->n) return fail(BFLY_EINVAL, "Target out of bounds");
+    if (a == b) return BFLY_OK;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(launch_swap(s->dtype, s->re, s->im, s->n, a, b, s->stream), "swap launch");
+    return BFLY_OK;
+}
+
+int bfly_norm2(bfly_state* s, double* out) {
+    if (!out) return fail(BFLY_EINVAL, "null output");
+    return run_reduce(s, nullptr, R_NORM2, 0, 0, out, 1);
+}
+
+int bfly_prob_mask(bfly_state* s, uint64_t mask, uint64_t value, double* out) {
+    if (!out) return fail(BFLY_EINVAL, "null output");
+    if (value & ~mask) return fail(BFLY_EINVAL, "value has bits outside mask");
+    return run_reduce(s, nullptr, R_PROB_MASK, mask, value, out, 1);
+}
+
+int bfly_marginal(bfly_state* s, const int32_t* targets, int32_t m, double* out) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (!out || (m > 0 && !targets)) return fail(BFLY_EINVAL, "null argument");
+    if (m < 1 || m > 20) return fail(BFLY_EINVAL, "marginal over 1..20 qubits");
+    uint64_t seen = 0;
+    for (int q = 0; q < m; ++q) {
+        if (targets[q] < 0 || targets[q] >= s->n) return fail(BFLY_EINVAL, "Target out of bounds");
+        if ((seen >> targets[q]) & 1) return fail(BFLY_EINVAL, "Duplicate target");
+        seen |= 1ull << targets[q];
+    }
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    if (m <= 6) {
+        int t[6];
+        for (int q = 0; q < m; ++q) t[q] = targets[q];
+        HIP_TRY(launch_marginal(s->dtype, s->re, s->im, 1ull << s->n, t, m, s->scratch, s->stream), "marginal launch");
+        HIP_TRY(hipMemcpyAsync(out, s->scratch, sizeof(double) * (1ull << m), hipMemcpyDeviceToHost, s->stream),
+                "hipMemcpyAsync");
+        HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        return BFLY_OK;
+    }
+    // wider registers: one masked probability per outcome
+    for (uint64_t v = 0; v < (1ull << m); ++v) {
+        uint64_t value = 0;
+        for (int q = 0; q < m; ++q) value |= ((v >> q) & 1ull) << targets[q];
+        if ((rc = run_reduce(s, nullptr, R_PROB_MASK, seen, value, &out[v], 1))) return rc;
+    }
+    return BFLY_OK;
+}
+
+int bfly_inner(bfly_state* a, bfly_state* b, double* re, double* im) {
+    if (!re || !im) return fail(BFLY_EINVAL, "null output");
+    double out[2];
+    int rc = run_reduce(a, b, R_INNER, 0, 0, out, 2);
+    if (rc) return rc;
+    *re = out[0];
+    *im = out[1];
+    return BFLY_OK;
+}
+
+int bfly_diff_norm2(bfly_state* a, bfly_state* b, double* out) {
+    if (!out) return fail(BFLY_EINVAL, "null output");
+    return run_reduce(a, b, R_DIFF2, 0, 0, out, 1);
+}
+
+int bfly_scale(bfly_state* s, double scale) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(launch_scale(s->dtype, s->re, s->im, 1ull << s->n, scale, s->stream), "scale launch");
+    return BFLY_OK;
+}
+
+int bfly_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (int i = 0; i < F_NFAM; ++i) { g_prof_launches[i] = 0; g_prof_ms[i] = 0; }
+    return BFLY_OK;
+}
+
+int bfly_profile_get(int32_t index, const char** name, int64_t* launches, double* total_ms) {
+    if (index < 0 || index >= F_NFAM) return BFLY_EINVAL;
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (name) *name = kFamilyNames[index];
+    if (launches) *launches = g_prof_launches[index];
+    if (total_ms) *total_ms = g_prof_ms[index];
+    return BFLY_OK;
+}
+
+}  // extern "C"

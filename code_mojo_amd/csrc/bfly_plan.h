@@ -1,0 +1,114 @@
+// bfly_plan.h -- execution-plan records shared by the host planner
+// (planner.cpp) and the HIP kernels (kernels.hip).  Plain C++ PODs: no HIP
+// types, so the planner compiles with the host compiler alone.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bfly_hip.h"
+
+namespace bfly {
+
+// Largest tile (2^k amplitudes staged in LDS) and the widest state handled.
+constexpr int kMaxTileBits = 13;
+constexpr int kMinTileBits = 3;
+constexpr int kMaxQubits = 40;
+
+// Gate application kinds inside a tile sweep or a stand-alone gate launch.
+// The specialised kinds reproduce the reference's specialised scalar kernels
+// (transformations_scalar.mojo:178-394,397-566); GEN is process_pair_scalar.
+enum InstrKind : int32_t {
+    I_GEN = 0,     // generic 2x2 on a pair
+    I_H = 1,       // (u +- v) * 0.7071067811865476
+    I_X = 2,       // exchange
+    I_RY = 3,      // real rotation: m[0] = cos(t/2), m[1] = sin(t/2)
+    I_DIAG = 4,    // a *= d0 (target bit 0) or d1 (target bit 1): m[0..1] = d0, m[2..3] = d1
+    I_PHASE1 = 5,  // a *= d1 where target bit is 1 only (P, controlled P)
+};
+
+// One gate inside a tile pass.  128 bytes.
+struct alignas(16) TileInstr {
+    int32_t kind;      // InstrKind
+    int32_t slot;      // pair ops: index 0..2 of the target in the sweep's slot set;
+                       // diagonal ops: local bit of the target, or -1 when it is outside the tile
+    uint32_t lmask;    // control bits inside the tile, as local-index bits
+    uint32_t pad0;
+    uint64_t omask;    // control bits outside the tile, as global-index bits
+    uint64_t obit;     // diagonal ops with target outside the tile: its global bit (else 0)
+    double m[8];       // matrix / parameters in FP64
+    float mf[8];       // the same in FP32 (complex64 states)
+};
+static_assert(sizeof(TileInstr) == 128, "TileInstr layout");
+
+// A sweep: every group of 8 amplitudes spanned by the three local bits
+// `s` is loaded from LDS into registers, all the sweep's gates run in
+// registers, and the group is written back.  32 bytes.
+struct alignas(16) TileSweep {
+    int32_t s[3];      // local bit positions, distinct
+    int32_t ninstr;
+    int32_t first;     // index of the first instruction
+    int32_t pad[3];
+};
+static_assert(sizeof(TileSweep) == 32, "TileSweep layout");
+
+// Launch descriptor of one tile pass (passed by value as a kernel argument).
+struct TilePass {
+    int32_t n;
+    int32_t k;          // tile bits
+    int32_t nsweeps;
+    int32_t first_sweep;
+    int32_t qpos[kMaxTileBits];   // global bit of local bit i (ascending)
+    int32_t nouter;
+    int32_t opos[kMaxQubits];     // global bits outside the tile (ascending)
+};
+
+// A stand-alone single-gate launch (unfused mode, and tiny states).
+struct GateLaunch {
+    int32_t kind;       // InstrKind
+    int32_t target;
+    int32_t npos;       // number of inserted-zero positions (target + controls)
+    int32_t pos[kMaxQubits + 1];  // ascending positions where zero bits are inserted
+    uint64_t cmask;     // control mask (global bits)
+    uint64_t count;     // work items: pairs (or single amplitudes for diagonal kinds)
+    double m[8];
+    float mf[8];
+};
+
+enum StepKind : int32_t { S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4 };
+
+struct Step {
+    StepKind kind;
+    TilePass tile;       // S_TILE
+    GateLaunch gate;     // S_GATE
+    uint64_t mask = 0;   // S_BITREV: target mask (0 = all)
+    int32_t a = 0, b = 0;  // S_SWAP qubits; S_UNITARY: a = target, b = m
+    uint64_t controls = 0; // S_UNITARY control mask
+    uint64_t uoff = 0;     // S_UNITARY: complex offset into the plan's unitary data
+};
+
+struct HostPlan {
+    int32_t n = 0;
+    int32_t dtype = BFLY_C128;
+    bfly_exec_opts opts{};
+    std::vector<Step> steps;
+    std::vector<TileSweep> sweeps;
+    std::vector<TileInstr> instrs;
+    std::vector<double> udata;   // complex, interleaved
+    int64_t gates = 0;
+    int64_t passes = 0;          // HBM sweeps over the state
+};
+
+// Builds `plan` from validated ops.  Returns BFLY_OK or an error code and
+// fills `err`.
+int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
+               const double* udata, uint64_t udata_len, const bfly_exec_opts& opts, std::string& err);
+
+// validate_circuit (butterfly/core/executors.mojo:129-185) for lowered ops.
+int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_len, std::string& err);
+
+// Default tile / low-bit choices for a dtype (also used by the autotuner).
+int default_tile_bits(int32_t dtype);
+int default_low_bits(int32_t dtype);
+
+}  // namespace bfly

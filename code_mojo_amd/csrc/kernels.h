@@ -1,0 +1,36 @@
+// kernels.h -- launch wrappers of the CDNA4 kernels (kernels.hip), called by
+// the C-ABI layer (api.hip).  Every wrapper enqueues on `stream` and returns
+// the hipError_t of the launch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bfly_plan.h"
+
+namespace bfly {
+
+// (b) one tile pass: every gate of the pass in one HBM read + write.
+hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P,
+                            const TileSweep* d_sweeps, const TileInstr* d_instrs, hipStream_t stream);
+// (a) one gate over the whole state (coalesced strided butterfly pairs).
+hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream);
+// (c) bit reversal of all bits (mask 0) or of the bits in `mask`, in place.
+hipError_t launch_bitrev(int dtype, void* re, void* im, int n, uint64_t mask, hipStream_t stream);
+hipError_t launch_swap(int dtype, void* re, void* im, int n, int a, int b, hipStream_t stream);
+// dense 2^m block on [target, target+m), udata complex interleaved FP64.
+hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int m, uint64_t controls,
+                          const double* d_u, hipStream_t stream);
+hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
+hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
+
+// (d) reductions.  Partials live in `scratch` (at least reduce_scratch_doubles()
+// doubles); the final value(s) are left in scratch[0..nout) on the device.
+enum ReduceMode : int { R_NORM2 = 0, R_PROB_MASK = 1, R_DIFF2 = 2, R_INNER = 3 };
+size_t reduce_scratch_doubles();
+hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre,
+                         const void* bim, uint64_t size, uint64_t mask, uint64_t value,
+                         double* scratch, hipStream_t stream);
+// marginal distribution over m <= 6 target bits; 2^m results in scratch[0..)
+hipError_t launch_marginal(int dtype, const void* re, const void* im, uint64_t size, const int* targets,
+                           int m, double* scratch, hipStream_t stream);
+
+}  // namespace bfly

@@ -1,0 +1,771 @@
+// kernels.hip -- hand-written CDNA4 (gfx950) kernels of the statevector engine.
+//
+// State layout in HBM: SoA, re[2^n] and im[2^n] of T (double for complex128,
+// float for complex64), exactly the reference's State (state.mojo:19-33), so
+// host copies are plain memcpys.
+//
+// Kernels:
+//   (a) gate_kernel       one 2x2 gate, pairs enumerated by inserting zero bits
+//                         at the target and control positions (only matching
+//                         pairs are read: a multi-controlled gate touches
+//                         2^(n-1-c) pairs, not the reference's 2^(n-1) scan,
+//                         transformations_scalar.mojo:586-594)
+//   (b) tile_pass_kernel  a 2^K-amplitude tile staged in LDS with coalesced
+//                         16-byte loads; the pass's gates run as register
+//                         sweeps over 8-amplitude groups; one HBM read + write
+//                         per pass (generalises execute_grid_fused's row-local
+//                         groups, executors.mojo:1062-1093)
+//   (c) bitrev kernels    in place: 32x32 LDS tiles pair index (h,m,l) with
+//                         (rev l, rev m, rev h), both sides coalesced
+//                         (bit_reverse_state, state.mojo:131-204, is out of place)
+//   (d) reduce kernels    FP64 TwoSum-compensated sums with a fixed reduction
+//                         tree (deterministic), for norms / probabilities /
+//                         inner products (grover_classical.mojo:8-31,109-131)
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace bfly {
+
+namespace {
+
+__device__ __forceinline__ uint64_t insert_zero64(uint64_t x, int pos) {
+    const uint64_t lo = x & ((1ull << pos) - 1);
+    return ((x >> pos) << (pos + 1)) | lo;
+}
+
+__device__ __forceinline__ uint32_t insert_zero32(uint32_t x, int pos) {
+    const uint32_t lo = x & ((1u << pos) - 1);
+    return ((x >> pos) << (pos + 1)) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_params(const double* m, const float* mf, T* g);
+template <>
+__device__ __forceinline__ void load_params<double>(const double* m, const float*, double* g) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) g[i] = m[i];
+}
+template <>
+__device__ __forceinline__ void load_params<float>(const double*, const float* mf, float* g) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) g[i] = mf[i];
+}
+
+// process_pair_scalar (transformations_scalar.mojo:10-40).
+template <typename T>
+__device__ __forceinline__ void pair_gen(T& ur, T& ui, T& vr, T& vi, const T* g) {
+    const T a = ur, b = ui, c = vr, d = vi;
+    ur = a * g[0] - b * g[1] + c * g[2] - d * g[3];
+    ui = a * g[1] + b * g[0] + c * g[3] + d * g[2];
+    vr = a * g[4] - b * g[5] + c * g[6] - d * g[7];
+    vi = a * g[5] + b * g[4] + c * g[7] + d * g[6];
+}
+
+// transform_h_scalar's (u +- v) * 0.7071067811865476 (transformations_scalar.mojo:241-283).
+template <typename T>
+__device__ __forceinline__ void pair_h(T& ur, T& ui, T& vr, T& vi) {
+    const T s = static_cast<T>(0.7071067811865476);
+    const T a = ur, b = ui, c = vr, d = vi;
+    ur = (a + c) * s;
+    ui = (b + d) * s;
+    vr = (a - c) * s;
+    vi = (b - d) * s;
+}
+
+// transform_ry_scalar (transformations_scalar.mojo:384-391): g[0]=cos, g[1]=sin.
+template <typename T>
+__device__ __forceinline__ void pair_ry(T& ur, T& ui, T& vr, T& vi, const T* g) {
+    const T a = ur, b = ui, c = vr, d = vi;
+    ur = a * g[0] - c * g[1];
+    ui = b * g[0] - d * g[1];
+    vr = a * g[1] + c * g[0];
+    vi = b * g[1] + d * g[0];
+}
+
+// v *= (dr + i di) as in transform_p_scalar (transformations_scalar.mojo:330-333).
+template <typename T>
+__device__ __forceinline__ void cmul(T& vr, T& vi, T dr, T di) {
+    const T a = vr, b = vi;
+    vr = a * dr - b * di;
+    vi = a * di + b * dr;
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_pair_kind(int kind, T& ur, T& ui, T& vr, T& vi, const T* g) {
+    switch (kind) {
+        case I_H: pair_h(ur, ui, vr, vi); break;
+        case I_X: { T t = ur; ur = vr; vr = t; t = ui; ui = vi; vi = t; } break;
+        case I_RY: pair_ry(ur, ui, vr, vi, g); break;
+        case I_DIAG: cmul(ur, ui, g[0], g[1]); cmul(vr, vi, g[2], g[3]); break;
+        case I_PHASE1: cmul(vr, vi, g[2], g[3]); break;
+        default: pair_gen(ur, ui, vr, vi, g); break;
+    }
+}
+
+// ---------------------------------------------------------------- (a) gate
+template <typename T>
+__global__ __launch_bounds__(256) void gate_kernel(T* __restrict__ re, T* __restrict__ im, const GateLaunch G) {
+    T g[8];
+    load_params<T>(G.m, G.mf, g);
+    const uint64_t tbit = 1ull << G.target;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < G.count; w += stride) {
+        uint64_t i0 = w;
+        for (int p = 0; p < G.npos; ++p) i0 = insert_zero64(i0, G.pos[p]);
+        i0 |= G.cmask;
+        if (G.kind == I_PHASE1) {
+            const uint64_t i1 = i0 | tbit;
+            T vr = re[i1], vi = im[i1];
+            cmul(vr, vi, g[2], g[3]);
+            re[i1] = vr;
+            im[i1] = vi;
+        } else {
+            const uint64_t i1 = i0 | tbit;
+            T ur = re[i0], ui = im[i0], vr = re[i1], vi = im[i1];
+            apply_pair_kind<T>(G.kind, ur, ui, vr, vi, g);
+            re[i0] = ur;
+            im[i0] = ui;
+            re[i1] = vr;
+            im[i1] = vi;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- (b) tile pass
+template <typename T>
+struct Vec;
+template <>
+struct Vec<double> { using type = double2; static constexpr int N = 2; };
+template <>
+struct Vec<float> { using type = float4; static constexpr int N = 4; };
+
+// XOR swizzle of the LDS index: folds bits [S, 2S) into [0, S) so that the
+// lanes of a sweep (which differ in the lowest non-slot bits) spread over
+// the LDS banks.  S = 4 for 8-byte elements, 5 for 4-byte elements.
+template <typename T>
+__device__ __forceinline__ uint32_t swz(uint32_t j) {
+    constexpr int S = sizeof(T) == 8 ? 4 : 5;
+    return j ^ ((j >> S) & ((1u << S) - 1));
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t local_to_global(uint32_t j, const TilePass& P) {
+    uint64_t g = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) g |= static_cast<uint64_t>((j >> i) & 1u) << P.qpos[i];
+    return g;
+}
+
+template <typename T, int SLOT>
+__device__ __forceinline__ void sweep_pairs(int kind, T* ar, T* ai, const uint32_t* jm, uint32_t lmask, const T* g) {
+#pragma unroll
+    for (int m0 = 0; m0 < 8; ++m0) {
+        if (m0 & (1 << SLOT)) continue;
+        const int m1 = m0 | (1 << SLOT);
+        if ((jm[m0] & lmask) != lmask) continue;
+        apply_pair_kind<T>(kind, ar[m0], ai[m0], ar[m1], ai[m1], g);
+    }
+}
+
+template <typename T, int K, int NT>
+__global__ __launch_bounds__(NT) void tile_pass_kernel(T* __restrict__ re, T* __restrict__ im, const TilePass P,
+                                                       const TileSweep* __restrict__ sweeps,
+                                                       const TileInstr* __restrict__ instrs) {
+    constexpr uint32_t TS = 1u << K;
+    constexpr int VN = Vec<T>::N;
+    using V = typename Vec<T>::type;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* s_re = reinterpret_cast<T*>(lds_raw);
+    T* s_im = s_re + TS;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tile = blockIdx.x;
+
+    uint64_t base = 0;
+    for (int i = 0; i < P.nouter; ++i) base |= ((tile >> i) & 1ull) << P.opos[i];
+
+    // stage in: VN consecutive local indices map to VN consecutive amplitudes
+    // (the low bits of every tile are the low qubits), one 16-byte load each.
+    for (uint32_t e = tid; e < TS / VN; e += NT) {
+        const uint32_t j = e * VN;
+        const uint64_t gidx = base + local_to_global<K>(j, P);
+        const V vr = *reinterpret_cast<const V*>(re + gidx);
+        const V vi = *reinterpret_cast<const V*>(im + gidx);
+        const T* pr = reinterpret_cast<const T*>(&vr);
+        const T* pi = reinterpret_cast<const T*>(&vi);
+#pragma unroll
+        for (int v = 0; v < VN; ++v) {
+            s_re[swz<T>(j + v)] = pr[v];
+            s_im[swz<T>(j + v)] = pi[v];
+        }
+    }
+    __syncthreads();
+
+    for (int s = 0; s < P.nsweeps; ++s) {
+        const TileSweep sw = sweeps[P.first_sweep + s];
+        const int s0 = sw.s[0], s1 = sw.s[1], s2 = sw.s[2];
+        const int lo = min(s0, min(s1, s2));
+        const int hi = max(s0, max(s1, s2));
+        const int mid = s0 + s1 + s2 - lo - hi;
+        uint32_t dep[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) dep[m] = ((m & 1u) << s0) | (((m >> 1) & 1u) << s1) | (((m >> 2) & 1u) << s2);
+        for (uint32_t gi = tid; gi < TS / 8; gi += NT) {
+            const uint32_t jb = insert_zero32(insert_zero32(insert_zero32(gi, lo), mid), hi);
+            uint32_t jm[8];
+            T ar[8], ai[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                jm[m] = jb | dep[m];
+                ar[m] = s_re[swz<T>(jm[m])];
+                ai[m] = s_im[swz<T>(jm[m])];
+            }
+            for (int q = 0; q < sw.ninstr; ++q) {
+                const TileInstr& in = instrs[sw.first + q];
+                if ((base & in.omask) != in.omask) continue;
+                const int kind = in.kind;
+                T g[8];
+                load_params<T>(in.m, in.mf, g);
+                if (kind == I_DIAG || kind == I_PHASE1) {
+                    const int lb = in.slot;
+                    const uint32_t obit = (base & in.obit) ? 1u : 0u;
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) {
+                        if ((jm[m] & in.lmask) != in.lmask) continue;
+                        const uint32_t bit = lb >= 0 ? ((jm[m] >> lb) & 1u) : obit;
+                        if (bit) cmul(ar[m], ai[m], g[2], g[3]);
+                        else if (kind == I_DIAG) cmul(ar[m], ai[m], g[0], g[1]);
+                    }
+                } else if (in.slot == 0) {
+                    sweep_pairs<T, 0>(kind, ar, ai, jm, in.lmask, g);
+                } else if (in.slot == 1) {
+                    sweep_pairs<T, 1>(kind, ar, ai, jm, in.lmask, g);
+                } else {
+                    sweep_pairs<T, 2>(kind, ar, ai, jm, in.lmask, g);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                s_re[swz<T>(jm[m])] = ar[m];
+                s_im[swz<T>(jm[m])] = ai[m];
+            }
+        }
+        __syncthreads();
+    }
+
+    // stage out
+    for (uint32_t e = tid; e < TS / VN; e += NT) {
+        const uint32_t j = e * VN;
+        const uint64_t gidx = base + local_to_global<K>(j, P);
+        V vr, vi;
+        T* pr = reinterpret_cast<T*>(&vr);
+        T* pi = reinterpret_cast<T*>(&vi);
+#pragma unroll
+        for (int v = 0; v < VN; ++v) {
+            pr[v] = s_re[swz<T>(j + v)];
+            pi[v] = s_im[swz<T>(j + v)];
+        }
+        *reinterpret_cast<V*>(re + gidx) = vr;
+        *reinterpret_cast<V*>(im + gidx) = vi;
+    }
+}
+
+constexpr int tile_threads(int k) { return k >= 11 ? 256 : (k >= 10 ? 128 : 64); }
+
+template <typename T, int K>
+hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
+    constexpr int NT = tile_threads(K);
+    const size_t lds = 2ull * (1ull << K) * sizeof(T);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_pass_kernel<T, K, NT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    hipLaunchKernelGGL((tile_pass_kernel<T, K, NT>), dim3(static_cast<unsigned>(ntiles)), dim3(NT), lds, st, re, im,
+                       P, sw, in);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tile_t(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
+    switch (P.k) {
+        case 3: return launch_tile_k<T, 3>(re, im, P, sw, in, st);
+        case 4: return launch_tile_k<T, 4>(re, im, P, sw, in, st);
+        case 5: return launch_tile_k<T, 5>(re, im, P, sw, in, st);
+        case 6: return launch_tile_k<T, 6>(re, im, P, sw, in, st);
+        case 7: return launch_tile_k<T, 7>(re, im, P, sw, in, st);
+        case 8: return launch_tile_k<T, 8>(re, im, P, sw, in, st);
+        case 9: return launch_tile_k<T, 9>(re, im, P, sw, in, st);
+        case 10: return launch_tile_k<T, 10>(re, im, P, sw, in, st);
+        case 11: return launch_tile_k<T, 11>(re, im, P, sw, in, st);
+        case 12: return launch_tile_k<T, 12>(re, im, P, sw, in, st);
+        case 13: return launch_tile_k<T, 13>(re, im, P, sw, in, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------- (c) permutations
+__device__ __forceinline__ uint32_t rev_bits32(uint32_t x, int nbits) {
+    return nbits == 0 ? 0u : (__brev(x) >> (32 - nbits));
+}
+
+__device__ __forceinline__ uint64_t rev_bits64(uint64_t x, int nbits) {
+    return nbits == 0 ? 0ull : (__brevll(x) >> (64 - nbits));
+}
+
+constexpr int kRevB = 5;  // 32 x 32 tiles
+
+// Index i = (h: top B bits | m: middle M bits | l: low B bits); its reversal is
+// (rev l | rev m | rev h).  Block m (with m <= rev m) swaps tile m with tile
+// rev m through LDS; loads and stores run along l (32 contiguous amplitudes).
+template <typename T>
+__global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* __restrict__ re, T* __restrict__ im, int n) {
+    constexpr int B = kRevB;
+    constexpr int W = 1 << B;
+    __shared__ T a_re[W][W + 1], a_im[W][W + 1], b_re[W][W + 1], b_im[W][W + 1];
+    const int M = n - 2 * B;
+    const uint64_t m = blockIdx.x;
+    const uint64_t rm = rev_bits64(m, M);
+    if (m > rm) return;
+    const bool self = (m == rm);
+    const int hs = n - B;
+    for (int e = threadIdx.x; e < W * W; e += 256) {
+        const int h = e >> B, l = e & (W - 1);
+        const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | l;
+        a_re[h][l] = re[ia];
+        a_im[h][l] = im[ia];
+        if (!self) {
+            const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | l;
+            b_re[h][l] = re[ib];
+            b_im[h][l] = im[ib];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < W * W; e += 256) {
+        const int h = e >> B, l = e & (W - 1);
+        const int rh = static_cast<int>(rev_bits32(h, B)), rl = static_cast<int>(rev_bits32(l, B));
+        const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | l;
+        if (self) {
+            re[ia] = a_re[rl][rh];
+            im[ia] = a_im[rl][rh];
+        } else {
+            re[ia] = b_re[rl][rh];
+            im[ia] = b_im[rl][rh];
+            const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | l;
+            re[ib] = a_re[rl][rh];
+            im[ib] = a_im[rl][rh];
+        }
+    }
+}
+
+// Reversal of the bits in `mask` (partial_bit_reverse_state, state.mojo:252-301)
+// or of all bits: exchange i and perm(i) when i < perm(i).
+template <typename T>
+__global__ __launch_bounds__(256) void bitrev_simple_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size,
+                                                            uint64_t mask, int n) {
+    int pos[64];
+    int k = 0;
+    for (int b = 0; b < n; ++b)
+        if ((mask >> b) & 1) pos[k++] = b;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        uint64_t j = i & ~mask;
+        for (int b = 0; b < k; ++b) j |= ((i >> pos[b]) & 1ull) << pos[k - 1 - b];
+        if (i < j) {
+            const T tr = re[i], ti = im[i];
+            re[i] = re[j];
+            im[i] = im[j];
+            re[j] = tr;
+            im[j] = ti;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t count, int lo,
+                                                   int hi, int a, int b) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < count; w += stride) {
+        const uint64_t base = insert_zero64(insert_zero64(w, lo), hi);
+        const uint64_t i = base | (1ull << a), j = base | (1ull << b);
+        const T tr = re[i], ti = im[i];
+        re[i] = re[j];
+        im[i] = im[j];
+        re[j] = tr;
+        im[j] = ti;
+    }
+}
+
+// Dense 2^m block (apply_unitary / apply_controlled_unitary,
+// transformations_unitary.mojo:22-139): a block handles G = 256 / dim groups;
+// each group's 2^m inputs are staged in LDS, each thread forms output rows.
+struct UnitaryArgs {
+    uint64_t ngroups;
+    uint64_t controls;
+    int32_t target, m, npos;
+    int32_t pos[kMaxQubits + 1];  // ascending positions of block and control bits
+};
+
+__device__ __forceinline__ uint64_t unitary_base(uint64_t grp, const UnitaryArgs& A) {
+    uint64_t base = grp;
+    for (int p = 0; p < A.npos; ++p) base = insert_zero64(base, A.pos[p]);
+    return base | A.controls;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void unitary_kernel(T* __restrict__ re, T* __restrict__ im, const UnitaryArgs A,
+                                                      const double* __restrict__ u) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int dim = 1 << A.m;
+    const int G = dim >= 256 ? 1 : 256 / dim;
+    T* x_re = reinterpret_cast<T*>(lds_raw);
+    T* x_im = x_re + G * dim;
+    for (uint64_t g0 = static_cast<uint64_t>(blockIdx.x) * G; g0 < A.ngroups;
+         g0 += static_cast<uint64_t>(gridDim.x) * G) {
+        for (int e = threadIdx.x; e < G * dim; e += 256) {
+            const int gl = e / dim, k = e % dim;
+            const uint64_t grp = g0 + gl;
+            if (grp >= A.ngroups) continue;
+            const uint64_t idx = unitary_base(grp, A) + (static_cast<uint64_t>(k) << A.target);
+            x_re[e] = re[idx];
+            x_im[e] = im[idx];
+        }
+        __syncthreads();
+        T o_re[4], o_im[4];  // up to 4 rows per thread (dim <= 1024)
+        int cnt = 0;
+        for (int e = threadIdx.x; e < G * dim; e += 256, ++cnt) {
+            const int gl = e / dim, row = e % dim;
+            T sr = 0, si = 0;
+            for (int col = 0; col < dim; ++col) {
+                const T ur = static_cast<T>(u[2 * (row * dim + col)]);
+                const T ui = static_cast<T>(u[2 * (row * dim + col) + 1]);
+                const T vr = x_re[gl * dim + col], vi = x_im[gl * dim + col];
+                sr += ur * vr - ui * vi;
+                si += ur * vi + ui * vr;
+            }
+            o_re[cnt] = sr;
+            o_im[cnt] = si;
+        }
+        __syncthreads();  // every input of the group is read before any output lands
+        cnt = 0;
+        for (int e = threadIdx.x; e < G * dim; e += 256, ++cnt) {
+            const int gl = e / dim, row = e % dim;
+            const uint64_t grp = g0 + gl;
+            if (grp >= A.ngroups) continue;
+            const uint64_t idx = unitary_base(grp, A) + (static_cast<uint64_t>(row) << A.target);
+            re[idx] = o_re[cnt];
+            im[idx] = o_im[cnt];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void reset_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        re[i] = i == 0 ? T(1) : T(0);
+        im[i] = T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size, double s) {
+    const T sc = static_cast<T>(s);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        re[i] *= sc;
+        im[i] *= sc;
+    }
+}
+
+// ---------------------------------------------------------------- (d) reductions
+// TwoSum accumulation (Knuth): s + c carries the sum with its rounding error.
+__device__ __forceinline__ void two_sum_add(double& s, double& c, double x) {
+    const double t = s + x;
+    const double bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+}
+
+__device__ __forceinline__ void acc_merge(double& s, double& c, double s2, double c2) {
+    two_sum_add(s, c, s2);
+    c += c2;
+}
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict__ are, const T* __restrict__ aim,
+                                                             const T* __restrict__ bre, const T* __restrict__ bim,
+                                                             uint64_t size, uint64_t mask, uint64_t value,
+                                                             double* __restrict__ partial) {
+    constexpr int NC = MODE == R_INNER ? 2 : 1;
+    __shared__ double sh[NC][2][kRedThreads];
+    double s[NC], c[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) { s[q] = 0; c[q] = 0; }
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kRedThreads;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kRedThreads + threadIdx.x; i < size; i += stride) {
+        const double ar = are[i], ai = aim[i];
+        if (MODE == R_NORM2) {
+            two_sum_add(s[0], c[0], ar * ar + ai * ai);
+        } else if (MODE == R_PROB_MASK) {
+            if ((i & mask) == value) two_sum_add(s[0], c[0], ar * ar + ai * ai);
+        } else if (MODE == R_DIFF2) {
+            const double dr = ar - static_cast<double>(bre[i]), di = ai - static_cast<double>(bim[i]);
+            two_sum_add(s[0], c[0], dr * dr + di * di);
+        } else {
+            const double br = bre[i], bi = bim[i];
+            two_sum_add(s[0], c[0], ar * br + ai * bi);
+            two_sum_add(s[NC - 1], c[NC - 1], ar * bi - ai * br);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) { sh[q][0][threadIdx.x] = s[q]; sh[q][1][threadIdx.x] = c[q]; }
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(w)) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
+                acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
+                sh[q][0][threadIdx.x] = ss;
+                sh[q][1][threadIdx.x] = cc;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            partial[2 * (q * gridDim.x + blockIdx.x)] = sh[q][0][0];
+            partial[2 * (q * gridDim.x + blockIdx.x) + 1] = sh[q][1][0];
+        }
+    }
+}
+
+// Final stage: channel q's nb partial (s, c) pairs -> out[q] = s + c.
+__global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double* __restrict__ partial, int nb, int nch,
+                                                                   double* __restrict__ out) {
+    __shared__ double sh[2][kRedThreads];
+    for (int q = 0; q < nch; ++q) {
+        double s = 0, c = 0;
+        for (int i = threadIdx.x; i < nb; i += kRedThreads)
+            acc_merge(s, c, partial[2 * (q * nb + i)], partial[2 * (q * nb + i) + 1]);
+        sh[0][threadIdx.x] = s;
+        sh[1][threadIdx.x] = c;
+        __syncthreads();
+        for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < static_cast<unsigned>(w)) {
+                double ss = sh[0][threadIdx.x], cc = sh[1][threadIdx.x];
+                acc_merge(ss, cc, sh[0][threadIdx.x + w], sh[1][threadIdx.x + w]);
+                sh[0][threadIdx.x] = ss;
+                sh[1][threadIdx.x] = cc;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = sh[0][0] + sh[1][0];
+        __syncthreads();
+    }
+}
+
+constexpr int kMargThreads = 64;
+constexpr int kMargMaxBits = 6;
+
+// Marginal over m target bits: each thread owns 2^m compensated bins in LDS.
+template <typename T>
+__global__ __launch_bounds__(kMargThreads) void marginal_kernel(const T* __restrict__ re, const T* __restrict__ im,
+                                                                uint64_t size, int m, int t0, int t1, int t2, int t3,
+                                                                int t4, int t5, double* __restrict__ partial) {
+    __shared__ double bins[kMargThreads][1 << kMargMaxBits][2];
+    const int nb = 1 << m;
+    const int t[6] = {t0, t1, t2, t3, t4, t5};
+    for (int b = 0; b < nb; ++b) { bins[threadIdx.x][b][0] = 0; bins[threadIdx.x][b][1] = 0; }
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kMargThreads;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kMargThreads + threadIdx.x; i < size; i += stride) {
+        int b = 0;
+        for (int q = 0; q < m; ++q) b |= static_cast<int>((i >> t[q]) & 1ull) << q;
+        const double ar = re[i], ai = im[i];
+        double s = bins[threadIdx.x][b][0], c = bins[threadIdx.x][b][1];
+        two_sum_add(s, c, ar * ar + ai * ai);
+        bins[threadIdx.x][b][0] = s;
+        bins[threadIdx.x][b][1] = c;
+    }
+    __syncthreads();
+    // fixed-order merge across threads, one bin per thread
+    for (int b = threadIdx.x; b < nb; b += kMargThreads) {
+        double s = 0, c = 0;
+        for (int th = 0; th < kMargThreads; ++th) acc_merge(s, c, bins[th][b][0], bins[th][b][1]);
+        partial[2 * (b * gridDim.x + blockIdx.x)] = s;
+        partial[2 * (b * gridDim.x + blockIdx.x) + 1] = c;
+    }
+}
+
+inline unsigned grid_for(uint64_t work, unsigned per_block = 256, unsigned cap = 16384) {
+    uint64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+template <typename T>
+hipError_t launch_gate_t(T* re, T* im, const GateLaunch& G, hipStream_t st) {
+    hipLaunchKernelGGL(gate_kernel<T>, dim3(grid_for(G.count)), dim3(256), 0, st, re, im, G);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
+    const uint64_t size = 1ull << n;
+    if (mask == 0 && n >= 2 * kRevB) {
+        const uint64_t blocks = 1ull << (n - 2 * kRevB);
+        hipLaunchKernelGGL(bitrev_tiled_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im, n);
+    } else {
+        if (mask == 0) mask = size - 1;
+        hipLaunchKernelGGL(bitrev_simple_kernel<T>, dim3(grid_for(size)), dim3(256), 0, st, re, im, size, mask, n);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- wrappers
+hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P, const TileSweep* d_sweeps,
+                            const TileInstr* d_instrs, hipStream_t stream) {
+    if (dtype == BFLY_C64)
+        return launch_tile_t<float>(static_cast<float*>(re), static_cast<float*>(im), P, d_sweeps, d_instrs, stream);
+    return launch_tile_t<double>(static_cast<double*>(re), static_cast<double*>(im), P, d_sweeps, d_instrs, stream);
+}
+
+hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream) {
+    if (dtype == BFLY_C64) return launch_gate_t<float>(static_cast<float*>(re), static_cast<float*>(im), G, stream);
+    return launch_gate_t<double>(static_cast<double*>(re), static_cast<double*>(im), G, stream);
+}
+
+hipError_t launch_bitrev(int dtype, void* re, void* im, int n, uint64_t mask, hipStream_t stream) {
+    if (dtype == BFLY_C64) return launch_bitrev_t<float>(static_cast<float*>(re), static_cast<float*>(im), n, mask, stream);
+    return launch_bitrev_t<double>(static_cast<double*>(re), static_cast<double*>(im), n, mask, stream);
+}
+
+hipError_t launch_swap(int dtype, void* re, void* im, int n, int a, int b, hipStream_t stream) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint64_t count = 1ull << (n - 2);
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(swap_kernel<float>, dim3(grid_for(count)), dim3(256), 0, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), count, lo, hi, a, b);
+    else
+        hipLaunchKernelGGL(swap_kernel<double>, dim3(grid_for(count)), dim3(256), 0, stream, static_cast<double*>(re),
+                           static_cast<double*>(im), count, lo, hi, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int m, uint64_t controls,
+                          const double* d_u, hipStream_t stream) {
+    UnitaryArgs A{};
+    A.controls = controls;
+    A.target = target;
+    A.m = m;
+    const uint64_t bits = (((1ull << m) - 1) << target) | controls;
+    for (int b = 0; b < n; ++b)
+        if ((bits >> b) & 1) A.pos[A.npos++] = b;
+    A.ngroups = 1ull << (n - A.npos);
+    const int dim = 1 << m;
+    const int G = dim >= 256 ? 1 : 256 / dim;
+    const size_t lds = 2ull * G * dim * (dtype == BFLY_C64 ? sizeof(float) : sizeof(double));
+    const unsigned grid = grid_for(A.ngroups, static_cast<unsigned>(G), 65535);
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(unitary_kernel<float>, dim3(grid), dim3(256), lds, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), A, d_u);
+    else
+        hipLaunchKernelGGL(unitary_kernel<double>, dim3(grid), dim3(256), lds, stream, static_cast<double*>(re),
+                           static_cast<double*>(im), A, d_u);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream) {
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(reset_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), size);
+    else
+        hipLaunchKernelGGL(reset_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<double*>(re),
+                           static_cast<double*>(im), size);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream) {
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), size, scale);
+    else
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<double*>(re),
+                           static_cast<double*>(im), size, scale);
+    return hipGetLastError();
+}
+
+size_t reduce_scratch_doubles() {
+    const size_t a = 2ull * 2 * kRedBlocks + 8;
+    const size_t b = 2ull * (1 << kMargMaxBits) * kRedBlocks + (1 << kMargMaxBits);
+    return a > b ? a : b;
+}
+
+namespace {
+template <typename T, int MODE>
+void launch_reduce_t(const void* are, const void* aim, const void* bre, const void* bim, uint64_t size, uint64_t mask,
+                     uint64_t value, double* scratch, unsigned nb, hipStream_t st) {
+    hipLaunchKernelGGL((reduce_kernel<T, MODE>), dim3(nb), dim3(kRedThreads), 0, st, static_cast<const T*>(are),
+                       static_cast<const T*>(aim), static_cast<const T*>(bre), static_cast<const T*>(bim), size, mask,
+                       value, scratch + 8);
+}
+}  // namespace
+
+hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre, const void* bim,
+                         uint64_t size, uint64_t mask, uint64_t value, double* scratch, hipStream_t stream) {
+    // the block count depends only on the size: same size -> same tree -> same bits
+    const unsigned nb = grid_for(size, kRedThreads * 4, kRedBlocks);
+    const bool f = dtype == BFLY_C64;
+    switch (mode) {
+        case R_NORM2: f ? launch_reduce_t<float, R_NORM2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
+                        : launch_reduce_t<double, R_NORM2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        case R_PROB_MASK: f ? launch_reduce_t<float, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
+                            : launch_reduce_t<double, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        case R_DIFF2: f ? launch_reduce_t<float, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
+                        : launch_reduce_t<double, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        case R_INNER: f ? launch_reduce_t<float, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
+                        : launch_reduce_t<double, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nch = mode == R_INNER ? 2 : 1;
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, scratch + 8, static_cast<int>(nb), nch,
+                       scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_marginal(int dtype, const void* re, const void* im, uint64_t size, const int* targets, int m,
+                           double* scratch, hipStream_t stream) {
+    if (m < 1 || m > kMargMaxBits) return hipErrorInvalidValue;
+    int t[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < m; ++q) t[q] = targets[q];
+    const unsigned nb = grid_for(size, kMargThreads * 16, kRedBlocks);
+    double* part = scratch + (1 << kMargMaxBits);
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(marginal_kernel<float>, dim3(nb), dim3(kMargThreads), 0, stream,
+                           static_cast<const float*>(re), static_cast<const float*>(im), size, m, t[0], t[1], t[2],
+                           t[3], t[4], t[5], part);
+    else
+        hipLaunchKernelGGL(marginal_kernel<double>, dim3(nb), dim3(kMargThreads), 0, stream,
+                           static_cast<const double*>(re), static_cast<const double*>(im), size, m, t[0], t[1], t[2],
+                           t[3], t[4], t[5], part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, part, static_cast<int>(nb), 1 << m,
+                       scratch);
+    return hipGetLastError();
+}
+
+}  // namespace bfly

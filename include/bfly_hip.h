@@ -1,0 +1,181 @@
+/*
+ * bfly_hip.h -- C ABI of the MI355X statevector engine (libbfly_hip.so).
+ *
+ * This library is the GPU execution strategy of the Butterfly simulator
+ * (learnqc/code_mojo).  A Butterfly host (the Mojo `ExecutorFn` shim, the
+ * Python mirror in code_mojo_amd/, or any C caller) lowers a circuit's
+ * transformation list into an array of `bfly_op` once per circuit and crosses
+ * this boundary once, instead of dispatching one CPU kernel per
+ * transformation.
+ *
+ * Conventions (all identical to the reference):
+ *   - amplitudes are SoA: re[2^n], im[2^n]; qubit q is bit q of the basis
+ *     index (LSB = qubit 0)                     butterfly/core/state.mojo:19-33
+ *   - a 1-qubit gate maps (a0, a1) -> G (a0, a1) with a0 = amp[i] (target bit
+ *     0) and a1 = amp[i + 2^t]     butterfly/core/transformations_scalar.mojo:10-40
+ *   - a gate with controls acts where (index & controls) == controls
+ *                                  butterfly/core/transformations_scalar.mojo:590
+ *   - an m-qubit unitary acts on qubits [target, target+m), local index k
+ *     <-> offset k << target, row-major, out = U in
+ *                                  butterfly/core/transformations_unitary.mojo:22-75
+ *   - BITREV reverses the order of the target bits (all bits when the mask is
+ *     0): out[i] = in[rev(i)]           butterfly/core/state.mojo:131-204,252-301
+ *
+ * Every entry point returns 0 on success or a negative BFLY_E* code; the
+ * message of the last failure on the calling thread is bfly_last_error().
+ * A state handle owns its HBM copy and one HIP stream; calls on one handle
+ * are ordered on that stream, calls on different handles may overlap.
+ * Pointers passed in are borrowed for the duration of the call.
+ */
+#ifndef BFLY_HIP_H
+#define BFLY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define BFLY_OK            0
+#define BFLY_EINVAL       -1  /* bad argument (qubit out of range, size mismatch, ...) */
+#define BFLY_ENOMEM       -2  /* device or host allocation failed / over budget */
+#define BFLY_EHIP         -3  /* HIP runtime error */
+#define BFLY_EUNSUPPORTED -4  /* transformation the GPU strategy cannot lower */
+#define BFLY_ENUMERIC     -5  /* norm check failed (non-finite or drifted state) */
+
+/* ---- amplitude precision ---------------------------------------------- */
+#define BFLY_C128 0           /* f64 re/im: reference default FloatType   types.mojo:7 */
+#define BFLY_C64  1           /* f32 re/im: reference `-D FLOAT32` build  types.mojo:7 */
+
+/* ---- op kinds ---------------------------------------------------------- */
+#define BFLY_OP_GATE     0    /* 2x2 gate on `target`, controls mask `controls`  circuit.mojo:46-67 */
+#define BFLY_OP_BITREV   1    /* reverse bits of mask `aux` (0 = all qubits)     quantum_circuit.mojo:99-136 */
+#define BFLY_OP_SWAP     2    /* exchange qubits `target` and `arg_i`            state.mojo:304-317, executors.mojo:231-265 */
+#define BFLY_OP_UNITARY  3    /* dense 2^m block, m = `arg_i`, on [target, target+m); matrix at udata[2*aux ..]
+                                 (re,im interleaved, row-major); optional controls mask  transformations_unitary.mojo:22-139 */
+
+/* ---- gate kinds: the reference's GateKind values (gates.mojo:83-91) ---- */
+#define BFLY_GATE_H      0
+#define BFLY_GATE_P      1
+#define BFLY_GATE_X      2
+#define BFLY_GATE_Y      3
+#define BFLY_GATE_Z      4
+#define BFLY_GATE_RX     5
+#define BFLY_GATE_RY     6
+#define BFLY_GATE_RZ     7
+#define BFLY_GATE_CUSTOM 8
+
+/* One lowered transformation.  104 bytes, 8-byte aligned. */
+typedef struct bfly_op {
+    int32_t  kind;      /* BFLY_OP_* */
+    int32_t  gate;      /* BFLY_GATE_* (GATE ops) */
+    int32_t  target;    /* target qubit / first qubit of a block / qubit a of a swap */
+    int32_t  arg_i;     /* SWAP: qubit b;  UNITARY: m */
+    uint64_t controls;  /* control-qubit mask (GATE, UNITARY) */
+    uint64_t aux;       /* BITREV: target mask;  UNITARY: complex offset into udata */
+    double   theta;     /* angle for P / RX / RY / RZ (GateInfo.arg) */
+    double   mat[8];    /* 2x2 gate, row-major, (re, im) per entry: g00 g01 g10 g11 */
+} bfly_op;
+
+/* ---- execution modes --------------------------------------------------- */
+#define BFLY_MODE_FUSED   0   /* LDS tile passes: many gates per HBM sweep (default) */
+#define BFLY_MODE_UNFUSED 1   /* one strided butterfly launch per gate */
+#define BFLY_MODE_BLOCK   2   /* gates fused into dense 2^k blocks (MFMA for c64) */
+
+typedef struct bfly_exec_opts {
+    int32_t mode;            /* BFLY_MODE_* */
+    int32_t tile_bits;       /* k: amplitudes per LDS tile = 2^k (0 = tuned default) */
+    int32_t low_bits;        /* L: low qubits kept in every tile for coalescing (0 = default) */
+    int32_t block_qubits;    /* BLOCK mode block width (0 = 5) */
+    int32_t max_gates_per_pass; /* 0 = unlimited */
+    int32_t renorm_every;    /* BLOCK mode: FP64 compensated norm check every N block passes (0 = off) */
+    int32_t threads;         /* workgroup size for tile passes (0 = tuned default) */
+    int32_t flags;           /* BFLY_FLAG_* */
+} bfly_exec_opts;
+
+#define BFLY_FLAG_SYNC      1 /* block until the device work is done before returning */
+#define BFLY_FLAG_PROFILE   2 /* bracket every launch with HIP events (see bfly_profile_*) */
+
+typedef struct bfly_state bfly_state;   /* opaque: device-resident SoA state */
+typedef struct bfly_plan  bfly_plan;    /* opaque: compiled pass schedule    */
+
+/* ---- library ----------------------------------------------------------- */
+/* Thread-local message describing the last failure (empty when none). */
+const char* bfly_last_error(void);
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int bfly_version(void);
+/* Number of visible devices (0 when no GPU); never fails. */
+int bfly_device_count(void);
+
+/* ---- state: replaces State / QuantumState (state.mojo:19-98) ----------- */
+/* Allocate a 2^n-amplitude state on the current device, initialised to |0..0>
+   (state.mojo:26-33).  Budget errors return BFLY_ENOMEM. */
+int bfly_state_create(bfly_state** out, int32_t num_qubits, int32_t dtype);
+int bfly_state_destroy(bfly_state* s);
+/* Re-initialise to |0..0>. */
+int bfly_state_reset(bfly_state* s);
+/* Copy host SoA arrays (len = 2^n elements of the state's dtype) in / out. */
+int bfly_state_upload(bfly_state* s, const void* re, const void* im, uint64_t len);
+int bfly_state_download(bfly_state* s, void* re, void* im, uint64_t len);
+/* Copy one state into another of the same shape and dtype (device to device). */
+int bfly_state_copy(bfly_state* dst, const bfly_state* src);
+int bfly_state_info(const bfly_state* s, int32_t* num_qubits, int32_t* dtype);
+/* Device pointers of the re / im arrays (for callers that share memory). */
+int bfly_state_device_ptrs(bfly_state* s, void** re, void** im);
+/* Wait for all work queued on the state's stream. */
+int bfly_state_synchronize(bfly_state* s);
+
+/* ---- execution: replaces execute() / ExecutorFn (executors.mojo:611-613,723-730) */
+/* Compile a pass schedule for an n-qubit circuit (host only, no device work). */
+int bfly_plan_create(bfly_plan** out, int32_t num_qubits, int32_t dtype,
+                     const bfly_op* ops, uint64_t nops,
+                     const double* udata, uint64_t udata_len,
+                     const bfly_exec_opts* opts);
+int bfly_plan_destroy(bfly_plan* p);
+/* Counts: passes = HBM sweeps over the state (tile passes + permutations +
+   dense blocks), launches = kernel launches, gates = lowered gate ops. */
+int bfly_plan_stats(const bfly_plan* p, int64_t* passes, int64_t* launches, int64_t* gates);
+/* Text description of the compiled schedule (one record per line; floats as
+   C99 hex literals, so the schedule can be replayed exactly).  Writes at most
+   `len` bytes including the terminating NUL and stores the full size
+   (including the NUL) in *needed.  No counterpart in the reference: the
+   schedule replaces its per-transformation dispatch. */
+int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed);
+/* Run a compiled plan on a state (asynchronous unless BFLY_FLAG_SYNC was set). */
+int bfly_plan_execute(bfly_plan* p, bfly_state* s);
+/* Plan + execute + destroy in one call. */
+int bfly_apply(bfly_state* s, const bfly_op* ops, uint64_t nops,
+               const double* udata, uint64_t udata_len, const bfly_exec_opts* opts);
+/* One-shot host strategy for an ExecutorFn: upload re/im, run ops, download
+   back into re/im in place (len = 2^n). */
+int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uint64_t len,
+                      const bfly_op* ops, uint64_t nops,
+                      const double* udata, uint64_t udata_len, const bfly_exec_opts* opts);
+
+/* ---- permutations: replaces bit_reverse_state / partial_bit_reverse_state
+        (state.mojo:131-204,252-301) and swap_bits_state (state.mojo:304-317) -- */
+int bfly_bitrev(bfly_state* s, uint64_t target_mask);
+int bfly_swap_qubits(bfly_state* s, int32_t a, int32_t b);
+
+/* ---- reductions: replaces probability_on_predicate / inner
+        (grover_classical.mojo:8-31,109-131) and verify_states_equal
+        (benchmark_verify.mojo:15-67); FP64 compensated, fixed order ---- */
+int bfly_norm2(bfly_state* s, double* out);                       /* sum |a|^2 */
+int bfly_prob_mask(bfly_state* s, uint64_t mask, uint64_t value, double* out); /* sum over (i & mask) == value */
+int bfly_marginal(bfly_state* s, const int32_t* targets, int32_t m, double* out); /* 2^m probabilities */
+int bfly_inner(bfly_state* a, bfly_state* b, double* re, double* im);          /* <a|b> */
+int bfly_diff_norm2(bfly_state* a, bfly_state* b, double* out);                /* sum |a-b|^2 */
+/* Multiply every amplitude by `scale` (renormalisation). */
+int bfly_scale(bfly_state* s, double scale);
+
+/* ---- profiling (BFLY_FLAG_PROFILE) -------------------------------------- */
+/* Per kernel family: launches and summed device milliseconds since reset.
+   `name` receives a static string.  Returns BFLY_EINVAL past the last family. */
+int bfly_profile_reset(void);
+int bfly_profile_get(int32_t index, const char** name, int64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BFLY_HIP_H */

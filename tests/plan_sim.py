@@ -1,0 +1,150 @@
+"""numpy interpreter of a dumped execution plan (test infrastructure).
+
+Replays the schedule that the planner compiled (tile passes with their tile
+qubits, sweeps, slots, local / outer control masks; stand-alone gates,
+bit reversals, swaps, dense blocks) on a complex128 state in host memory.
+Comparing its result with the oracle checks the planner's commutation-based
+reordering and its mask translation on the CPU, independently of the GPU
+arithmetic."""
+from __future__ import annotations
+
+import numpy as np
+
+I_GEN, I_H, I_X, I_RY, I_DIAG, I_PHASE1 = range(6)
+SQ_HALF = 0.7071067811865476
+
+
+def _parse_m(tok):
+    return [float.fromhex(x) for x in tok.split("=", 1)[1].split(",")]
+
+
+def _kv(parts):
+    out = {}
+    for p in parts:
+        if "=" in p:
+            k, v = p.split("=", 1)
+            out[k] = v
+    return out
+
+
+def parse(text: str):
+    lines = [ln for ln in text.strip().split("\n") if ln]
+    head = _kv(lines[0].split()[1:])
+    steps = []
+    cur_tile = None
+    cur_sweep = None
+    for ln in lines[1:]:
+        parts = ln.split()
+        kind = parts[0]
+        kv = _kv(parts[1:])
+        if kind == "tile":
+            cur_tile = {"kind": "tile", "k": int(kv["k"]), "qpos": [int(x) for x in kv["qpos"].split(",")],
+                        "sweeps": []}
+            steps.append(cur_tile)
+        elif kind == "sweep":
+            cur_sweep = {"s": [int(x) for x in kv["s"].split(",")], "instrs": []}
+            cur_tile["sweeps"].append(cur_sweep)
+        elif kind == "instr":
+            cur_sweep["instrs"].append({"kind": int(kv["kind"]), "slot": int(kv["slot"]), "lmask": int(kv["lmask"]),
+                                        "omask": int(kv["omask"]), "obit": int(kv["obit"]), "m": _parse_m(parts[-1])})
+        elif kind == "gate":
+            steps.append({"kind": "gate", "gkind": int(kv["kind"]), "target": int(kv["target"]),
+                          "cmask": int(kv["cmask"]), "m": _parse_m(parts[-1])})
+        elif kind == "bitrev":
+            steps.append({"kind": "bitrev", "mask": int(kv["mask"])})
+        elif kind == "swap":
+            steps.append({"kind": "swap", "a": int(kv["a"]), "b": int(kv["b"])})
+        elif kind == "unitary":
+            steps.append({"kind": "unitary", "target": int(kv["target"]), "m": int(kv["m"]),
+                          "controls": int(kv["controls"]), "uoff": int(kv["uoff"])})
+    return int(head["n"]), steps
+
+
+def _apply_pair(kind, m, u, v):
+    if kind == I_H:
+        return (u + v) * SQ_HALF, (u - v) * SQ_HALF
+    if kind == I_X:
+        return v, u
+    if kind == I_RY:
+        c, s = m[0], m[1]
+        return u * c - v * s, u * s + v * c
+    if kind == I_DIAG:
+        return u * complex(m[0], m[1]), v * complex(m[2], m[3])
+    if kind == I_PHASE1:
+        return u, v * complex(m[2], m[3])
+    g = [complex(m[0], m[1]), complex(m[2], m[3]), complex(m[4], m[5]), complex(m[6], m[7])]
+    return g[0] * u + g[1] * v, g[2] * u + g[3] * v
+
+
+def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
+    n, steps = parse(text)
+    psi = psi.astype(np.complex128).copy()
+    idx = np.arange(1 << n, dtype=np.int64)
+    for st in steps:
+        if st["kind"] == "tile":
+            k, qpos = st["k"], st["qpos"]
+            outer = [b for b in range(n) if b not in qpos]
+            ntiles = 1 << (n - k)
+            t = np.arange(ntiles, dtype=np.int64)
+            base = np.zeros(ntiles, dtype=np.int64)
+            for i, b in enumerate(outer):
+                base |= ((t >> i) & 1) << b
+            j = np.arange(1 << k, dtype=np.int64)
+            l2g = np.zeros(1 << k, dtype=np.int64)
+            for i, b in enumerate(qpos):
+                l2g |= ((j >> i) & 1) << b
+            G = base[:, None] + l2g[None, :]
+            A = psi[G]
+            for sw in st["sweeps"]:
+                for ins in sw["instrs"]:
+                    tile_ok = (base & ins["omask"]) == ins["omask"]
+                    lm = ins["lmask"]
+                    if ins["kind"] in (I_DIAG, I_PHASE1):
+                        m = ins["m"]
+                        cond = tile_ok[:, None] & ((j & lm) == lm)[None, :]
+                        if ins["slot"] >= 0:
+                            bit = ((j >> ins["slot"]) & 1)[None, :].astype(bool) & np.ones((ntiles, 1), bool)
+                        else:
+                            bit = ((base & ins["obit"]) != 0)[:, None] & np.ones((1, 1 << k), bool)
+                        d1 = complex(m[2], m[3])
+                        d0 = complex(m[0], m[1]) if ins["kind"] == I_DIAG else 1.0
+                        f = np.where(bit, d1, d0)
+                        A = np.where(cond, A * f, A)
+                    else:
+                        lb = sw["s"][ins["slot"]]
+                        j0 = j[((j >> lb) & 1) == 0]
+                        j1 = j0 | (1 << lb)
+                        cond = tile_ok[:, None] & ((j0 & lm) == lm)[None, :]
+                        u, v = A[:, j0], A[:, j1]
+                        nu, nv = _apply_pair(ins["kind"], ins["m"], u, v)
+                        A[:, j0] = np.where(cond, nu, u)
+                        A[:, j1] = np.where(cond, nv, v)
+            psi[G] = A
+        elif st["kind"] == "gate":
+            t = st["target"]
+            cm = st["cmask"]
+            i0 = idx[(((idx >> t) & 1) == 0) & ((idx & cm) == cm)]
+            i1 = i0 | (1 << t)
+            nu, nv = _apply_pair(st["gkind"], st["m"], psi[i0], psi[i1])
+            psi[i0], psi[i1] = nu, nv
+        elif st["kind"] == "bitrev":
+            mask = st["mask"] or ((1 << n) - 1)
+            pos = [b for b in range(n) if (mask >> b) & 1]
+            perm = idx & ~mask
+            for i, b in enumerate(pos):
+                perm |= ((idx >> b) & 1) << pos[len(pos) - 1 - i]
+            psi = psi[perm]
+        elif st["kind"] == "swap":
+            a, b = st["a"], st["b"]
+            x = ((idx >> a) & 1) ^ ((idx >> b) & 1)
+            psi = psi[idx ^ ((x << a) | (x << b))]
+        elif st["kind"] == "unitary":
+            m, t, cm = st["m"], st["target"], st["controls"]
+            dim = 1 << m
+            u = udata[2 * st["uoff"]: 2 * (st["uoff"] + dim * dim)]
+            U = (u[0::2] + 1j * u[1::2]).reshape(dim, dim)
+            block = ((1 << m) - 1) << t
+            bases = idx[((idx & block) == 0) & ((idx & cm) == cm)]
+            cols = bases[:, None] + (np.arange(dim)[None, :] << t)
+            psi[cols] = psi[cols] @ U.T
+    return psi

@@ -1,0 +1,126 @@
+"""CPU tests of the host planner inside libbfly_hip.so (no device calls):
+every compiled schedule, replayed by the numpy plan interpreter, must equal
+the oracle on the same inputs, and planning must terminate with sane counts."""
+import numpy as np
+import pytest
+
+from code_mojo_amd import _lib, algos, lower_circuit
+from code_mojo_amd import workloads as W
+from code_mojo_amd.state import Plan
+from oracle import oracle as O
+
+from .helpers import random_ops
+from .plan_sim import run_plan
+
+
+def oracle_run(n, ops, ud, psi):
+    re, im = psi.real.copy(), psi.imag.copy()
+    O.execute(n, re, im, ops, ud)
+    return re + 1j * im
+
+
+CASES = [(3, 0, 0), (4, 3, 0), (5, 4, 0), (5, 4, 1), (6, 0, 3), (7, 5, 2), (9, 6, 0), (10, 8, 4),
+         (12, 8, 0), (12, 12, 0), (13, 10, 6)]
+
+
+@pytest.mark.parametrize("n,tile_bits,low_bits", CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fused_schedule_matches_oracle(n, tile_bits, low_bits, dtype):
+    rng = np.random.default_rng(7 * n + tile_bits)
+    ops, ud = random_ops(rng, n, 80)
+    psi = W.normal_state(n, n).to_complex()
+    plan = Plan(n, dtype, ops, ud, _lib.opts(tile_bits=tile_bits, low_bits=low_bits))
+    got = run_plan(plan.dump(), psi, ud)
+    ref = oracle_run(n, ops, ud, psi)
+    assert np.abs(got - ref).max() < 1e-12
+    st = plan.stats()
+    assert 1 <= st["passes"] <= len(ops)
+
+
+@pytest.mark.parametrize("max_gates", [1, 3, 17])
+def test_gate_cap_respected(max_gates):
+    n = 10
+    rng = np.random.default_rng(max_gates)
+    ops, ud = random_ops(rng, n, 60, with_blocks=False, with_perms=False)
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(max_gates_per_pass=max_gates))
+    text = plan.dump()
+    for block in text.split("tile ")[1:]:
+        assert block.count("instr ") <= max_gates
+    psi = W.normal_state(n, 3).to_complex()
+    assert np.abs(run_plan(text, psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
+
+
+def test_unfused_schedule_matches_oracle():
+    n = 8
+    rng = np.random.default_rng(5)
+    ops, ud = random_ops(rng, n, 60)
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(mode=_lib.MODE_UNFUSED))
+    psi = W.normal_state(n, 4).to_complex()
+    assert np.abs(run_plan(plan.dump(), psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
+
+
+def test_config_circuits_pass_counts():
+    """QFT: H gates are the only non-diagonal ops, so a 12-qubit tile covers a
+    20-qubit QFT in ceil((20-12)/(12-4)) + 1 = 2 passes plus the bit reversal;
+    layered circuits need about three tile passes per layer at 28 qubits."""
+    ops, ud = lower_circuit(W.qft_circuit(20))
+    st = Plan(20, np.float64, ops, ud, _lib.opts()).stats()
+    assert st["passes"] == 3, st
+    ops, ud = lower_circuit(W.random_layered(28, 4, seed=1))
+    st = Plan(28, np.float64, ops, ud, _lib.opts()).stats()
+    assert st["passes"] <= 4 * 3 + 1, st
+    ops, ud = lower_circuit(algos.encode_value_circuit(30, 123456789, exact_angles=True, swap=True))
+    st = Plan(30, np.float64, ops, ud, _lib.opts()).stats()
+    assert st["passes"] <= 5, st
+
+
+def test_layered_schedule_matches_oracle():
+    n = 12
+    qc = W.random_layered(n, 6, seed=9)
+    ops, ud = lower_circuit(qc)
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1
+    for tb in (0, 6, 9):
+        plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
+        assert np.abs(run_plan(plan.dump(), psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
+
+
+def test_grover_and_ae_schedules():
+    n = 7
+    qc, marked = W.grover_search(n, 3)
+    ops, ud = lower_circuit(qc)
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=5))
+    got = run_plan(plan.dump(), psi, ud)
+    assert np.abs(got - oracle_run(n, ops, ud, psi)).max() < 1e-12
+    qc, a = W.amplitude_estimation(4)
+    ops, ud = lower_circuit(qc)
+    n = qc.num_qubits
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=6))
+    assert np.abs(run_plan(plan.dump(), psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
+
+
+def test_invalid_ops_rejected():
+    n = 4
+    ops, ud = lower_circuit(W.qft_circuit(n))
+    bad = ops.copy()
+    bad["target"][0] = 9
+    with pytest.raises(_lib.BflyError, match="Target out of bounds"):
+        Plan(n, np.float64, bad, ud, _lib.opts())
+    bad = ops.copy()
+    bad["controls"][1] = 1 << int(bad["target"][1])
+    with pytest.raises(_lib.BflyError, match="overlaps"):
+        Plan(n, np.float64, bad, ud, _lib.opts())
+    bad = ops.copy()
+    bad["kind"][0] = 42
+    with pytest.raises(_lib.BflyError):
+        Plan(n, np.float64, bad, ud, _lib.opts())
+    bad = ops.copy()
+    bad["mat"][0, 0] = np.nan
+    with pytest.raises(_lib.BflyError, match="non-finite"):
+        Plan(n, np.float64, bad, ud, _lib.opts())
+    with pytest.raises(_lib.BflyError, match="execution mode"):
+        Plan(n, np.float64, ops, ud, _lib.opts(mode=7))
