@@ -347,11 +347,13 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                 out += "\n";
                 for (int s = 0; s < tp.nsweeps; ++s) {
                     const TileSweep& sw = hp.sweeps[tp.first_sweep + s];
-                    put("sweep s=%d,%d,%d ninstr=%d\n", sw.s[0], sw.s[1], sw.s[2], sw.ninstr);
+                    put("sweep s=%d,%d,%d,%d ninstr=%d\n", sw.s[0], sw.s[1], sw.s[2], sw.s[3], sw.ninstr);
                     for (int q = 0; q < sw.ninstr; ++q) {
                         const TileInstr& in = hp.instrs[sw.first + q];
-                        put("instr kind=%d slot=%d lmask=%u omask=%llu obit=%llu", in.kind, in.slot, in.lmask,
-                            static_cast<unsigned long long>(in.omask), static_cast<unsigned long long>(in.obit));
+                        put("instr kind=%d slot=%d cslots=%u lmask=%u dlocal=%d mm1=%u mm0=%u omask=%llu obit=%llu",
+                            in.kind, in.slot, in.cslots, in.lmask, in.dlocal, static_cast<unsigned>(in.mm1),
+                            static_cast<unsigned>(in.mm0), static_cast<unsigned long long>(in.omask),
+                            static_cast<unsigned long long>(in.obit));
                         put_m(in.m);
                     }
                 }

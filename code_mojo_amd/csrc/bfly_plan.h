@@ -12,7 +12,7 @@ namespace bfly {
 
 // Largest tile (2^k amplitudes staged in LDS) and the widest state handled.
 constexpr int kMaxTileBits = 13;
-constexpr int kMinTileBits = 3;
+constexpr int kMinTileBits = 4;
 constexpr int kMaxQubits = 40;
 
 // Gate application kinds inside a tile sweep or a stand-alone gate launch.
@@ -25,30 +25,48 @@ enum InstrKind : int32_t {
     I_RY = 3,      // real rotation: m[0] = cos(t/2), m[1] = sin(t/2)
     I_DIAG = 4,    // a *= d0 (target bit 0) or d1 (target bit 1): m[0..1] = d0, m[2..3] = d1
     I_PHASE1 = 5,  // a *= d1 where target bit is 1 only (P, controlled P)
+    I_NEG1 = 6,    // a = -a where target bit is 1 (Z, CZ, multi-controlled Z): exact
 };
 
-// One gate inside a tile pass.  128 bytes.
+// Instruction kinds inside a tile sweep (register form of the gates above):
+// every pair gate is a 2x2 matrix on one slot; diagonal gates either flip
+// signs (Z family, exact) or multiply by a phase.
+enum TileKind : int32_t {
+    T_PAIR = 0,    // m[0..7]: 2x2 matrix on slot `slot`
+    T_NEG = 1,     // negate the members selected by mm1 (and the target bit)
+    T_DIAG = 2,    // multiply members selected by mm1 by d1 = m[2..3], by mm0 by d0 = m[0..1]
+};
+
+// Amplitudes per thread in a tile sweep: 2^kSlots, spanned by kSlots local bits.
+constexpr int kSlots = 4;
+constexpr int kGroup = 1 << kSlots;
+
+// One gate inside a tile pass.  144 bytes.
 struct alignas(16) TileInstr {
-    int32_t kind;      // InstrKind
-    int32_t slot;      // pair ops: index 0..2 of the target in the sweep's slot set;
-                       // diagonal ops: local bit of the target, or -1 when it is outside the tile
-    uint32_t lmask;    // control bits inside the tile, as local-index bits
-    uint32_t pad0;
+    int32_t kind;      // TileKind
+    int32_t slot;      // target's index in the sweep's slot set, or -1 (diagonal
+                       // ops only: target outside the slots)
+    uint32_t cslots;   // control bits that are slots, as a mask over slot indices
+    uint32_t lmask;    // control bits local to the tile but not slots (local-index bits)
+    int32_t dlocal;    // diagonal op with target local but not a slot: its local bit, else -1
+    uint16_t mm1;      // diagonal ops: members (of 16) whose slot controls hold and whose
+                       // target bit is 1 (every slot-control member when the target is no slot)
+    uint16_t mm0;      // diagonal ops: the same with target bit 0
     uint64_t omask;    // control bits outside the tile, as global-index bits
-    uint64_t obit;     // diagonal ops with target outside the tile: its global bit (else 0)
+    uint64_t obit;     // diagonal op with target outside the tile: its global bit (else 0)
     double m[8];       // matrix / parameters in FP64
     float mf[8];       // the same in FP32 (complex64 states)
 };
-static_assert(sizeof(TileInstr) == 128, "TileInstr layout");
+static_assert(sizeof(TileInstr) == 144, "TileInstr layout");
 
-// A sweep: every group of 8 amplitudes spanned by the three local bits
+// A sweep: every group of 16 amplitudes spanned by the kSlots local bits
 // `s` is loaded from LDS into registers, all the sweep's gates run in
 // registers, and the group is written back.  32 bytes.
 struct alignas(16) TileSweep {
-    int32_t s[3];      // local bit positions, distinct
+    int32_t s[kSlots]; // local bit positions, distinct
     int32_t ninstr;
     int32_t first;     // index of the first instruction
-    int32_t pad[3];
+    int32_t pad[2];
 };
 static_assert(sizeof(TileSweep) == 32, "TileSweep layout");
 

@@ -99,6 +99,7 @@ __device__ __forceinline__ void apply_pair_kind(int kind, T& ur, T& ui, T& vr, T
         case I_RY: pair_ry(ur, ui, vr, vi, g); break;
         case I_DIAG: cmul(ur, ui, g[0], g[1]); cmul(vr, vi, g[2], g[3]); break;
         case I_PHASE1: cmul(vr, vi, g[2], g[3]); break;
+        case I_NEG1: vr = -vr; vi = -vi; break;
         default: pair_gen(ur, ui, vr, vi, g); break;
     }
 }
@@ -114,10 +115,11 @@ __global__ __launch_bounds__(256) void gate_kernel(T* __restrict__ re, T* __rest
         uint64_t i0 = w;
         for (int p = 0; p < G.npos; ++p) i0 = insert_zero64(i0, G.pos[p]);
         i0 |= G.cmask;
-        if (G.kind == I_PHASE1) {
+        if (G.kind == I_PHASE1 || G.kind == I_NEG1) {
             const uint64_t i1 = i0 | tbit;
             T vr = re[i1], vi = im[i1];
-            cmul(vr, vi, g[2], g[3]);
+            if (G.kind == I_NEG1) { vr = -vr; vi = -vi; }
+            else cmul(vr, vi, g[2], g[3]);
             re[i1] = vr;
             im[i1] = vi;
         } else {
@@ -157,22 +159,79 @@ __device__ __forceinline__ uint64_t local_to_global(uint32_t j, const TilePass& 
     return g;
 }
 
+// A 2x2 matrix on slot SLOT of a 16-amplitude register group: pairs
+// (m0, m0 | 2^SLOT).  Pairs whose slot controls do not hold take the
+// identity; the test is on the member index only, so the selection is scalar
+// and no register array is copied.
 template <typename T, int SLOT>
-__device__ __forceinline__ void sweep_pairs(int kind, T* ar, T* ai, const uint32_t* jm, uint32_t lmask, const T* g) {
+__device__ __forceinline__ void slot_pairs(T* ar, T* ai, uint32_t cslots, const T* g) {
 #pragma unroll
-    for (int m0 = 0; m0 < 8; ++m0) {
-        if (m0 & (1 << SLOT)) continue;
+    for (int p = 0; p < kGroup / 2; ++p) {
+        constexpr int lo_mask = (1 << SLOT) - 1;
+        const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
         const int m1 = m0 | (1 << SLOT);
-        if ((jm[m0] & lmask) != lmask) continue;
-        apply_pair_kind<T>(kind, ar[m0], ai[m0], ar[m1], ai[m1], g);
+        const bool on = (static_cast<uint32_t>(m0) & cslots) == cslots;
+        T h[8];
+        h[0] = on ? g[0] : T(1);
+        h[1] = on ? g[1] : T(0);
+        h[2] = on ? g[2] : T(0);
+        h[3] = on ? g[3] : T(0);
+        h[4] = on ? g[4] : T(0);
+        h[5] = on ? g[5] : T(0);
+        h[6] = on ? g[6] : T(1);
+        h[7] = on ? g[7] : T(0);
+        pair_gen(ar[m0], ai[m0], ar[m1], ai[m1], h);
     }
 }
 
-template <typename T, int K, int NT>
-__global__ __launch_bounds__(NT) void tile_pass_kernel(T* __restrict__ re, T* __restrict__ im, const TilePass P,
-                                                       const TileSweep* __restrict__ sweeps,
-                                                       const TileInstr* __restrict__ instrs) {
+__device__ __forceinline__ void flip_sign(double& x, uint32_t on) {
+    uint64_t b = __double_as_longlong(x);
+    b ^= static_cast<uint64_t>(on) << 63;
+    x = __longlong_as_double(b);
+}
+__device__ __forceinline__ void flip_sign(float& x, uint32_t on) {
+    uint32_t b = __float_as_uint(x);
+    b ^= on << 31;
+    x = __uint_as_float(b);
+}
+
+// Z family: flip the sign bit of the members selected in sel1 (exactly the
+// product by -1).
+template <typename T>
+__device__ __forceinline__ void neg_apply(T* ar, T* ai, uint32_t sel1) {
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+        const uint32_t on = (sel1 >> m) & 1u;
+        flip_sign(ar[m], on);
+        flip_sign(ai[m], on);
+    }
+}
+
+// Phase family: members in sel1 are multiplied by d1, in sel0 by d0, the
+// others keep their value (factor 1 + 0i, exact).
+template <typename T>
+__device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t sel0, const T* g) {
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+        const bool s1 = (sel1 >> m) & 1u;
+        const bool s0 = (sel0 >> m) & 1u;
+        const T fr = s1 ? g[2] : (s0 ? g[0] : T(1));
+        const T fi = s1 ? g[3] : (s0 ? g[1] : T(0));
+        cmul(ar[m], ai[m], fr, fi);
+    }
+}
+
+// (b) One tile pass.  A tile is 2^K amplitudes: local bit i is global bit
+// P.qpos[i] (the low local bits are the low qubits, so VN consecutive local
+// indices are VN consecutive amplitudes), the other bits come from the tile
+// index.  NT = 2^(K-4) threads, one 16-amplitude group each per sweep.
+template <typename T, int K>
+__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64) void tile_pass_kernel(
+        T* __restrict__ re, T* __restrict__ im, const TilePass P, const TileSweep* __restrict__ sweeps,
+        const TileInstr* __restrict__ instrs) {
     constexpr uint32_t TS = 1u << K;
+    constexpr uint32_t NG = TS / kGroup;           // groups per tile
+    constexpr uint32_t NT = K >= 10 ? NG : 64;     // threads per block
     constexpr int VN = Vec<T>::N;
     using V = typename Vec<T>::type;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -184,8 +243,7 @@ __global__ __launch_bounds__(NT) void tile_pass_kernel(T* __restrict__ re, T* __
     uint64_t base = 0;
     for (int i = 0; i < P.nouter; ++i) base |= ((tile >> i) & 1ull) << P.opos[i];
 
-    // stage in: VN consecutive local indices map to VN consecutive amplitudes
-    // (the low bits of every tile are the low qubits), one 16-byte load each.
+    // stage in: one 16-byte vector per iteration, fully coalesced runs
     for (uint32_t e = tid; e < TS / VN; e += NT) {
         const uint32_t j = e * VN;
         const uint64_t gidx = base + local_to_global<K>(j, P);
@@ -203,51 +261,64 @@ __global__ __launch_bounds__(NT) void tile_pass_kernel(T* __restrict__ re, T* __
 
     for (int s = 0; s < P.nsweeps; ++s) {
         const TileSweep sw = sweeps[P.first_sweep + s];
-        const int s0 = sw.s[0], s1 = sw.s[1], s2 = sw.s[2];
-        const int lo = min(s0, min(s1, s2));
-        const int hi = max(s0, max(s1, s2));
-        const int mid = s0 + s1 + s2 - lo - hi;
-        uint32_t dep[8];
+        if (tid < NG) {
+            int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
+            // ascending slot positions for the zero-bit insertion
+            int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
 #pragma unroll
-        for (int m = 0; m < 8; ++m) dep[m] = ((m & 1u) << s0) | (((m >> 1) & 1u) << s1) | (((m >> 2) & 1u) << s2);
-        for (uint32_t gi = tid; gi < TS / 8; gi += NT) {
-            const uint32_t jb = insert_zero32(insert_zero32(insert_zero32(gi, lo), mid), hi);
-            uint32_t jm[8];
-            T ar[8], ai[8];
+            for (int a = 0; a < kSlots; ++a)
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                jm[m] = jb | dep[m];
-                ar[m] = s_re[swz<T>(jm[m])];
-                ai[m] = s_im[swz<T>(jm[m])];
+                for (int b2 = a + 1; b2 < kSlots; ++b2)
+                    if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
+            uint32_t jb = tid;
+#pragma unroll
+            for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
+            // member m of the group sits at local index jb | dep(m); dep is
+            // uniform, so it lives in scalar registers
+            auto dep = [&](int m) {
+                uint32_t d = 0;
+#pragma unroll
+                for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
+                return d;
+            };
+            T ar[kGroup], ai[kGroup];
+#pragma unroll
+            for (int m = 0; m < kGroup; ++m) {
+                const uint32_t jm = swz<T>(jb | dep(m));
+                ar[m] = s_re[jm];
+                ai[m] = s_im[jm];
             }
             for (int q = 0; q < sw.ninstr; ++q) {
                 const TileInstr& in = instrs[sw.first + q];
                 if ((base & in.omask) != in.omask) continue;
+                if ((jb & in.lmask) != in.lmask) continue;
                 const int kind = in.kind;
                 T g[8];
                 load_params<T>(in.m, in.mf, g);
-                if (kind == I_DIAG || kind == I_PHASE1) {
-                    const int lb = in.slot;
-                    const uint32_t obit = (base & in.obit) ? 1u : 0u;
-#pragma unroll
-                    for (int m = 0; m < 8; ++m) {
-                        if ((jm[m] & in.lmask) != in.lmask) continue;
-                        const uint32_t bit = lb >= 0 ? ((jm[m] >> lb) & 1u) : obit;
-                        if (bit) cmul(ar[m], ai[m], g[2], g[3]);
-                        else if (kind == I_DIAG) cmul(ar[m], ai[m], g[0], g[1]);
+                if (kind == T_PAIR) {
+                    switch (in.slot) {
+                        case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
+                        case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
+                        case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
+                        default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
                     }
-                } else if (in.slot == 0) {
-                    sweep_pairs<T, 0>(kind, ar, ai, jm, in.lmask, g);
-                } else if (in.slot == 1) {
-                    sweep_pairs<T, 1>(kind, ar, ai, jm, in.lmask, g);
                 } else {
-                    sweep_pairs<T, 2>(kind, ar, ai, jm, in.lmask, g);
+                    uint32_t sel1 = in.mm1, sel0 = in.mm0;
+                    if (in.slot < 0) {
+                        // target outside the slots: one bit per thread (local) or per tile (outer)
+                        const uint32_t bit = in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
+                        sel1 = bit ? sel1 : 0u;
+                        sel0 = bit ? 0u : sel0;
+                    }
+                    if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
+                    else diag_apply<T>(ar, ai, sel1, sel0, g);
                 }
             }
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                s_re[swz<T>(jm[m])] = ar[m];
-                s_im[swz<T>(jm[m])] = ai[m];
+            for (int m = 0; m < kGroup; ++m) {
+                const uint32_t jm = swz<T>(jb | dep(m));
+                s_re[jm] = ar[m];
+                s_im[jm] = ai[m];
             }
         }
         __syncthreads();
@@ -270,7 +341,7 @@ __global__ __launch_bounds__(NT) void tile_pass_kernel(T* __restrict__ re, T* __
     }
 }
 
-constexpr int tile_threads(int k) { return k >= 11 ? 256 : (k >= 10 ? 128 : 64); }
+constexpr int tile_threads(int k) { return k >= 10 ? (1 << (k - 4)) : 64; }
 
 template <typename T, int K>
 hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
@@ -278,21 +349,20 @@ hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, c
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_pass_kernel<T, K, NT>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_pass_kernel<T, K>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const uint64_t ntiles = 1ull << (P.n - P.k);
-    hipLaunchKernelGGL((tile_pass_kernel<T, K, NT>), dim3(static_cast<unsigned>(ntiles)), dim3(NT), lds, st, re, im,
-                       P, sw, in);
+    hipLaunchKernelGGL((tile_pass_kernel<T, K>), dim3(static_cast<unsigned>(ntiles)), dim3(NT), lds, st, re, im, P,
+                       sw, in);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_tile_t(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
     switch (P.k) {
-        case 3: return launch_tile_k<T, 3>(re, im, P, sw, in, st);
         case 4: return launch_tile_k<T, 4>(re, im, P, sw, in, st);
         case 5: return launch_tile_k<T, 5>(re, im, P, sw, in, st);
         case 6: return launch_tile_k<T, 6>(re, im, P, sw, in, st);

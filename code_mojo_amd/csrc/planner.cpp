@@ -42,7 +42,9 @@ Classified classify(const bfly_op& op) {
     const bool mat_diag = g[2] == 0.0 && g[3] == 0.0 && g[4] == 0.0 && g[5] == 0.0;
     auto set_diag = [&](double d0r, double d0i, double d1r, double d1i) {
         c.diag = true;
-        if (d0r == 1.0 && d0i == 0.0) {
+        if (d0r == 1.0 && d0i == 0.0 && d1r == -1.0 && d1i == 0.0) {
+            c.kind = I_NEG1;  // same values as the complex product by -1, exactly
+        } else if (d0r == 1.0 && d0i == 0.0) {
             c.kind = I_PHASE1;
         } else {
             c.kind = I_DIAG;
@@ -72,6 +74,24 @@ Classified classify(const bfly_op& op) {
 
 void fill_floats(const double* m, float* mf) {
     for (int i = 0; i < 8; ++i) mf[i] = static_cast<float>(m[i]);
+}
+
+// Register form of a classified gate for a tile sweep.  Pair kinds become
+// their 2x2 matrix (H with the reference's 0.7071067811865476 entries, X as
+// the exchange matrix, RY from cos / sin of theta/2); diagonal kinds keep
+// d0 = m[0..1], d1 = m[2..3].
+void pair_matrix(const Classified& c, double* out) {
+    const double s = 0.7071067811865476;
+    switch (c.kind) {
+        case I_H: { const double h[8] = {s, 0, s, 0, s, 0, -s, 0}; std::memcpy(out, h, sizeof h); return; }
+        case I_X: { const double x[8] = {0, 0, 1, 0, 1, 0, 0, 0}; std::memcpy(out, x, sizeof x); return; }
+        case I_RY: {
+            const double r[8] = {c.m[0], 0, -c.m[1], 0, c.m[1], 0, c.m[0], 0};
+            std::memcpy(out, r, sizeof r);
+            return;
+        }
+        default: std::memcpy(out, c.m, 8 * sizeof(double)); return;
+    }
 }
 
 uint64_t op_qubits(const bfly_op& op, int n) {
@@ -152,6 +172,8 @@ int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_le
 namespace {
 
 // Pack the gates of one pass into register sweeps and append the records.
+// A sweep owns up to kSlots local bits; its gates' control masks are split
+// into slot bits / other local bits / outer bits once the slot set is final.
 void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std::vector<PendingGate>& gates) {
     Step st{};
     st.kind = S_TILE;
@@ -167,49 +189,64 @@ void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std:
     tp.nouter = oi;
     tp.first_sweep = static_cast<int32_t>(plan.sweeps.size());
 
-    TileSweep cur{};
+    int slots[kSlots];
     int nslots = 0;
-    cur.first = static_cast<int32_t>(plan.instrs.size());
+    std::vector<const PendingGate*> cur;
+    auto slot_of = [&](int lb) {
+        for (int s = 0; s < nslots; ++s)
+            if (slots[s] == lb) return s;
+        return -1;
+    };
     auto close = [&]() {
-        // complete the slot set with the lowest unused local bits
-        for (int b = 0; nslots < 3 && b < k; ++b) {
-            bool used = false;
-            for (int s = 0; s < nslots; ++s) used |= cur.s[s] == b;
-            if (!used) cur.s[nslots++] = b;
+        for (int b = 0; nslots < kSlots && b < k; ++b)
+            if (slot_of(b) < 0) slots[nslots++] = b;
+        TileSweep sw{};
+        for (int s = 0; s < kSlots; ++s) sw.s[s] = slots[s];
+        sw.first = static_cast<int32_t>(plan.instrs.size());
+        for (const PendingGate* pg : cur) {
+            const bfly_op& op = *pg->op;
+            TileInstr in{};
+            const int ik = pg->c.kind;
+            in.kind = ik == I_NEG1 ? T_NEG : (pg->c.diag ? T_DIAG : T_PAIR);
+            pair_matrix(pg->c, in.m);
+            fill_floats(in.m, in.mf);
+            in.dlocal = -1;
+            for (int b = 0; b < n; ++b) {
+                if (!((op.controls >> b) & 1)) continue;
+                const int lb = local_of[b];
+                if (lb < 0) in.omask |= 1ull << b;
+                else if (slot_of(lb) >= 0) in.cslots |= 1u << slot_of(lb);
+                else in.lmask |= 1u << lb;
+            }
+            const int lt = local_of[op.target];
+            in.slot = lt >= 0 ? slot_of(lt) : -1;
+            if (pg->c.diag) {
+                if (lt < 0) in.obit = 1ull << op.target;
+                else if (in.slot < 0) in.dlocal = lt;
+                for (uint32_t m = 0; m < static_cast<uint32_t>(kGroup); ++m) {
+                    if ((m & in.cslots) != in.cslots) continue;
+                    if (in.slot < 0 || ((m >> in.slot) & 1u)) in.mm1 |= static_cast<uint16_t>(1u << m);
+                    if (in.slot < 0 || !((m >> in.slot) & 1u)) in.mm0 |= static_cast<uint16_t>(1u << m);
+                }
+                // only I_DIAG touches target-bit-0 members (P / controlled P / Z leave them as is)
+                if (ik != I_DIAG) in.mm0 = 0;
+            }
+            plan.instrs.push_back(in);
         }
-        cur.ninstr = static_cast<int32_t>(plan.instrs.size()) - cur.first;
-        plan.sweeps.push_back(cur);
-        cur = TileSweep{};
+        sw.ninstr = static_cast<int32_t>(plan.instrs.size()) - sw.first;
+        plan.sweeps.push_back(sw);
+        cur.clear();
         nslots = 0;
-        cur.first = static_cast<int32_t>(plan.instrs.size());
     };
     for (const PendingGate& pg : gates) {
-        const bfly_op& op = *pg.op;
-        TileInstr in{};
-        in.kind = pg.c.kind;
-        std::memcpy(in.m, pg.c.m, sizeof in.m);
-        fill_floats(in.m, in.mf);
-        for (int b = 0; b < n; ++b) {
-            if (!((op.controls >> b) & 1)) continue;
-            if (local_of[b] >= 0) in.lmask |= 1u << local_of[b];
-            else in.omask |= 1ull << b;
-        }
-        const int lt = local_of[op.target];
-        if (pg.c.diag) {
-            in.slot = lt;  // -1 when outside the tile
-            in.obit = lt < 0 ? (1ull << op.target) : 0;
-        } else {
-            int slot = -1;
-            for (int s = 0; s < nslots; ++s)
-                if (cur.s[s] == lt) slot = s;
-            if (slot < 0) {
-                if (nslots == 3) close();
-                cur.s[nslots] = lt;
-                slot = nslots++;
+        if (!pg.c.diag) {
+            const int lt = local_of[pg.op->target];
+            if (slot_of(lt) < 0) {
+                if (nslots == kSlots) close();
+                slots[nslots++] = lt;
             }
-            in.slot = slot;
         }
-        plan.instrs.push_back(in);
+        cur.push_back(&pg);
     }
     close();
     tp.nsweeps = static_cast<int32_t>(plan.sweeps.size()) - tp.first_sweep;

@@ -10,7 +10,8 @@ from __future__ import annotations
 
 import numpy as np
 
-I_GEN, I_H, I_X, I_RY, I_DIAG, I_PHASE1 = range(6)
+I_GEN, I_H, I_X, I_RY, I_DIAG, I_PHASE1, I_NEG1 = range(7)
+T_PAIR, T_NEG, T_DIAG = range(3)
 SQ_HALF = 0.7071067811865476
 
 
@@ -45,7 +46,9 @@ def parse(text: str):
             cur_sweep = {"s": [int(x) for x in kv["s"].split(",")], "instrs": []}
             cur_tile["sweeps"].append(cur_sweep)
         elif kind == "instr":
-            cur_sweep["instrs"].append({"kind": int(kv["kind"]), "slot": int(kv["slot"]), "lmask": int(kv["lmask"]),
+            cur_sweep["instrs"].append({"kind": int(kv["kind"]), "slot": int(kv["slot"]), "cslots": int(kv["cslots"]),
+                                        "lmask": int(kv["lmask"]), "dlocal": int(kv["dlocal"]),
+                                        "mm1": int(kv["mm1"]), "mm0": int(kv["mm0"]),
                                         "omask": int(kv["omask"]), "obit": int(kv["obit"]), "m": _parse_m(parts[-1])})
         elif kind == "gate":
             steps.append({"kind": "gate", "gkind": int(kv["kind"]), "target": int(kv["target"]),
@@ -72,6 +75,8 @@ def _apply_pair(kind, m, u, v):
         return u * complex(m[0], m[1]), v * complex(m[2], m[3])
     if kind == I_PHASE1:
         return u, v * complex(m[2], m[3])
+    if kind == I_NEG1:
+        return u, -v
     g = [complex(m[0], m[1]), complex(m[2], m[3]), complex(m[4], m[5]), complex(m[6], m[7])]
     return g[0] * u + g[1] * v, g[2] * u + g[3] * v
 
@@ -96,29 +101,46 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
             G = base[:, None] + l2g[None, :]
             A = psi[G]
             for sw in st["sweeps"]:
+                assert len(set(sw["s"])) == 4 and all(0 <= x < k for x in sw["s"])
                 for ins in sw["instrs"]:
                     tile_ok = (base & ins["omask"]) == ins["omask"]
                     lm = ins["lmask"]
-                    if ins["kind"] in (I_DIAG, I_PHASE1):
-                        m = ins["m"]
-                        cond = tile_ok[:, None] & ((j & lm) == lm)[None, :]
-                        if ins["slot"] >= 0:
-                            bit = ((j >> ins["slot"]) & 1)[None, :].astype(bool) & np.ones((ntiles, 1), bool)
-                        else:
-                            bit = ((base & ins["obit"]) != 0)[:, None] & np.ones((1, 1 << k), bool)
-                        d1 = complex(m[2], m[3])
-                        d0 = complex(m[0], m[1]) if ins["kind"] == I_DIAG else 1.0
-                        f = np.where(bit, d1, d0)
-                        A = np.where(cond, A * f, A)
-                    else:
+                    lane_ok = (j & lm) == lm
+                    # member index of every local index within its 16-group
+                    member = np.zeros(1 << k, dtype=np.int64)
+                    for si in range(4):
+                        member |= ((j >> sw["s"][si]) & 1) << si
+                    m = ins["m"]
+                    if ins["kind"] == T_PAIR:
+                        cs = ins["cslots"]
                         lb = sw["s"][ins["slot"]]
                         j0 = j[((j >> lb) & 1) == 0]
                         j1 = j0 | (1 << lb)
-                        cond = tile_ok[:, None] & ((j0 & lm) == lm)[None, :]
+                        on = ((member[j0] & cs) == cs) & lane_ok[j0]
+                        cond = tile_ok[:, None] & on[None, :]
                         u, v = A[:, j0], A[:, j1]
-                        nu, nv = _apply_pair(ins["kind"], ins["m"], u, v)
+                        nu, nv = _apply_pair(I_GEN, m, u, v)
                         A[:, j0] = np.where(cond, nu, u)
                         A[:, j1] = np.where(cond, nv, v)
+                    else:
+                        sel1 = (ins["mm1"] >> member) & 1
+                        sel0 = (ins["mm0"] >> member) & 1
+                        if ins["slot"] < 0:
+                            if ins["dlocal"] >= 0:
+                                bit = ((j >> ins["dlocal"]) & 1)[None, :] * np.ones((ntiles, 1), np.int64)
+                            else:
+                                bit = ((base & ins["obit"]) != 0).astype(np.int64)[:, None] * np.ones((1, 1 << k), np.int64)
+                            s1 = sel1[None, :] * bit
+                            s0 = sel0[None, :] * (1 - bit)
+                        else:
+                            s1 = sel1[None, :] * np.ones((ntiles, 1), np.int64)
+                            s0 = sel0[None, :] * np.ones((ntiles, 1), np.int64)
+                        active = tile_ok[:, None] & lane_ok[None, :]
+                        if ins["kind"] == T_NEG:
+                            f = np.where(s1 == 1, -1.0, 1.0)
+                        else:
+                            f = np.where(s1 == 1, complex(m[2], m[3]), np.where(s0 == 1, complex(m[0], m[1]), 1.0))
+                        A = np.where(active, A * f, A)
             psi[G] = A
         elif st["kind"] == "gate":
             t = st["target"]
