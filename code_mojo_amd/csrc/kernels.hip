@@ -164,7 +164,21 @@ __device__ __forceinline__ uint64_t local_to_global(uint32_t j, const TilePass& 
 // identity; the test is on the member index only, so the selection is scalar
 // and no register array is copied.
 template <typename T, int SLOT>
+__device__ __forceinline__ void slot_pairs_all(T* ar, T* ai, const T* g) {
+#pragma unroll
+    for (int p = 0; p < kGroup / 2; ++p) {
+        constexpr int lo_mask = (1 << SLOT) - 1;
+        const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
+        pair_gen(ar[m0], ai[m0], ar[m0 | (1 << SLOT)], ai[m0 | (1 << SLOT)], g);
+    }
+}
+
+template <typename T, int SLOT>
 __device__ __forceinline__ void slot_pairs(T* ar, T* ai, uint32_t cslots, const T* g) {
+    if (cslots == 0) {  // no slot controls: every pair, no selection
+        slot_pairs_all<T, SLOT>(ar, ai, g);
+        return;
+    }
 #pragma unroll
     for (int p = 0; p < kGroup / 2; ++p) {
         constexpr int lo_mask = (1 << SLOT) - 1;
@@ -225,119 +239,161 @@ __device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t
 // P.qpos[i] (the low local bits are the low qubits, so VN consecutive local
 // indices are VN consecutive amplitudes), the other bits come from the tile
 // index.  NT = 2^(K-4) threads, one 16-amplitude group each per sweep.
+// Blocks are persistent: block b handles tiles b, b + gridDim.x, ...; the
+// next tile's HBM loads are issued into registers before the current tile's
+// sweeps and land in LDS only after its stage-out, so HBM traffic overlaps
+// the arithmetic.
 template <typename T, int K>
-__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64) void tile_pass_kernel(
+__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_kernel(
         T* __restrict__ re, T* __restrict__ im, const TilePass P, const TileSweep* __restrict__ sweeps,
         const TileInstr* __restrict__ instrs) {
     constexpr uint32_t TS = 1u << K;
     constexpr uint32_t NG = TS / kGroup;           // groups per tile
     constexpr uint32_t NT = K >= 10 ? NG : 64;     // threads per block
     constexpr int VN = Vec<T>::N;
+    constexpr uint32_t NV = TS / VN;               // 16-byte vectors per tile (re or im)
+    constexpr uint32_t PV = (NV + NT - 1) / NT;    // vectors per thread
     using V = typename Vec<T>::type;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     T* s_re = reinterpret_cast<T*>(lds_raw);
     T* s_im = s_re + TS;
     const uint32_t tid = threadIdx.x;
-    const uint64_t tile = blockIdx.x;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    uint64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
 
-    uint64_t base = 0;
-    for (int i = 0; i < P.nouter; ++i) base |= ((tile >> i) & 1ull) << P.opos[i];
-
-    // stage in: one 16-byte vector per iteration, fully coalesced runs
-    for (uint32_t e = tid; e < TS / VN; e += NT) {
-        const uint32_t j = e * VN;
-        const uint64_t gidx = base + local_to_global<K>(j, P);
-        const V vr = *reinterpret_cast<const V*>(re + gidx);
-        const V vi = *reinterpret_cast<const V*>(im + gidx);
-        const T* pr = reinterpret_cast<const T*>(&vr);
-        const T* pi = reinterpret_cast<const T*>(&vi);
+    auto tile_base = [&](uint64_t t) {
+        uint64_t b = 0;
+        for (int i = 0; i < P.nouter; ++i) b |= ((t >> i) & 1ull) << P.opos[i];
+        return b;
+    };
+    // local index of this thread's c-th vector is (tid + c*NT)*VN; its global
+    // offset splits into a per-thread part and a uniform part (disjoint bits)
+    const uint64_t off_t = local_to_global<K>(tid * VN, P);
+    V pr[PV], pi[PV];
+    auto load_tile = [&](uint64_t b) {
 #pragma unroll
-        for (int v = 0; v < VN; ++v) {
-            s_re[swz<T>(j + v)] = pr[v];
-            s_im[swz<T>(j + v)] = pi[v];
+        for (uint32_t c = 0; c < PV; ++c) {
+            if (tid + c * NT < NV) {
+                const uint64_t g = b + off_t + local_to_global<K>(c * NT * VN, P);
+                pr[c] = *reinterpret_cast<const V*>(re + g);
+                pi[c] = *reinterpret_cast<const V*>(im + g);
+            }
         }
-    }
-    __syncthreads();
+    };
+    uint64_t base = tile_base(tile);
+    load_tile(base);
 
-    for (int s = 0; s < P.nsweeps; ++s) {
-        const TileSweep sw = sweeps[P.first_sweep + s];
-        if (tid < NG) {
-            int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
-            // ascending slot positions for the zero-bit insertion
-            int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
+    for (;;) {
+        // registers -> LDS
 #pragma unroll
-            for (int a = 0; a < kSlots; ++a)
+        for (uint32_t c = 0; c < PV; ++c) {
+            if (tid + c * NT < NV) {
+                const uint32_t j = (tid + c * NT) * VN;
+                const T* a = reinterpret_cast<const T*>(&pr[c]);
+                const T* b = reinterpret_cast<const T*>(&pi[c]);
 #pragma unroll
-                for (int b2 = a + 1; b2 < kSlots; ++b2)
-                    if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
-            uint32_t jb = tid;
-#pragma unroll
-            for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
-            // member m of the group sits at local index jb | dep(m); dep is
-            // uniform, so it lives in scalar registers
-            auto dep = [&](int m) {
-                uint32_t d = 0;
-#pragma unroll
-                for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
-                return d;
-            };
-            T ar[kGroup], ai[kGroup];
-#pragma unroll
-            for (int m = 0; m < kGroup; ++m) {
-                const uint32_t jm = swz<T>(jb | dep(m));
-                ar[m] = s_re[jm];
-                ai[m] = s_im[jm];
-            }
-            for (int q = 0; q < sw.ninstr; ++q) {
-                const TileInstr& in = instrs[sw.first + q];
-                if ((base & in.omask) != in.omask) continue;
-                if ((jb & in.lmask) != in.lmask) continue;
-                const int kind = in.kind;
-                T g[8];
-                load_params<T>(in.m, in.mf, g);
-                if (kind == T_PAIR) {
-                    switch (in.slot) {
-                        case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
-                        case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
-                        case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
-                        default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
-                    }
-                } else {
-                    uint32_t sel1 = in.mm1, sel0 = in.mm0;
-                    if (in.slot < 0) {
-                        // target outside the slots: one bit per thread (local) or per tile (outer)
-                        const uint32_t bit = in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
-                        sel1 = bit ? sel1 : 0u;
-                        sel0 = bit ? 0u : sel0;
-                    }
-                    if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
-                    else diag_apply<T>(ar, ai, sel1, sel0, g);
+                for (int v = 0; v < VN; ++v) {
+                    s_re[swz<T>(j + v)] = a[v];
+                    s_im[swz<T>(j + v)] = b[v];
                 }
-            }
-#pragma unroll
-            for (int m = 0; m < kGroup; ++m) {
-                const uint32_t jm = swz<T>(jb | dep(m));
-                s_re[jm] = ar[m];
-                s_im[jm] = ai[m];
             }
         }
         __syncthreads();
-    }
-
-    // stage out
-    for (uint32_t e = tid; e < TS / VN; e += NT) {
-        const uint32_t j = e * VN;
-        const uint64_t gidx = base + local_to_global<K>(j, P);
-        V vr, vi;
-        T* pr = reinterpret_cast<T*>(&vr);
-        T* pi = reinterpret_cast<T*>(&vi);
-#pragma unroll
-        for (int v = 0; v < VN; ++v) {
-            pr[v] = s_re[swz<T>(j + v)];
-            pi[v] = s_im[swz<T>(j + v)];
+        const uint64_t next = tile + gridDim.x;
+        uint64_t nbase = 0;
+        if (next < ntiles) {
+            nbase = tile_base(next);
+            load_tile(nbase);  // in flight during the sweeps
         }
-        *reinterpret_cast<V*>(re + gidx) = vr;
-        *reinterpret_cast<V*>(im + gidx) = vi;
+
+        for (int s = 0; s < P.nsweeps; ++s) {
+            const TileSweep sw = sweeps[P.first_sweep + s];
+            if (tid < NG) {
+                int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
+                int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
+#pragma unroll
+                for (int a = 0; a < kSlots; ++a)
+#pragma unroll
+                    for (int b2 = a + 1; b2 < kSlots; ++b2)
+                        if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
+                uint32_t jb = tid;
+#pragma unroll
+                for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
+                // member m of the group sits at local index jb | dep(m); dep is
+                // uniform, so it lives in scalar registers
+                auto dep = [&](int m) {
+                    uint32_t d = 0;
+#pragma unroll
+                    for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
+                    return d;
+                };
+                T ar[kGroup], ai[kGroup];
+#pragma unroll
+                for (int m = 0; m < kGroup; ++m) {
+                    const uint32_t jm = swz<T>(jb | dep(m));
+                    ar[m] = s_re[jm];
+                    ai[m] = s_im[jm];
+                }
+                for (int q = 0; q < sw.ninstr; ++q) {
+                    const TileInstr& in = instrs[sw.first + q];
+                    if ((base & in.omask) != in.omask) continue;
+                    if ((jb & in.lmask) != in.lmask) continue;
+                    const int kind = in.kind;
+                    T g[8];
+                    load_params<T>(in.m, in.mf, g);
+                    if (kind == T_PAIR) {
+                        switch (in.slot) {
+                            case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
+                            case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
+                            case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
+                            default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
+                        }
+                    } else {
+                        uint32_t sel1 = in.mm1, sel0 = in.mm0;
+                        if (in.slot < 0) {
+                            // target outside the slots: one bit per thread (local) or per tile (outer)
+                            const uint32_t bit =
+                                in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
+                            sel1 = bit ? sel1 : 0u;
+                            sel0 = bit ? 0u : sel0;
+                        }
+                        if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
+                        else diag_apply<T>(ar, ai, sel1, sel0, g);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < kGroup; ++m) {
+                    const uint32_t jm = swz<T>(jb | dep(m));
+                    s_re[jm] = ar[m];
+                    s_im[jm] = ai[m];
+                }
+            }
+            __syncthreads();
+        }
+
+        // stage out: LDS -> HBM with 16-byte stores
+#pragma unroll
+        for (uint32_t c = 0; c < PV; ++c) {
+            if (tid + c * NT < NV) {
+                const uint32_t j = (tid + c * NT) * VN;
+                V vr, vi;
+                T* a = reinterpret_cast<T*>(&vr);
+                T* b = reinterpret_cast<T*>(&vi);
+#pragma unroll
+                for (int v = 0; v < VN; ++v) {
+                    a[v] = s_re[swz<T>(j + v)];
+                    b[v] = s_im[swz<T>(j + v)];
+                }
+                const uint64_t g = base + off_t + local_to_global<K>(c * NT * VN, P);
+                *reinterpret_cast<V*>(re + g) = vr;
+                *reinterpret_cast<V*>(im + g) = vi;
+            }
+        }
+        if (next >= ntiles) break;
+        __syncthreads();  // every stage-out read is done before the next tile lands in LDS
+        tile = next;
+        base = nbase;
     }
 }
 
@@ -347,16 +403,24 @@ template <typename T, int K>
 hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
     constexpr int NT = tile_threads(K);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
-    static bool attr_set = false;
-    if (!attr_set) {
+    // persistent grid: as many blocks as fit on the device at once
+    static int grid_cap = 0;
+    if (grid_cap == 0) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_pass_kernel<T, K>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
         if (e != hipSuccess) return e;
-        attr_set = true;
+        int per_cu = 0, dev = 0, cus = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tile_pass_kernel<T, K>, NT, lds);
+        if (e != hipSuccess) return e;
+        e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        grid_cap = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
     }
     const uint64_t ntiles = 1ull << (P.n - P.k);
-    hipLaunchKernelGGL((tile_pass_kernel<T, K>), dim3(static_cast<unsigned>(ntiles)), dim3(NT), lds, st, re, im, P,
-                       sw, in);
+    const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
+    hipLaunchKernelGGL((tile_pass_kernel<T, K>), dim3(grid), dim3(NT), lds, st, re, im, P, sw, in);
     return hipGetLastError();
 }
 

@@ -231,6 +231,16 @@ void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std:
                 // only I_DIAG touches target-bit-0 members (P / controlled P / Z leave them as is)
                 if (ik != I_DIAG) in.mm0 = 0;
             }
+            // Sign flips commute and compose by XOR: a Z-family gate whose
+            // selection depends on slot bits only merges into a preceding one.
+            const bool uniform_neg = in.kind == T_NEG && in.slot >= 0 && in.lmask == 0 && in.omask == 0;
+            if (uniform_neg && static_cast<int32_t>(plan.instrs.size()) > sw.first) {
+                TileInstr& prev = plan.instrs.back();
+                if (prev.kind == T_NEG && prev.slot >= 0 && prev.lmask == 0 && prev.omask == 0) {
+                    prev.mm1 = static_cast<uint16_t>(prev.mm1 ^ in.mm1);
+                    continue;
+                }
+            }
             plan.instrs.push_back(in);
         }
         sw.ninstr = static_cast<int32_t>(plan.instrs.size()) - sw.first;
