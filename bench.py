@@ -93,7 +93,18 @@ def main() -> int:
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
+                    help="only copy one N-qubit complex128 state device to device (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        from code_mojo_amd.state import GpuState
+        a, b = GpuState(args.copy_probe), GpuState(args.copy_probe)
+        b.copy_from(a)
+        b.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 2 * (1 << args.copy_probe) * 8 * 2,
+                          "copy_probe_bytes_written": 2 * (1 << args.copy_probe) * 8 * 2}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the statevector strategy runs on a single device "

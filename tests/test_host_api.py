@@ -1,0 +1,183 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+symbol include/bfly_hip.h declares (no compute calls), the Python mirror of
+the reference API (circuit builders, inverse, registers, config, strategy
+registry) and the lowering to the op array."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from code_mojo_amd import (Circuit, ExecContext, ExecutionStrategy, LoweringError, QuantumCircuit, Register,
+                           _lib, algos, lower_circuit)
+from code_mojo_amd.circuit import (MeasurementTransformation, bit_reverse, measure, permute_qubits, swap_tr)
+from code_mojo_amd.context import execution_strategy_name, load_config
+from code_mojo_amd.executors import get_executor
+from code_mojo_amd.lowering import OP_BITREV, OP_DTYPE, OP_GATE, OP_SWAP, OP_UNITARY
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "bfly_hip.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\**(bfly_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = header_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.EXPORTED)
+
+
+def test_op_layout_matches_header():
+    assert OP_DTYPE.itemsize == 104
+    assert [OP_DTYPE.fields[f][1] for f in ("kind", "gate", "target", "arg_i", "controls", "aux", "theta", "mat")] == \
+        [0, 4, 8, 12, 16, 24, 32, 40]
+
+
+def test_version_and_errors_without_gpu_calls():
+    lib = _lib.load()
+    assert lib.bfly_version() >= 100
+    assert lib.bfly_last_error() is not None
+
+
+def test_strategy_registry():
+    assert ExecutionStrategy.GPU == 8
+    assert execution_strategy_name(8) == "GPU"
+    assert callable(get_executor(ExecutionStrategy.GPU))
+    with pytest.raises(ValueError, match="Unknown execution strategy"):
+        get_executor(ExecutionStrategy.SCALAR)
+    with pytest.raises(ValueError):
+        ExecutionStrategy.from_int(9)
+
+
+def test_config_file(tmp_path):
+    cfg = tmp_path / "butterfly.conf"
+    cfg.write_text("execution_strategy=8\n# comment\ngpu_mode = unfused\ngpu_tile_bits=11\nvalidate_circuit=false\n")
+    ctx = ExecContext.from_config(load_config(str(cfg)))
+    assert ctx.execution_strategy == 8 and ctx.gpu_mode == "unfused" and ctx.gpu_tile_bits == 11
+    assert ctx.validate_circuit is False
+
+
+def test_lowering_kinds_and_masks():
+    qc = QuantumCircuit(5)
+    qc.h(0)
+    qc.cp(1, 3, 0.5)
+    qc.mcp([0, 2], 4, math.pi)
+    qc.swap(1, 2)
+    qc.qubit_reversal([0, 1, 2, 3])
+    bit_reverse(qc, [1, 3])
+    bit_reverse(qc)
+    qc.unitary(np.eye(4), 1, 2, "u")
+    qc.c_unitary(np.eye(2), 0, 3)
+    ops, ud = lower_circuit(qc)
+    kinds = list(ops["kind"])
+    assert kinds == [OP_GATE, OP_GATE, OP_GATE, OP_SWAP, OP_SWAP, OP_SWAP, OP_BITREV, OP_BITREV, OP_UNITARY,
+                     OP_UNITARY]
+    assert ops["controls"][1] == 1 << 1 and ops["controls"][2] == 0b101
+    assert (ops["target"][4], ops["arg_i"][4]) == (0, 3) and (ops["target"][5], ops["arg_i"][5]) == (1, 2)
+    assert ops["aux"][6] == 0b1010 and ops["aux"][7] == 0
+    assert ops["arg_i"][8] == 2 and ops["aux"][9] == 16 and ops["controls"][9] == 1
+    assert ud.size == 2 * (16 + 4)
+
+
+def test_lowering_permute_is_swap_sequence():
+    qc = QuantumCircuit(4)
+    permute_qubits(qc, [2, 0, 3, 1])
+    ops, _ = lower_circuit(qc)
+    assert all(ops["kind"] == OP_SWAP)
+    idx = np.arange(16)
+    src = idx.copy()  # src[i] = original index now stored at position i
+    for a, b in zip(ops["target"], ops["arg_i"]):
+        x = ((idx >> a) & 1) ^ ((idx >> b) & 1)
+        src = src[idx ^ ((x << a) | (x << b))]
+    # apply_permute_qubits (state.mojo:393-431): new bit q = old bit order[q]
+    order = [2, 0, 3, 1]
+    for i in range(16):
+        assert all(((src[i] >> order[q]) & 1) == ((i >> q) & 1) for q in range(4))
+
+
+def test_lowering_errors():
+    qc = QuantumCircuit(3)
+    qc.h(5)
+    with pytest.raises(LoweringError, match="Target out of bounds"):
+        lower_circuit(qc)
+    qc = QuantumCircuit(3)
+    measure(qc, [0])
+    with pytest.raises(LoweringError, match="measurement"):
+        lower_circuit(qc)
+    qc = QuantumCircuit(3)
+    qc.add_classical("mystery", [0], lambda s, t: None)
+    with pytest.raises(LoweringError, match="cannot be lowered"):
+        lower_circuit(qc)
+    qc = QuantumCircuit(3)
+    qc.c_unitary(np.eye(4), 1, 0, 2, "u")
+    with pytest.raises(LoweringError, match="overlaps"):
+        lower_circuit(qc)
+    with pytest.raises(ValueError, match="Unitary size mismatch"):
+        QuantumCircuit(3).unitary(np.eye(3), 0, 1)
+
+
+def test_self_controlled_gate_is_dropped():
+    qc = QuantumCircuit(2)
+    qc.cx(1, 1)
+    ops, _ = lower_circuit(qc)
+    assert ops.size == 0
+
+
+def test_inverse_and_registers():
+    regs = [Register("a", 2), Register("b", 3)]
+    qc = Circuit(regs)
+    assert qc.num_qubits == 5 and qc.registers[1].start == 2
+    sub = QuantumCircuit(3)
+    sub.ry(0, 0.3)
+    sub.cp(0, 2, 0.7)
+    assert qc.append_circuit_by_name("b", sub)
+    assert qc.transformations[0].target == 2 and qc.transformations[1].controls == [2]
+    inv = qc.inverse()
+    assert inv.transformations[0].gate_info.arg == -0.7 and inv.transformations[1].gate_info.arg == -0.3
+    cq = QuantumCircuit(5)
+    cq.c_append_circuit(0, sub, Register("b", 3, 2))
+    assert cq.transformations[0].controls == [0] and cq.transformations[1].controls == [0, 2]
+    qc2 = QuantumCircuit(2)
+    qc2.add(0, __import__("code_mojo_amd").Custom_Gate(np.eye(2)))
+    with pytest.raises(ValueError, match="Custom gate inversion"):
+        qc2.inverse()
+    m = QuantumCircuit(1)
+    measure(m, [0])
+    with pytest.raises(ValueError, match="measurements"):
+        m.inverse()
+
+
+def test_algorithm_gate_counts():
+    n = 6
+    ve = algos.encode_value_circuit(n, 13)
+    assert len(ve.transformations) == 3 * n + n * (n - 1) // 2
+    g = algos.grover_search_circuit(4, 3, 2)
+    assert sum(isinstance(t, type(g.transformations[0])) for t in g.transformations) == len(g.transformations)
+    ae = algos.amplitude_estimation_circuit(3, _ry_prep(), _z_oracle())
+    assert ae.num_qubits == 4
+
+
+def _ry_prep():
+    p = QuantumCircuit(1)
+    p.ry(0, 0.4)
+    return p
+
+
+def _z_oracle():
+    o = QuantumCircuit(1)
+    o.z(0)
+    return o
+
+
+def test_swap_classical_builder():
+    qc = QuantumCircuit(3)
+    qc.transformations.append(swap_tr(0, 2))
+    ops, _ = lower_circuit(qc)
+    assert ops["kind"][0] == OP_SWAP and (ops["target"][0], ops["arg_i"][0]) == (0, 2)
+    assert isinstance(MeasurementTransformation([0]).values, list)
