@@ -20,6 +20,7 @@ BFLY_OK, BFLY_EINVAL, BFLY_ENOMEM, BFLY_EHIP, BFLY_EUNSUPPORTED, BFLY_ENUMERIC =
 C128, C64 = 0, 1
 MODE_FUSED, MODE_UNFUSED, MODE_BLOCK = 0, 1, 2
 FLAG_SYNC, FLAG_PROFILE = 1, 2
+BATCH_GRAPH = 1
 
 ERROR_NAMES = {BFLY_EINVAL: "EINVAL", BFLY_ENOMEM: "ENOMEM", BFLY_EHIP: "EHIP",
                BFLY_EUNSUPPORTED: "EUNSUPPORTED", BFLY_ENUMERIC: "ENUMERIC"}
@@ -63,6 +64,12 @@ _SIGS = {
                               ct.POINTER(ExecOpts)]),
     "bfly_execute_host": (ct.c_int, [ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_void_p, ct.c_uint64, ct.c_void_p,
                                      ct.c_uint64, ct.c_void_p, ct.c_uint64, ct.POINTER(ExecOpts)]),
+    "bfly_batch_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p),
+                                     ct.c_int32, ct.c_int32]),
+    "bfly_batch_launch": (ct.c_int, [ct.c_void_p]),
+    "bfly_batch_synchronize": (ct.c_int, [ct.c_void_p]),
+    "bfly_batch_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
+    "bfly_batch_destroy": (ct.c_int, [ct.c_void_p]),
     "bfly_bitrev": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "bfly_swap_qubits": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32]),
     "bfly_norm2": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double)]),

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -455,6 +456,207 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
     bfly_state_destroy(s);
     if (rc) g_last_error = keep;
     return rc;
+}
+
+}  // extern "C"
+
+// ---- batched execution -------------------------------------------------------
+struct bfly_batch {
+    struct Launch {
+        int kind;              // S_TILE: grouped tile launch; otherwise one step of one state
+        int dtype = 0, k = 0;
+        size_t item_off = 0;   // S_TILE: first item in d_items
+        int nitems = 0;
+        uint64_t total_tiles = 0;
+        bfly_state* s = nullptr;
+        bfly_plan* p = nullptr;
+        const Step* step = nullptr;
+    };
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<bfly_state*> states;
+    std::vector<Launch> launches;
+    std::vector<BatchItem> items;
+    BatchItem* d_items = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int64_t rounds = 0;
+};
+
+namespace {
+
+hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t stream) {
+    switch (st.kind) {
+        case S_TILE: return launch_tile_pass(s->dtype, s->re, s->im, st.tile, p->d_sweeps, p->d_instrs, stream);
+        case S_GATE: return launch_gate(s->dtype, s->re, s->im, st.gate, stream);
+        case S_BITREV: return launch_bitrev(s->dtype, s->re, s->im, s->n, st.mask, stream);
+        case S_SWAP: return launch_swap(s->dtype, s->re, s->im, s->n, st.a, st.b, stream);
+        case S_UNITARY:
+            return launch_unitary(s->dtype, s->re, s->im, s->n, st.a, st.b, st.controls, p->d_udata + 2 * st.uoff,
+                                  stream);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t enqueue_batch(bfly_batch* b) {
+    for (const auto& L : b->launches) {
+        hipError_t e;
+        if (L.kind == S_TILE) e = launch_tile_batch(L.dtype, L.k, b->d_items + L.item_off, L.nitems, L.total_tiles, b->stream);
+        else e = launch_step(*L.step, L.s, L.p, b->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+void free_batch(bfly_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* const* plans, int32_t count,
+                      int32_t flags) {
+    if (!out) return fail(BFLY_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (count < 1 || !states || !plans) return fail(BFLY_EINVAL, "empty batch");
+    for (int i = 0; i < count; ++i) {
+        int rc = check_state(states[i]);
+        if (rc) return rc;
+        if (!plans[i]) return fail(BFLY_EINVAL, "null plan");
+        if (plans[i]->hp.n != states[i]->n || plans[i]->hp.dtype != states[i]->dtype)
+            return fail(BFLY_EINVAL, "plan / state mismatch at item " + std::to_string(i));
+        if (states[i]->device != states[0]->device) return fail(BFLY_EINVAL, "states on different devices");
+        for (int j = 0; j < i; ++j)
+            if (states[j] == states[i]) return fail(BFLY_EINVAL, "a state appears twice in one batch");
+    }
+    auto* b = new bfly_batch();
+    b->device = states[0]->device;
+    b->states.assign(states, states + count);
+    HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch stream"); }
+    size_t max_steps = 0;
+    for (int i = 0; i < count; ++i) {
+        int rc = upload_plan(plans[i], b->device);
+        if (rc) { free_batch(b); return rc; }
+        max_steps = std::max(max_steps, plans[i]->hp.steps.size());
+    }
+    // rounds: step r of every plan; tile steps grouped by (dtype, k)
+    for (size_t r = 0; r < max_steps; ++r) {
+        std::vector<std::pair<int, std::vector<int>>> groups;  // key = dtype * 64 + k
+        for (int i = 0; i < count; ++i) {
+            const auto& steps = plans[i]->hp.steps;
+            if (r >= steps.size()) continue;
+            const Step& st = steps[r];
+            if (st.kind == S_TILE) {
+                const int key = states[i]->dtype * 64 + st.tile.k;
+                auto it = std::find_if(groups.begin(), groups.end(), [&](auto& g) { return g.first == key; });
+                if (it == groups.end()) groups.push_back({key, {i}});
+                else it->second.push_back(i);
+            } else {
+                bfly_batch::Launch L{};
+                L.kind = st.kind;
+                L.s = states[i];
+                L.p = plans[i];
+                L.step = &st;
+                b->launches.push_back(L);
+            }
+        }
+        for (auto& g : groups) {
+            bfly_batch::Launch L{};
+            L.kind = S_TILE;
+            L.dtype = g.first / 64;
+            L.k = g.first % 64;
+            L.item_off = b->items.size();
+            uint64_t tiles = 0;
+            for (int i : g.second) {
+                const Step& st = plans[i]->hp.steps[r];
+                BatchItem it{};
+                it.re = states[i]->re;
+                it.im = states[i]->im;
+                it.sweeps = plans[i]->d_sweeps;
+                it.instrs = plans[i]->d_instrs;
+                it.tile_begin = tiles;
+                it.P = st.tile;
+                tiles += 1ull << (st.tile.n - st.tile.k);
+                b->items.push_back(it);
+            }
+            L.nitems = static_cast<int>(g.second.size());
+            L.total_tiles = tiles;
+            e = prepare_tile_kernels(L.dtype, L.k);
+            if (e != hipSuccess) { free_batch(b); return hip_fail(e, "tile kernel preparation"); }
+            b->launches.push_back(L);
+        }
+    }
+    b->rounds = static_cast<int64_t>(max_steps);
+    if (!b->items.empty()) {
+        e = hipMalloc(&b->d_items, b->items.size() * sizeof(BatchItem));
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch items"); }
+    }
+    if (flags & BFLY_BATCH_GRAPH) {
+        e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t e2 = enqueue_batch(b);
+            e = hipStreamEndCapture(b->stream, &b->graph);
+            if (e2 != hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch graph capture"); }
+    }
+    *out = b;
+    return BFLY_OK;
+}
+
+int bfly_batch_launch(bfly_batch* b) {
+    if (!b) return fail(BFLY_EINVAL, "null batch");
+    HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    // the batch runs after pending work on every state, and later work on a
+    // state runs after the batch
+    for (bfly_state* s : b->states) {
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        hipError_t e = hipEventRecord(ev, s->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, ev, 0);
+        (void)hipEventDestroy(ev);
+        if (e != hipSuccess) return hip_fail(e, "batch ordering");
+    }
+    if (b->gexec) HIP_TRY(hipGraphLaunch(b->gexec, b->stream), "hipGraphLaunch");
+    else HIP_TRY(enqueue_batch(b), "batch launch");
+    HIP_TRY(hipEventRecord(b->done, b->stream), "hipEventRecord");
+    for (bfly_state* s : b->states) HIP_TRY(hipStreamWaitEvent(s->stream, b->done, 0), "hipStreamWaitEvent");
+    return BFLY_OK;
+}
+
+int bfly_batch_synchronize(bfly_batch* b) {
+    if (!b) return fail(BFLY_EINVAL, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return BFLY_OK;
+}
+
+int bfly_batch_stats(const bfly_batch* b, int64_t* rounds, int64_t* launches) {
+    if (!b) return fail(BFLY_EINVAL, "null batch");
+    if (rounds) *rounds = b->rounds;
+    if (launches) *launches = static_cast<int64_t>(b->launches.size());
+    return BFLY_OK;
+}
+
+int bfly_batch_destroy(bfly_batch* b) {
+    if (!b) return BFLY_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_batch(b);
+    return BFLY_OK;
 }
 
 int bfly_bitrev(bfly_state* s, uint64_t target_mask) {

@@ -11,6 +11,23 @@ namespace bfly {
 // (b) one tile pass: every gate of the pass in one HBM read + write.
 hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P,
                             const TileSweep* d_sweeps, const TileInstr* d_instrs, hipStream_t stream);
+// One circuit's tile pass inside a batched launch (device-resident array).
+struct BatchItem {
+    void* re;
+    void* im;
+    const TileSweep* sweeps;
+    const TileInstr* instrs;
+    uint64_t tile_begin;   // first global tile of this item in the launch
+    TilePass P;
+};
+// (b') the next tile pass of many circuits in one persistent launch; all
+// items share the tile size k and the dtype.  d_items == nullptr only
+// prepares the kernel (attributes, occupancy) without launching.
+hipError_t launch_tile_batch(int dtype, int k, const BatchItem* d_items, int nitems, uint64_t total_tiles,
+                             hipStream_t stream);
+// Sets kernel attributes and sizes the persistent grids of the tile kernels
+// for (dtype, k) outside any stream capture.
+hipError_t prepare_tile_kernels(int dtype, int k);
 // (a) one gate over the whole state (coalesced strided butterfly pairs).
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream);
 // (c) bit reversal of all bits (mask 0) or of the bits in `mask`, in place.

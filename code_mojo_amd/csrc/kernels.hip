@@ -235,6 +235,89 @@ __device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t
     }
 }
 
+// Sweeps of one tile pass on the tile staged in LDS (s_re, s_im): each of
+// the 2^(K-4) groups of 16 amplitudes spanned by the sweep's 4 slot bits is
+// loaded into registers, every gate of the sweep runs in registers, and the
+// group is written back.  Ends with a barrier after every sweep.
+template <typename T, int K>
+__device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, const TileSweep* __restrict__ sweeps,
+                                           const TileInstr* __restrict__ instrs, uint64_t base, uint32_t tid) {
+    constexpr uint32_t NG = (1u << K) / kGroup;
+    const int nsweeps = P.nsweeps;
+    const int first_sweep = P.first_sweep;
+    for (int s = 0; s < nsweeps; ++s) {
+        const TileSweep sw = sweeps[first_sweep + s];
+        if (tid < NG) {
+            int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
+            int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
+#pragma unroll
+            for (int a = 0; a < kSlots; ++a)
+#pragma unroll
+                for (int b2 = a + 1; b2 < kSlots; ++b2)
+                    if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
+            uint32_t jb = tid;
+#pragma unroll
+            for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
+            // member m of the group sits at local index jb | dep(m); dep is
+            // uniform, so it lives in scalar registers
+            auto dep = [&](int m) {
+                uint32_t d = 0;
+#pragma unroll
+                for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
+                return d;
+            };
+            T ar[kGroup], ai[kGroup];
+#pragma unroll
+            for (int m = 0; m < kGroup; ++m) {
+                const uint32_t jm = swz<T>(jb | dep(m));
+                ar[m] = s_re[jm];
+                ai[m] = s_im[jm];
+            }
+            for (int q = 0; q < sw.ninstr; ++q) {
+                const TileInstr& in = instrs[sw.first + q];
+                if ((base & in.omask) != in.omask) continue;
+                if ((jb & in.lmask) != in.lmask) continue;
+                const int kind = in.kind;
+                T g[8];
+                load_params<T>(in.m, in.mf, g);
+                if (kind == T_PAIR) {
+                    switch (in.slot) {
+                        case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
+                        case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
+                        case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
+                        default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
+                    }
+                } else {
+                    uint32_t sel1 = in.mm1, sel0 = in.mm0;
+                    if (in.slot < 0) {
+                        // target outside the slots: one bit per thread (local) or per tile (outer)
+                        const uint32_t bit = in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
+                        sel1 = bit ? sel1 : 0u;
+                        sel0 = bit ? 0u : sel0;
+                    }
+                    if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
+                    else diag_apply<T>(ar, ai, sel1, sel0, g);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < kGroup; ++m) {
+                const uint32_t jm = swz<T>(jb | dep(m));
+                s_re[jm] = ar[m];
+                s_im[jm] = ai[m];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t tile_base(uint64_t t, const TilePass& P) {
+    uint64_t b = 0;
+    const int nouter = P.nouter;
+    for (int i = 0; i < nouter; ++i) b |= ((t >> i) & 1ull) << P.opos[i];
+    return b;
+}
+
 // (b) One tile pass.  A tile is 2^K amplitudes: local bit i is global bit
 // P.qpos[i] (the low local bits are the low qubits, so VN consecutive local
 // indices are VN consecutive amplitudes), the other bits come from the tile
@@ -248,8 +331,7 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
         T* __restrict__ re, T* __restrict__ im, const TilePass P, const TileSweep* __restrict__ sweeps,
         const TileInstr* __restrict__ instrs) {
     constexpr uint32_t TS = 1u << K;
-    constexpr uint32_t NG = TS / kGroup;           // groups per tile
-    constexpr uint32_t NT = K >= 10 ? NG : 64;     // threads per block
+    constexpr uint32_t NT = K >= 10 ? TS / kGroup : 64;  // threads per block
     constexpr int VN = Vec<T>::N;
     constexpr uint32_t NV = TS / VN;               // 16-byte vectors per tile (re or im)
     constexpr uint32_t PV = (NV + NT - 1) / NT;    // vectors per thread
@@ -262,11 +344,6 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
     uint64_t tile = blockIdx.x;
     if (tile >= ntiles) return;
 
-    auto tile_base = [&](uint64_t t) {
-        uint64_t b = 0;
-        for (int i = 0; i < P.nouter; ++i) b |= ((t >> i) & 1ull) << P.opos[i];
-        return b;
-    };
     // local index of this thread's c-th vector is (tid + c*NT)*VN; its global
     // offset splits into a per-thread part and a uniform part (disjoint bits)
     const uint64_t off_t = local_to_global<K>(tid * VN, P);
@@ -281,7 +358,7 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
             }
         }
     };
-    uint64_t base = tile_base(tile);
+    uint64_t base = tile_base<K>(tile, P);
     load_tile(base);
 
     for (;;) {
@@ -303,74 +380,11 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
         const uint64_t next = tile + gridDim.x;
         uint64_t nbase = 0;
         if (next < ntiles) {
-            nbase = tile_base(next);
+            nbase = tile_base<K>(next, P);
             load_tile(nbase);  // in flight during the sweeps
         }
 
-        for (int s = 0; s < P.nsweeps; ++s) {
-            const TileSweep sw = sweeps[P.first_sweep + s];
-            if (tid < NG) {
-                int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
-                int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
-#pragma unroll
-                for (int a = 0; a < kSlots; ++a)
-#pragma unroll
-                    for (int b2 = a + 1; b2 < kSlots; ++b2)
-                        if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
-                uint32_t jb = tid;
-#pragma unroll
-                for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
-                // member m of the group sits at local index jb | dep(m); dep is
-                // uniform, so it lives in scalar registers
-                auto dep = [&](int m) {
-                    uint32_t d = 0;
-#pragma unroll
-                    for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
-                    return d;
-                };
-                T ar[kGroup], ai[kGroup];
-#pragma unroll
-                for (int m = 0; m < kGroup; ++m) {
-                    const uint32_t jm = swz<T>(jb | dep(m));
-                    ar[m] = s_re[jm];
-                    ai[m] = s_im[jm];
-                }
-                for (int q = 0; q < sw.ninstr; ++q) {
-                    const TileInstr& in = instrs[sw.first + q];
-                    if ((base & in.omask) != in.omask) continue;
-                    if ((jb & in.lmask) != in.lmask) continue;
-                    const int kind = in.kind;
-                    T g[8];
-                    load_params<T>(in.m, in.mf, g);
-                    if (kind == T_PAIR) {
-                        switch (in.slot) {
-                            case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
-                            case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
-                            case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
-                            default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
-                        }
-                    } else {
-                        uint32_t sel1 = in.mm1, sel0 = in.mm0;
-                        if (in.slot < 0) {
-                            // target outside the slots: one bit per thread (local) or per tile (outer)
-                            const uint32_t bit =
-                                in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
-                            sel1 = bit ? sel1 : 0u;
-                            sel0 = bit ? 0u : sel0;
-                        }
-                        if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
-                        else diag_apply<T>(ar, ai, sel1, sel0, g);
-                    }
-                }
-#pragma unroll
-                for (int m = 0; m < kGroup; ++m) {
-                    const uint32_t jm = swz<T>(jb | dep(m));
-                    s_re[jm] = ar[m];
-                    s_im[jm] = ai[m];
-                }
-            }
-            __syncthreads();
-        }
+        run_sweeps<T, K>(s_re, s_im, P, sweeps, instrs, base, tid);
 
         // stage out: LDS -> HBM with 16-byte stores
 #pragma unroll
@@ -397,31 +411,130 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
     }
 }
 
+// Batched tile passes: one launch runs the next tile pass of many circuits
+// (each with its own state and plan).  Item i owns global tiles
+// [items[i].tile_begin, items[i].tile_begin + 2^(n_i - K)); blocks walk the
+// global tile range persistently and find their item by binary search.
+template <typename T, int K>
+__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_batch_kernel(
+        const BatchItem* __restrict__ items, int nitems, uint64_t total_tiles) {
+    constexpr uint32_t TS = 1u << K;
+    constexpr uint32_t NT = K >= 10 ? TS / kGroup : 64;
+    constexpr int VN = Vec<T>::N;
+    constexpr uint32_t NV = TS / VN;
+    using V = typename Vec<T>::type;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* s_re = reinterpret_cast<T*>(lds_raw);
+    T* s_im = s_re + TS;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].tile_begin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchItem& it = items[lo];
+        T* re = static_cast<T*>(it.re);
+        T* im = static_cast<T*>(it.im);
+        const uint64_t base = tile_base<K>(t - it.tile_begin, it.P);
+        for (uint32_t e = tid; e < NV; e += NT) {
+            const uint32_t j = e * VN;
+            const uint64_t g = base + local_to_global<K>(j, it.P);
+            const V vr = *reinterpret_cast<const V*>(re + g);
+            const V vi = *reinterpret_cast<const V*>(im + g);
+            const T* a = reinterpret_cast<const T*>(&vr);
+            const T* b = reinterpret_cast<const T*>(&vi);
+#pragma unroll
+            for (int v = 0; v < VN; ++v) {
+                s_re[swz<T>(j + v)] = a[v];
+                s_im[swz<T>(j + v)] = b[v];
+            }
+        }
+        __syncthreads();
+        run_sweeps<T, K>(s_re, s_im, it.P, it.sweeps, it.instrs, base, tid);
+        for (uint32_t e = tid; e < NV; e += NT) {
+            const uint32_t j = e * VN;
+            V vr, vi;
+            T* a = reinterpret_cast<T*>(&vr);
+            T* b = reinterpret_cast<T*>(&vi);
+#pragma unroll
+            for (int v = 0; v < VN; ++v) {
+                a[v] = s_re[swz<T>(j + v)];
+                b[v] = s_im[swz<T>(j + v)];
+            }
+            const uint64_t g = base + local_to_global<K>(j, it.P);
+            *reinterpret_cast<V*>(re + g) = vr;
+            *reinterpret_cast<V*>(im + g) = vi;
+        }
+        __syncthreads();  // stage-out reads finish before the next tile lands in LDS
+    }
+}
+
 constexpr int tile_threads(int k) { return k >= 10 ? (1 << (k - 4)) : 64; }
+
+// Kernel attributes (dynamic LDS beyond 64 KiB) and the persistent grid size
+// (blocks that fit on the device at once) for one kernel instance.
+template <typename KernelT>
+hipError_t persistent_grid(KernelT kernel, int nt, size_t lds, int* grid_cap) {
+    if (*grid_cap > 0) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    int per_cu = 0, dev = 0, cus = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, lds);
+    if (e != hipSuccess) return e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    *grid_cap = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    return hipSuccess;
+}
 
 template <typename T, int K>
 hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
     constexpr int NT = tile_threads(K);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
-    // persistent grid: as many blocks as fit on the device at once
     static int grid_cap = 0;
-    if (grid_cap == 0) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&tile_pass_kernel<T, K>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return e;
-        int per_cu = 0, dev = 0, cus = 0;
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tile_pass_kernel<T, K>, NT, lds);
-        if (e != hipSuccess) return e;
-        e = hipGetDevice(&dev);
-        if (e != hipSuccess) return e;
-        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e != hipSuccess) return e;
-        grid_cap = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
-    }
+    hipError_t e = persistent_grid(tile_pass_kernel<T, K>, NT, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    if (!re) return hipSuccess;  // preparation only
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
     hipLaunchKernelGGL((tile_pass_kernel<T, K>), dim3(grid), dim3(NT), lds, st, re, im, P, sw, in);
     return hipGetLastError();
+}
+
+template <typename T, int K>
+hipError_t launch_batch_k(const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
+    constexpr int NT = tile_threads(K);
+    const size_t lds = 2ull * (1ull << K) * sizeof(T);
+    static int grid_cap = 0;
+    hipError_t e = persistent_grid(tile_batch_kernel<T, K>, NT, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    if (!items) return hipSuccess;  // preparation only
+    const unsigned grid =
+        static_cast<unsigned>(total_tiles < static_cast<uint64_t>(grid_cap) ? total_tiles : grid_cap);
+    hipLaunchKernelGGL((tile_batch_kernel<T, K>), dim3(grid), dim3(NT), lds, st, items, nitems, total_tiles);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch_t(int k, const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
+    switch (k) {
+        case 4: return launch_batch_k<T, 4>(items, nitems, total_tiles, st);
+        case 5: return launch_batch_k<T, 5>(items, nitems, total_tiles, st);
+        case 6: return launch_batch_k<T, 6>(items, nitems, total_tiles, st);
+        case 7: return launch_batch_k<T, 7>(items, nitems, total_tiles, st);
+        case 8: return launch_batch_k<T, 8>(items, nitems, total_tiles, st);
+        case 9: return launch_batch_k<T, 9>(items, nitems, total_tiles, st);
+        case 10: return launch_batch_k<T, 10>(items, nitems, total_tiles, st);
+        case 11: return launch_batch_k<T, 11>(items, nitems, total_tiles, st);
+        case 12: return launch_batch_k<T, 12>(items, nitems, total_tiles, st);
+        case 13: return launch_batch_k<T, 13>(items, nitems, total_tiles, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -773,6 +886,21 @@ hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P, co
     if (dtype == BFLY_C64)
         return launch_tile_t<float>(static_cast<float*>(re), static_cast<float*>(im), P, d_sweeps, d_instrs, stream);
     return launch_tile_t<double>(static_cast<double*>(re), static_cast<double*>(im), P, d_sweeps, d_instrs, stream);
+}
+
+hipError_t prepare_tile_kernels(int dtype, int k) {
+    TilePass P{};
+    P.k = k;
+    P.n = k;
+    hipError_t e = launch_tile_pass(dtype, nullptr, nullptr, P, nullptr, nullptr, nullptr);
+    if (e != hipSuccess) return e;
+    return launch_tile_batch(dtype, k, nullptr, 0, 0, nullptr);
+}
+
+hipError_t launch_tile_batch(int dtype, int k, const BatchItem* d_items, int nitems, uint64_t total_tiles,
+                             hipStream_t stream) {
+    if (dtype == BFLY_C64) return launch_batch_t<float>(k, d_items, nitems, total_tiles, stream);
+    return launch_batch_t<double>(k, d_items, nitems, total_tiles, stream);
 }
 
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream) {

@@ -228,3 +228,42 @@ class Plan:
             self.free()
         except Exception:
             pass
+
+
+class Batch:
+    """Many circuits executed round by round (bfly_batch): round r runs step
+    r of every plan, tile passes of one round grouped into one launch per
+    tile size.  graph=True captures all rounds into one hipGraph."""
+
+    def __init__(self, states, plans, graph: bool = False):
+        if len(states) != len(plans) or not states:
+            raise ValueError("one plan per state")
+        lib = _lib.load()
+        self._states, self._plans = list(states), list(plans)  # keep alive
+        sa = (ct.c_void_p * len(states))(*[s.handle.value for s in states])
+        pa = (ct.c_void_p * len(plans))(*[p._h.value for p in plans])
+        h = ct.c_void_p()
+        check(lib.bfly_batch_create(ct.byref(h), sa, pa, len(states), _lib.BATCH_GRAPH if graph else 0))
+        self._h = h
+
+    def launch(self) -> None:
+        check(_lib.load().bfly_batch_launch(self._h))
+
+    def synchronize(self) -> None:
+        check(_lib.load().bfly_batch_synchronize(self._h))
+
+    def stats(self) -> dict:
+        r, l = ct.c_int64(), ct.c_int64()
+        check(_lib.load().bfly_batch_stats(self._h, ct.byref(r), ct.byref(l)))
+        return {"rounds": r.value, "launches": l.value}
+
+    def free(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            _lib.load().bfly_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

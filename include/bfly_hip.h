@@ -153,6 +153,22 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
                       const bfly_op* ops, uint64_t nops,
                       const double* udata, uint64_t udata_len, const bfly_exec_opts* opts);
 
+/* ---- batched execution: many circuits, one launch per round ------------
+   No counterpart in the reference (it runs one circuit at a time).  Round r
+   runs step r of every plan; the tile passes of one round that share a tile
+   size are one persistent launch.  States and plans are borrowed and must
+   outlive the batch; each state gets its own plan. */
+typedef struct bfly_batch bfly_batch;
+#define BFLY_BATCH_GRAPH 1    /* capture every round into one hipGraph, replayed by bfly_batch_launch */
+int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* const* plans, int32_t count,
+                      int32_t flags);
+/* Enqueue all rounds (ordered after pending work on every state's stream;
+   later calls on those states are ordered after the batch). */
+int bfly_batch_launch(bfly_batch* b);
+int bfly_batch_synchronize(bfly_batch* b);
+int bfly_batch_stats(const bfly_batch* b, int64_t* rounds, int64_t* launches);
+int bfly_batch_destroy(bfly_batch* b);
+
 /* ---- permutations: replaces bit_reverse_state / partial_bit_reverse_state
         (state.mojo:131-204,252-301) and swap_bits_state (state.mojo:304-317) -- */
 int bfly_bitrev(bfly_state* s, uint64_t target_mask);
