@@ -40,6 +40,23 @@ class ExecOpts(ct.Structure):
                 ("renorm_every", ct.c_int32), ("threads", ct.c_int32), ("flags", ct.c_int32)]
 
 
+class EngineOpts(ct.Structure):
+    _fields_ = [("budget_bytes", ct.c_uint64), ("max_inflight_rounds", ct.c_int32), ("large_qubits", ct.c_int32),
+                ("max_round_requests", ct.c_int32), ("flags", ct.c_int32)]
+
+
+class RequestDesc(ct.Structure):
+    _fields_ = [("num_qubits", ct.c_int32), ("dtype", ct.c_int32), ("ops", ct.c_void_p), ("nops", ct.c_uint64),
+                ("udata", ct.c_void_p), ("udata_len", ct.c_uint64), ("init_re", ct.c_void_p),
+                ("init_im", ct.c_void_p), ("init_kind", ct.c_int32), ("pad0", ct.c_int32),
+                ("init_seed", ct.c_uint64), ("exec", ExecOpts)]
+
+
+INIT_ZERO, INIT_HOST, INIT_RANDOM = 0, 1, 2
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+BFLY_ETIMEDOUT, BFLY_ESTATE = -6, -7
+ERROR_NAMES.update({BFLY_ETIMEDOUT: "ETIMEDOUT", BFLY_ESTATE: "ESTATE"})
+
 _SIGS = {
     "bfly_last_error": (ct.c_char_p, []),
     "bfly_version": (ct.c_int, []),
@@ -70,6 +87,20 @@ _SIGS = {
     "bfly_batch_synchronize": (ct.c_int, [ct.c_void_p]),
     "bfly_batch_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "bfly_batch_destroy": (ct.c_int, [ct.c_void_p]),
+    "bfly_engine_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.POINTER(EngineOpts)]),
+    "bfly_engine_submit": (ct.c_int, [ct.c_void_p, ct.POINTER(RequestDesc), ct.POINTER(ct.c_uint64)]),
+    "bfly_engine_status": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double),
+                                      ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
+    "bfly_engine_wait": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_double]),
+    "bfly_engine_fetch": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_void_p, ct.c_void_p, ct.c_uint64]),
+    "bfly_engine_release": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "bfly_engine_cancel": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "bfly_engine_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64),
+                                     ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_uint64),
+                                     ct.POINTER(ct.c_uint64)]),
+    "bfly_engine_error": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_char_p, ct.c_uint64]),
+    "bfly_engine_destroy": (ct.c_int, [ct.c_void_p]),
+    "bfly_state_fill_random": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "bfly_bitrev": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "bfly_swap_qubits": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32]),
     "bfly_norm2": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double)]),

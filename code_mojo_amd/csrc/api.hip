@@ -9,37 +9,21 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
 
-#include "bfly_plan.h"
-#include "kernels.h"
+#include "internal.h"
 
 using namespace bfly;
 
-struct bfly_state {
-    int32_t n = 0;
-    int32_t dtype = BFLY_C128;
-    int device = 0;
-    void* re = nullptr;
-    void* im = nullptr;
-    double* scratch = nullptr;   // reduction partials
-    hipStream_t stream = nullptr;
-};
-
-struct bfly_plan {
-    HostPlan hp;
-    int device = -1;
-    TileSweep* d_sweeps = nullptr;
-    TileInstr* d_instrs = nullptr;
-    double* d_udata = nullptr;
-};
-
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace bfly {
 
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
@@ -50,11 +34,9 @@ int hip_fail(hipError_t e, const char* what) {
     return fail(BFLY_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)                              \
-    do {                                                 \
-        hipError_t _e = (expr);                          \
-        if (_e != hipSuccess) return hip_fail(_e, what); \
-    } while (0)
+}  // namespace bfly
+
+namespace {
 
 size_t elem_size(int dtype) { return dtype == BFLY_C64 ? sizeof(float) : sizeof(double); }
 
@@ -100,6 +82,10 @@ struct ProfScope {
     }
 };
 
+}  // namespace
+
+namespace bfly {
+
 int check_state(const bfly_state* s) {
     if (!s || !s->re || !s->im) return fail(BFLY_EINVAL, "null or destroyed state");
     return BFLY_OK;
@@ -126,6 +112,10 @@ int upload_plan(bfly_plan* p, int device) {
     p->device = device;
     return BFLY_OK;
 }
+
+}  // namespace bfly
+
+namespace {
 
 int run_reduce(bfly_state* a, bfly_state* b, int mode, uint64_t mask, uint64_t value, double* out, int nout) {
     int rc = check_state(a);
@@ -484,7 +474,7 @@ struct bfly_batch {
     int64_t rounds = 0;
 };
 
-namespace {
+namespace bfly {
 
 hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t stream) {
     switch (st.kind) {
@@ -498,6 +488,10 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
     }
     return hipErrorInvalidValue;
 }
+
+}  // namespace bfly
+
+namespace {
 
 hipError_t enqueue_batch(bfly_batch* b) {
     for (const auto& L : b->launches) {
@@ -735,6 +729,18 @@ int bfly_inner(bfly_state* a, bfly_state* b, double* re, double* im) {
 int bfly_diff_norm2(bfly_state* a, bfly_state* b, double* out) {
     if (!out) return fail(BFLY_EINVAL, "null output");
     return run_reduce(a, b, R_DIFF2, 0, 0, out, 1);
+}
+
+int bfly_state_fill_random(bfly_state* s, uint64_t seed) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    HIP_TRY(launch_random_fill(s->dtype, s->re, s->im, 1ull << s->n, seed, s->stream), "random fill launch");
+    double nrm = 0;
+    if ((rc = run_reduce(s, nullptr, R_NORM2, 0, 0, &nrm, 1))) return rc;
+    if (!(nrm > 0)) return fail(BFLY_ENUMERIC, "random state has zero norm");
+    HIP_TRY(launch_scale(s->dtype, s->re, s->im, 1ull << s->n, 1.0 / std::sqrt(nrm), s->stream), "scale launch");
+    return BFLY_OK;
 }
 
 int bfly_scale(bfly_state* s, double scale) {

@@ -1,0 +1,48 @@
+// internal.h -- objects behind the C ABI's opaque handles, shared by the
+// translation units of libbfly_hip.so (api.hip, engine.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "bfly_plan.h"
+#include "kernels.h"
+
+#include <vector>
+
+struct bfly_state {
+    int32_t n = 0;
+    int32_t dtype = BFLY_C128;
+    int device = 0;
+    void* re = nullptr;
+    void* im = nullptr;
+    double* scratch = nullptr;   // reduction partials
+    hipStream_t stream = nullptr;
+};
+
+struct bfly_plan {
+    bfly::HostPlan hp;
+    int device = -1;
+    bfly::TileSweep* d_sweeps = nullptr;
+    bfly::TileInstr* d_instrs = nullptr;
+    double* d_udata = nullptr;
+};
+
+namespace bfly {
+
+// Record `msg` as the calling thread's last error and return `code`.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int check_state(const bfly_state* s);
+// Copy a plan's sweep / instruction / unitary records to `device` (once).
+int upload_plan(bfly_plan* p, int device);
+// Enqueue one plan step on `stream`.
+hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t stream);
+
+}  // namespace bfly
+
+#define HIP_TRY(expr, what)                                      \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return bfly::hip_fail(_e, what);   \
+    } while (0)

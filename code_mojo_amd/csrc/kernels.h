@@ -38,6 +38,8 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
                           const double* d_u, hipStream_t stream);
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
+// re, im ~ N(0,1) from a SplitMix64 stream (not normalised).
+hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint64_t seed, hipStream_t stream);
 
 // (d) reductions.  Partials live in `scratch` (at least reduce_scratch_doubles()
 // doubles); the final value(s) are left in scratch[0..nout) on the device.

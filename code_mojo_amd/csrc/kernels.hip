@@ -718,6 +718,30 @@ __global__ __launch_bounds__(256) void reset_kernel(T* __restrict__ re, T* __res
     }
 }
 
+// SplitMix64 output number `idx` (1-based) of the stream started at `seed`.
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t idx) {
+    uint64_t z = seed + idx * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// re, im ~ N(0,1): uniforms u = top 53 bits of outputs 2i+1, 2i+2;
+// Box-Muller with u1 = 1 - u (in (0, 1]).
+template <typename T>
+__global__ __launch_bounds__(256) void random_fill_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size,
+                                                          uint64_t seed) {
+    const double two_pi = 6.283185307179586;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        const double u1 = 1.0 - static_cast<double>(splitmix64_at(seed, 2 * i + 1) >> 11) * 0x1p-53;
+        const double u2 = static_cast<double>(splitmix64_at(seed, 2 * i + 2) >> 11) * 0x1p-53;
+        const double r = sqrt(-2.0 * log(u1));
+        re[i] = static_cast<T>(r * cos(two_pi * u2));
+        im[i] = static_cast<T>(r * sin(two_pi * u2));
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size, double s) {
     const T sc = static_cast<T>(s);
@@ -955,6 +979,16 @@ hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_
     else
         hipLaunchKernelGGL(reset_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<double*>(re),
                            static_cast<double*>(im), size);
+    return hipGetLastError();
+}
+
+hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint64_t seed, hipStream_t stream) {
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(random_fill_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream,
+                           static_cast<float*>(re), static_cast<float*>(im), size, seed);
+    else
+        hipLaunchKernelGGL(random_fill_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream,
+                           static_cast<double*>(re), static_cast<double*>(im), size, seed);
     return hipGetLastError();
 }
 

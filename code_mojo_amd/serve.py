@@ -1,0 +1,295 @@
+"""Serving loop and batch driver around the native engine (bfly_engine).
+
+The engine (code_mojo_amd/csrc/engine.hip) admits circuits against an HBM
+budget, keeps their states resident and advances every running circuit
+together, fusing the tile passes of a round into one batched launch per
+tile size.  This module builds the seeded request streams of the serving
+and batching workloads, submits them (at Poisson arrival times for the
+serving stream), and reports circuits/s and latency percentiles.  The
+reference has no serving layer; its per-call `execute()` is the
+"one circuit per launch" baseline measured by `run_sequential`.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib, algos, workloads as W
+from ._lib import check
+from .circuit import QuantumCircuit
+from .lowering import lower_circuit
+from .state import GpuState, Plan, QuantumState, dtype_code
+
+
+@dataclass
+class Job:
+    """One circuit to serve: lowered ops + initial state recipe."""
+
+    name: str
+    n: int
+    dtype: type
+    ops: np.ndarray
+    udata: np.ndarray
+    init_kind: int = _lib.INIT_ZERO
+    init_seed: int = 0
+    init: Optional[QuantumState] = None
+    meta: Dict = field(default_factory=dict)
+
+
+class Engine:
+    """Python handle of a bfly_engine."""
+
+    def __init__(self, budget_bytes: int = 0, max_inflight_rounds: int = 0, large_qubits: int = 0,
+                 max_round_requests: int = 0):
+        lib = _lib.load()
+        opts = _lib.EngineOpts(budget_bytes, max_inflight_rounds, large_qubits, max_round_requests, 0)
+        h = ct.c_void_p()
+        check(lib.bfly_engine_create(ct.byref(h), ct.byref(opts)))
+        self._h = h
+        self._keep: Dict[int, Tuple] = {}
+
+    def submit(self, job: Job, exec_opts: Optional[_lib.ExecOpts] = None) -> int:
+        rq = _lib.RequestDesc()
+        rq.num_qubits = job.n
+        rq.dtype = dtype_code(job.dtype)
+        rq.ops = _lib.ops_ptr(job.ops)
+        rq.nops = job.ops.size
+        rq.udata = _lib.udata_ptr(job.udata)
+        rq.udata_len = job.udata.size // 2
+        rq.init_kind = job.init_kind
+        rq.init_seed = job.init_seed
+        if job.init_kind == _lib.INIT_HOST:
+            rq.init_re = job.init.re.ctypes.data
+            rq.init_im = job.init.im.ctypes.data
+        rq.exec = exec_opts or _lib.opts()
+        rid = ct.c_uint64()
+        check(_lib.load().bfly_engine_submit(self._h, ct.byref(rq), ct.byref(rid)))
+        return rid.value
+
+    def status(self, rid: int) -> Tuple[int, float, float, float]:
+        st, a, b, c = ct.c_int32(), ct.c_double(), ct.c_double(), ct.c_double()
+        check(_lib.load().bfly_engine_status(self._h, rid, ct.byref(st), ct.byref(a), ct.byref(b), ct.byref(c)))
+        return st.value, a.value, b.value, c.value
+
+    def wait(self, rid: int, timeout: float = -1.0) -> None:
+        check(_lib.load().bfly_engine_wait(self._h, rid, timeout))
+
+    def fetch(self, rid: int, n: int, dtype) -> QuantumState:
+        out = QuantumState(n, dtype)
+        check(_lib.load().bfly_engine_fetch(self._h, rid, out.re.ctypes.data, out.im.ctypes.data, out.size()))
+        return out
+
+    def release(self, rid: int) -> None:
+        check(_lib.load().bfly_engine_release(self._h, rid))
+
+    def cancel(self, rid: int) -> None:
+        check(_lib.load().bfly_engine_cancel(self._h, rid))
+
+    def error(self, rid: int) -> str:
+        buf = ct.create_string_buffer(512)
+        check(_lib.load().bfly_engine_error(self._h, rid, buf, 512))
+        return buf.value.decode()
+
+    def stats(self) -> dict:
+        a, c, r, l = ct.c_int64(), ct.c_int64(), ct.c_int64(), ct.c_int64()
+        u, p = ct.c_uint64(), ct.c_uint64()
+        check(_lib.load().bfly_engine_stats(self._h, ct.byref(a), ct.byref(c), ct.byref(r), ct.byref(l),
+                                            ct.byref(u), ct.byref(p)))
+        return {"admitted": a.value, "completed": c.value, "rounds": r.value, "launches": l.value,
+                "bytes_in_use": u.value, "peak_bytes": p.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            _lib.load().bfly_engine_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -- workloads ----------------------------------------------------------------
+def layered_job(n: int, depth: int, seed: int, dtype) -> Job:
+    ops, ud = lower_circuit(W.random_layered(n, depth, seed))
+    return Job(f"layered{n}x{depth}", n, dtype, ops, ud)
+
+
+def qft_job(n: int, seed: int, dtype) -> Job:
+    ops, ud = lower_circuit(W.qft_circuit(n))
+    return Job(f"qft{n}", n, dtype, ops, ud, _lib.INIT_RANDOM, seed)
+
+
+def grover_ops(n: int, marked: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Lowered ops of `algos.grover_search_circuit(n, marked)`, built from one
+    lowered iterate repeated (lowering is gate by gate, so the result is
+    identical to lowering the whole circuit, in a fraction of the time)."""
+    iters = int(math.floor(math.pi / 4 * math.sqrt(2 ** n)))
+    head = QuantumCircuit(n)
+    for i in range(n):
+        head.h(i)
+    h_ops, h_ud = lower_circuit(head)
+    step = algos.grover_iterate_circuit(algos.phase_oracle_match(n, [marked]), n)
+    s_ops, s_ud = lower_circuit(step)
+    assert h_ud.size == 0 and s_ud.size == 0
+    return np.concatenate([h_ops, np.tile(s_ops, iters)]), h_ud
+
+
+def grover_job(n: int, seed: int, dtype) -> Job:
+    marked = int(W.splitmix64(seed, 1)[0] % np.uint64(1 << n))
+    ops, ud = grover_ops(n, marked)
+    return Job(f"grover{n}", n, dtype, ops, ud, meta={"marked": marked})
+
+
+def ae_job(seed: int, dtype, eval_qubits: int = 8) -> Job:
+    qc, a = W.amplitude_estimation(seed, eval_qubits)
+    ops, ud = lower_circuit(qc)
+    return Job(f"ae{qc.num_qubits}", qc.num_qubits, dtype, ops, ud, meta={"a": a})
+
+
+def batching_jobs(count: int, seed: int, lo: int = 10, hi: int = 22, dtype=np.float32,
+                  grover_max: int = 22, layers: int = 20) -> List[Job]:
+    """Batching workload: `count` circuits, qubit counts log-uniform in
+    [lo, hi]; half Grover (one seeded mark, floor(pi/4 sqrt(2^n))
+    iterations), half `layers`-layer random U3+CZ brick circuits."""
+    ns = W.loguniform_qubits(seed, count, lo, hi)
+    jobs = []
+    for i, n in enumerate(ns):
+        if i % 2 == 0:
+            jobs.append(grover_job(min(n, grover_max), seed * 7919 + i, dtype))
+        else:
+            jobs.append(layered_job(n, layers, seed * 104729 + i, dtype))
+    return jobs
+
+
+@dataclass
+class StreamSpec:
+    """Serving stream: Poisson arrivals at `rate` per second; mix 40 %
+    layered (depth 20-60), 30 % QFT on seeded random states, 20 % Grover,
+    10 % amplitude estimation (8 evaluation qubits); half complex128, half
+    complex64.  Qubit counts are drawn uniformly from [lo, hi] and capped
+    per family so that one request stays within a few seconds of GPU work
+    (Grover <= grover_max, layered <= layered_max); `big` adds one
+    `big_n`-qubit complex128 QFT to the stream."""
+
+    count: int = 200
+    rate: float = 200.0
+    lo: int = 12
+    hi: int = 32
+    grover_max: int = 16
+    layered_max: int = 26
+    qft_max: int = 30
+    seed: int = 5
+    big: bool = False
+    big_n: int = 33
+
+
+def serving_jobs(spec: StreamSpec) -> List[Job]:
+    u = W.uniforms(spec.seed, 8 * spec.count)
+    jobs: List[Job] = []
+    for i in range(spec.count):
+        r, rn, rd, rdepth = u[8 * i], u[8 * i + 1], u[8 * i + 2], u[8 * i + 3]
+        n = spec.lo + int(rn * (spec.hi - spec.lo + 1))
+        dtype = np.float64 if rd < 0.5 else np.float32
+        seed = spec.seed * 1000003 + i
+        if r < 0.4:
+            jobs.append(layered_job(min(n, spec.layered_max), 20 + int(rdepth * 41), seed, dtype))
+        elif r < 0.7:
+            jobs.append(qft_job(min(n, spec.qft_max), seed, dtype))
+        elif r < 0.9:
+            jobs.append(grover_job(min(n, spec.grover_max), seed, dtype))
+        else:
+            jobs.append(ae_job(seed, dtype))
+    if spec.big:
+        jobs.insert(len(jobs) // 2, qft_job(spec.big_n, spec.seed, np.float64))
+    return jobs
+
+
+def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
+    u = W.uniforms(seed + 17, count)
+    gaps = -np.log(1.0 - u) / rate
+    return np.cumsum(gaps)
+
+
+def percentile(xs: Sequence[float], q: float) -> float:
+    if not xs:
+        return float("nan")
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def run_stream(jobs: List[Job], arrivals: Optional[np.ndarray], engine: Engine, timeout: float = 600.0,
+               keep_results: bool = False) -> dict:
+    """Submit jobs (at arrival times if given, else all at once), wait for
+    all, and report throughput and latency (submit -> done)."""
+    t0 = time.perf_counter()
+    ids = []
+    for i, job in enumerate(jobs):
+        if arrivals is not None:
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+        ids.append(engine.submit(job))
+    lat, failed = [], []
+    results = []
+    for rid, job in zip(ids, jobs):
+        engine.wait(rid, timeout)
+        st, ts, ta, td = engine.status(rid)
+        if st != _lib.REQ_DONE:
+            failed.append((job.name, st, engine.error(rid) if st == _lib.REQ_FAILED else ""))
+            engine.release(rid)
+            continue
+        lat.append(td - ts)
+        if keep_results:
+            results.append(engine.fetch(rid, job.n, job.dtype))
+        else:
+            engine.release(rid)
+    wall = time.perf_counter() - t0
+    out = {"circuits": len(jobs), "completed": len(lat), "failed": failed, "wall_s": wall,
+           "circuits_per_s": len(lat) / wall if wall > 0 else float("nan"),
+           "p50_ms": percentile(lat, 50) * 1e3, "p99_ms": percentile(lat, 99) * 1e3,
+           "max_ms": (max(lat) * 1e3) if lat else float("nan"), "engine": engine.stats()}
+    if keep_results:
+        out["results"] = results
+    return out
+
+
+def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None) -> dict:
+    """One circuit per launch sequence: each circuit planned and executed
+    alone, one after the other (the reference's execute() call pattern)."""
+    t0 = time.perf_counter()
+    lat = []
+    for i, job in enumerate(jobs):
+        if arrivals is not None:
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+        ts = time.perf_counter()
+        g = GpuState(job.n, job.dtype)
+        if job.init_kind == _lib.INIT_RANDOM:
+            g.fill_random(job.init_seed)
+        elif job.init_kind == _lib.INIT_HOST:
+            g.upload(job.init)
+        p = Plan(job.n, job.dtype, job.ops, job.udata, _lib.opts())
+        p.execute(g)
+        g.synchronize()
+        t_done = time.perf_counter()
+        ref = ts if arrivals is None else t0 + arrivals[i]
+        lat.append(t_done - ref)
+        p.free()
+        g.free()
+    wall = time.perf_counter() - t0
+    return {"circuits": len(jobs), "wall_s": wall, "circuits_per_s": len(jobs) / wall,
+            "p50_ms": percentile(lat, 50) * 1e3, "p99_ms": percentile(lat, 99) * 1e3}

@@ -187,6 +187,10 @@ class GpuState:
     def scale(self, s: float) -> None:
         check(_lib.load().bfly_scale(self.handle, float(s)))
 
+    def fill_random(self, seed: int) -> None:
+        """re, im ~ N(0,1) from `seed` (SplitMix64 + Box-Muller), normalised."""
+        check(_lib.load().bfly_state_fill_random(self.handle, int(seed)))
+
 
 class Plan:
     """A compiled pass schedule for one circuit (bfly_plan)."""

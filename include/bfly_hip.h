@@ -43,6 +43,8 @@ extern "C" {
 #define BFLY_EHIP         -3  /* HIP runtime error */
 #define BFLY_EUNSUPPORTED -4  /* transformation the GPU strategy cannot lower */
 #define BFLY_ENUMERIC     -5  /* norm check failed (non-finite or drifted state) */
+#define BFLY_ETIMEDOUT    -6  /* bfly_engine_wait: the request did not finish in time */
+#define BFLY_ESTATE       -7  /* request in the wrong state for the call (not finished, cancelled, failed) */
 
 /* ---- amplitude precision ---------------------------------------------- */
 #define BFLY_C128 0           /* f64 re/im: reference default FloatType   types.mojo:7 */
@@ -168,6 +170,74 @@ int bfly_batch_launch(bfly_batch* b);
 int bfly_batch_synchronize(bfly_batch* b);
 int bfly_batch_stats(const bfly_batch* b, int64_t* rounds, int64_t* launches);
 int bfly_batch_destroy(bfly_batch* b);
+
+/* ---- serving engine: continuous batching with HBM-budget admission ------
+   No counterpart in the reference.  Requests (one circuit + initial state
+   each) are admitted first-fit against a budget of 16*2^n bytes (complex128)
+   or 8*2^n bytes (complex64) per resident state; admitted requests keep their
+   state in HBM; a worker thread issues rounds in which the next step of every
+   small running request joins one batched launch per tile size, while large
+   requests (n >= large_qubits) run their whole plan on their own stream and
+   overlap with the rounds.  Every result equals the request's result alone. */
+typedef struct bfly_engine bfly_engine;
+typedef struct bfly_engine_opts {
+    uint64_t budget_bytes;        /* 0 = 90 % of the device memory free at creation */
+    int32_t max_inflight_rounds;  /* rounds enqueued ahead of completion (0 = 3) */
+    int32_t large_qubits;         /* requests with n >= this run on their own stream (0 = 26) */
+    int32_t max_round_requests;   /* requests per round (0 = unlimited) */
+    int32_t flags;
+} bfly_engine_opts;
+
+#define BFLY_INIT_ZERO   0        /* |0..0> */
+#define BFLY_INIT_HOST   1        /* init_re / init_im, copied at submit */
+#define BFLY_INIT_RANDOM 2        /* re, im ~ N(0,1) from init_seed (SplitMix64 + Box-Muller), normalised */
+
+typedef struct bfly_request {
+    int32_t num_qubits;
+    int32_t dtype;
+    const bfly_op* ops;
+    uint64_t nops;
+    const double* udata;
+    uint64_t udata_len;
+    const void* init_re;
+    const void* init_im;
+    int32_t init_kind;            /* BFLY_INIT_* */
+    int32_t pad0;
+    uint64_t init_seed;
+    bfly_exec_opts exec;          /* planning options of this request */
+} bfly_request;
+
+#define BFLY_REQ_QUEUED    0
+#define BFLY_REQ_RUNNING   1
+#define BFLY_REQ_DONE      2
+#define BFLY_REQ_FAILED    3
+#define BFLY_REQ_CANCELLED 4
+
+int bfly_engine_create(bfly_engine** out, const bfly_engine_opts* opts);
+/* Plans the circuit (caller's thread) and queues it; *id names it afterwards.
+   A state larger than the whole budget is refused with BFLY_ENOMEM. */
+int bfly_engine_submit(bfly_engine* e, const bfly_request* r, uint64_t* id);
+/* Status and timestamps (seconds since engine creation; 0 when not reached). */
+int bfly_engine_status(bfly_engine* e, uint64_t id, int32_t* status, double* t_submit, double* t_admit,
+                       double* t_done);
+/* Block until the request is DONE / FAILED / CANCELLED or timeout_s passes
+   (BFLY_ETIMEDOUT); timeout_s < 0 waits forever. */
+int bfly_engine_wait(bfly_engine* e, uint64_t id, double timeout_s);
+/* Copy a DONE request's final state to the host and free its HBM. */
+int bfly_engine_fetch(bfly_engine* e, uint64_t id, void* re, void* im, uint64_t len);
+/* Free a finished request without copying it out. */
+int bfly_engine_release(bfly_engine* e, uint64_t id);
+/* Queued: dropped at once; running: no further steps are issued. */
+int bfly_engine_cancel(bfly_engine* e, uint64_t id);
+int bfly_engine_stats(bfly_engine* e, int64_t* admitted, int64_t* completed, int64_t* rounds, int64_t* launches,
+                      uint64_t* bytes_in_use, uint64_t* peak_bytes);
+/* Message of a FAILED request. */
+int bfly_engine_error(bfly_engine* e, uint64_t id, char* buf, uint64_t len);
+int bfly_engine_destroy(bfly_engine* e);
+
+/* Fill a state with re, im ~ N(0,1) from `seed` (SplitMix64 counter stream,
+   Box-Muller in FP64) and normalise it with the compensated norm. */
+int bfly_state_fill_random(bfly_state* s, uint64_t seed);
 
 /* ---- permutations: replaces bit_reverse_state / partial_bit_reverse_state
         (state.mojo:131-204,252-301) and swap_bits_state (state.mojo:304-317) -- */
