@@ -77,6 +77,87 @@ def cpu_baseline(n: int, layers: int, seed: int, budget_s: float, threads: int) 
     }
 
 
+def bench_batching(args) -> int:
+    """4096 independent complex64 circuits, qubit counts log-uniform in
+    [10, 22]: half Grover search (one seeded mark), half 20-layer random
+    U3 + CZ brick circuits.  Timed: one hipGraph launch running every round
+    of the batch (states reset between timed runs), against the same
+    circuits executed one after another.  Every 8th final state is compared
+    with a complex128 run of its circuit."""
+    from code_mojo_amd import serve
+
+    count = args.circuits or 4096
+    t0 = time.perf_counter()
+    jobs = serve.batching_jobs(count, args.seed)
+    log(f"[bench] {count} batching circuits built in {time.perf_counter() - t0:.1f}s")
+    res = serve.bench_batching(jobs, graph=True, repeats=max(args.steps, 1), check=True, log=log, check_every=8)
+    secs = res["batch_s"]
+    out = {
+        "metric": "circuits/s, batched persistent launches (complex64, 10-22 qubits)",
+        "value": round(res["circuits_per_s"], 2), "unit": "circuits/s", "n_gpus": 1,
+        "steps": max(args.steps, 1), "warmup": 1, "ms_per_step": round(secs * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Grover and random layered circuits)",
+        "config": {"workload": "batching_grover_layered", "circuits": count, "qubits": "10-22 log-uniform",
+                   "seed": args.seed, "graph": True, "tile_passes": res["passes"], "rounds": res["rounds"],
+                   "launches": res["launches"]},
+        "one_circuit_per_launch_circuits_per_s": round(res["sequential_circuits_per_s"], 2),
+        "speedup_vs_one_circuit_per_launch": round(res["speedup"], 3),
+        "max_l2_vs_c128": res.get("max_l2_vs_c128"), "min_fidelity_vs_c128": res.get("min_fidelity_vs_c128"),
+        "checked_vs_c128": res.get("checked_vs_c128"),
+        "max_amp_err_batch_vs_alone": res.get("max_amp_err_vs_seq"),
+        "roofline": {"bound": "hbm", "achieved": round(res["algorithmic_bytes"] / secs / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(res["algorithmic_bytes"] / secs / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batch (every launch of the graph)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_serving(args) -> int:
+    """A seeded Poisson stream (40 % layered, 30 % QFT on random states,
+    20 % Grover, 10 % amplitude estimation; half complex128, half complex64)
+    through the serving engine, and the same stream one circuit at a time.
+    Reported: throughput with every request submitted at once (capacity), and
+    p50 / p99 latency at the stated arrival rate."""
+    from code_mojo_amd import serve
+
+    spec = serve.StreamSpec(count=args.circuits or 200, rate=args.rate, seed=args.seed % 1000 + 5, big=args.big)
+    t0 = time.perf_counter()
+    jobs = serve.serving_jobs(spec)
+    log(f"[bench] {len(jobs)} serving requests built in {time.perf_counter() - t0:.1f}s")
+    arrivals = serve.poisson_arrivals(len(jobs), spec.rate, spec.seed)
+    sat = serve.bench_serving(jobs, None, log=log)
+    live = serve.bench_serving(jobs, arrivals, log=log)
+    eng = sat["engine"]
+    out = {
+        "metric": "circuits/s through the serving engine (saturated), p99 latency at the stated rate",
+        "value": round(eng["circuits_per_s"], 2), "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(eng["wall_s"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": "serving_stream", "requests": len(jobs), "rate_per_s": spec.rate,
+                   "qubits": f"{spec.lo}-{spec.hi}", "caps": {"grover": spec.grover_max,
+                                                              "layered": spec.layered_max, "qft": spec.qft_max},
+                   "big_qft_qubits": spec.big_n if spec.big else None, "seed": spec.seed},
+        "one_circuit_per_launch_circuits_per_s": round(sat["sequential"]["circuits_per_s"], 2),
+        "speedup_vs_one_circuit_per_launch": round(sat["speedup"], 3),
+        "saturated": {"p50_ms": round(eng["p50_ms"], 3), "p99_ms": round(eng["p99_ms"], 3),
+                      "failed": len(eng["failed"])},
+        "at_rate": {"rate_per_s": spec.rate,
+                    "engine": {k: round(live["engine"][k], 3) for k in ("circuits_per_s", "p50_ms", "p99_ms", "max_ms")},
+                    "one_circuit_per_launch": {k: round(live["sequential"][k], 3)
+                                               for k in ("circuits_per_s", "p50_ms", "p99_ms")}},
+        "engine_stats": eng["engine"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +174,14 @@ def main() -> int:
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["layered", "batching", "serving"], default="layered",
+                    help="layered: the headline state-GB/s workload (default); batching: many independent "
+                         "circuits in batched persistent launches; serving: a Poisson request stream "
+                         "through the serving engine")
+    ap.add_argument("--circuits", type=int, default=0, help="batching / serving: request count "
+                    "(default 4096 / 200)")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--big", action="store_true", help="serving: add one 33-qubit complex128 QFT")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only copy one N-qubit complex128 state device to device (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
@@ -119,6 +208,10 @@ def main() -> int:
     if lib.bfly_device_count() < 1:
         log("no GPU visible")
         return 1
+    if args.workload == "batching":
+        return bench_batching(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     n = args.qubits
     dtype = np.float64 if args.dtype == "c128" else np.float32
     b = 16 if args.dtype == "c128" else 8

@@ -7,11 +7,11 @@ ExecutionStrategy.GPU.  All amplitude work happens in libbfly_hip.so
 (hand-written HIP kernels behind a C ABI, include/bfly_hip.h); this Python
 layer only builds circuits, lowers them to the op array and calls the ABI.
 """
-from .circuit import (Circuit, ClassicalTransformation, ControlKind, ControlledUnitaryTransformation,
+from .circuit import (Circuit, ClassicalReplacementKind, ClassicalTransformation, ControlKind, ControlledUnitaryTransformation,
                       FusedPairTransformation, GateTransformation, MeasurementTransformation,
                       QuantumCircuit, QuantumRegister, QubitReversalTransformation, Register,
                       SwapTransformation, UnitaryTransformation, bit_reverse, measure,
-                      permute_qubits)
+                      permute_qubits, replace_quantum_with_classical)
 from .context import ExecContext, ExecutionStrategy, execution_strategy_name
 from .executors import compile_plan, execute, execute_gpu, executor_registry, get_executor
 from .gates import (Custom_Gate, GateInfo, GateKind, H, P, P_Gate, RX, RX_Gate, RY, RY_Gate, RZ,

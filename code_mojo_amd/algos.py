@@ -9,8 +9,8 @@ from __future__ import annotations
 import math
 from typing import List, Optional, Sequence
 
-from .circuit import (Circuit, ClassicalTransformation, QuantumCircuit, QuantumRegister, Register,
-                      bit_reverse)
+from .circuit import (Circuit, ClassicalReplacementKind, ClassicalTransformation, QuantumCircuit,
+                      QuantumRegister, Register, bit_reverse, replace_quantum_with_classical)
 from .gates import pi
 
 
@@ -113,6 +113,29 @@ def grover_iterate_circuit(oracle, num_qubits: int) -> Circuit:
         qc.add_classical(oracle.name, oracle.targets, oracle.apply)
     qc.append_circuit(diffuser_circuit(num_qubits), QuantumRegister("q", num_qubits))
     return qc
+
+
+def grover_iterate_circuit_classical_diffuser(oracle, num_qubits: int) -> Circuit:
+    """grover_iterate_circuit with the diffuser rewritten into the classical
+    GROVER_DIFFUSER transform (grover.mojo:122-133)."""
+    qc = grover_iterate_circuit(oracle, num_qubits)
+    replace_quantum_with_classical(qc, [ClassicalReplacementKind.GROVER_DIFFUSER])
+    return qc
+
+
+def append_grover_to_register(qc: Circuit, register: Register, oracle, iterations: int) -> None:
+    """Oracle + diffuser `iterations` times on `register` (grover.mojo:167-180)."""
+    step = grover_iterate_circuit(oracle, register.length)
+    for _ in range(iterations):
+        qc.append_circuit(step, register)
+
+
+def append_grover_to_register_classical_diffuser(qc: Circuit, register: Register, oracle,
+                                                 iterations: int) -> None:
+    """The same with the classical diffuser (grover.mojo:183-197)."""
+    step = grover_iterate_circuit_classical_diffuser(oracle, register.length)
+    for _ in range(iterations):
+        qc.append_circuit(step, register)
 
 
 def grover_circuit(items: Sequence[int], num_qubits: int, iterations: int, use_shortcut: bool = False) -> Circuit:

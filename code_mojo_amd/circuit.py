@@ -8,8 +8,9 @@ op array lives in lowering.py.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
-from typing import Callable, List, Optional, Sequence, Union
+from typing import Callable, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -410,3 +411,120 @@ def permute_qubits(circuit: Circuit, order: Sequence[int]) -> None:
 
 def measure(circuit: Circuit, targets: Sequence[int], values=None, seed: Optional[int] = None) -> None:
     circuit.transformations.append(MeasurementTransformation(list(targets), seed, list(values or [])))
+
+
+# -- classical replacement rules (quantum_circuit.mojo:33-75, 185-495) ---------
+class ClassicalReplacementKind:
+    """Which quantum sub-sequences `replace_quantum_with_classical` rewrites
+    (butterfly/core/quantum_circuit.mojo:33-75)."""
+
+    SWAP = 0
+    QUBIT_REVERSAL = 1
+    GROVER_DIFFUSER = 2
+    GROVER_INVERSION_0 = 3
+    names = {0: "SWAP", 1: "QUBIT_REVERSAL", 2: "GROVER_DIFFUSER", 3: "GROVER_INVERSION_0"}
+
+
+def _plain(tr, kind: int, target: int) -> bool:
+    return (isinstance(tr, GateTransformation) and not tr.controls and tr.gate_info.kind == kind
+            and tr.target == target)
+
+
+def _mcp_pi(tr, base: int, n: int) -> bool:
+    """P(pi) on base+n-1 controlled by base .. base+n-2, in that order."""
+    return (isinstance(tr, GateTransformation) and tr.gate_info.kind == GateKind.P
+            and tr.gate_info.arg is not None and tr.gate_info.arg == math.pi
+            and tr.target == base + n - 1 and list(tr.controls) == list(range(base, base + n - 1)))
+
+
+def _rule_swap(trs, i, out) -> int:
+    tr = trs[i]
+    if not isinstance(tr, SwapTransformation):
+        return 0
+    out.append(swap_tr(tr.a, tr.b))
+    return 1
+
+
+def _rule_qrev(trs, i, out) -> int:
+    tr = trs[i]
+    if not isinstance(tr, QubitReversalTransformation):
+        return 0
+    out.append(bit_reverse_tr(tr.targets if tr.targets else None))
+    return 1
+
+
+def _run(trs, i, kind) -> Tuple[int, int]:
+    """Length and base of the run kind(base), kind(base+1), ... at i."""
+    first = trs[i]
+    if not (isinstance(first, GateTransformation) and not first.controls and first.gate_info.kind == kind):
+        return 0, 0
+    base, n = first.target, 0
+    while i + n < len(trs) and _plain(trs[i + n], kind, base + n):
+        n += 1
+    return n, base
+
+
+def _rule_diffuser(trs, i, out) -> int:
+    """H(all) X(all) MCP(pi) X(all) H(all) on base .. base+n-1."""
+    n, base = _run(trs, i, GateKind.H)
+    if n == 0 or i + 4 * n >= len(trs):
+        return 0
+    if not all(_plain(trs[i + n + j], GateKind.X, base + j) for j in range(n)):
+        return 0
+    if not _mcp_pi(trs[i + 2 * n], base, n):
+        return 0
+    if not all(_plain(trs[i + 2 * n + 1 + j], GateKind.X, base + j) for j in range(n)):
+        return 0
+    if not all(_plain(trs[i + 3 * n + 1 + j], GateKind.H, base + j) for j in range(n)):
+        return 0
+    out.append(ClassicalTransformation("GROVER_DIFFUSER", list(range(base, base + n))))
+    return 4 * n + 1
+
+
+def _rule_inversion_0(trs, i, out) -> int:
+    """X(all) MCP(pi) X(all) on base .. base+n-1."""
+    n, base = _run(trs, i, GateKind.X)
+    if n == 0 or i + 2 * n >= len(trs):
+        return 0
+    if not _mcp_pi(trs[i + n], base, n):
+        return 0
+    if not all(_plain(trs[i + n + 1 + j], GateKind.X, base + j) for j in range(n)):
+        return 0
+    out.append(ClassicalTransformation("GROVER_INVERSION_0", list(range(base, base + n))))
+    return 2 * n + 1
+
+
+_RULES = {ClassicalReplacementKind.SWAP: _rule_swap, ClassicalReplacementKind.QUBIT_REVERSAL: _rule_qrev,
+          ClassicalReplacementKind.GROVER_DIFFUSER: _rule_diffuser,
+          ClassicalReplacementKind.GROVER_INVERSION_0: _rule_inversion_0}
+
+
+def replace_quantum_with_classical(circuit: Circuit, kinds: Optional[Sequence[int]] = None) -> int:
+    """Rewrite matching transformation runs into classical transforms, first
+    matching rule wins at each position; returns the number of rewrites
+    (apply_classical_rewrites, quantum_circuit.mojo:185-226).  Without
+    `kinds` the default rules are SWAP and QUBIT_REVERSAL (:229-233)."""
+    if kinds is None:
+        kinds = [ClassicalReplacementKind.SWAP, ClassicalReplacementKind.QUBIT_REVERSAL]
+    rules = []
+    for k in kinds:
+        if k not in _RULES:
+            raise ValueError(f"Unknown classical replacement kind: {k}")
+        rules.append(_RULES[k])
+    trs = circuit.transformations
+    out: List = []
+    i, replaced = 0, 0
+    while i < len(trs):
+        consumed = 0
+        for rule in rules:
+            consumed = rule(trs, i, out)
+            if consumed > 0:
+                replaced += 1
+                break
+        if consumed <= 0:
+            out.append(trs[i])
+            i += 1
+        else:
+            i += consumed
+    circuit.transformations = out
+    return replaced

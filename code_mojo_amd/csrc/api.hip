@@ -41,9 +41,9 @@ namespace {
 size_t elem_size(int dtype) { return dtype == BFLY_C64 ? sizeof(float) : sizeof(double); }
 
 // ---- profiling ------------------------------------------------------------
-enum Family { F_TILE = 0, F_GATE, F_BITREV, F_SWAP, F_UNITARY, F_REDUCE, F_NFAM };
+enum Family { F_TILE = 0, F_GATE, F_BITREV, F_SWAP, F_UNITARY, F_REDUCE, F_FLIP, F_DIFFUSE, F_NFAM };
 const char* kFamilyNames[F_NFAM] = {"tile_pass_kernel", "gate_kernel", "bitrev_kernel", "swap_kernel",
-                                    "unitary_kernel", "reduce_kernel"};
+                                    "unitary_kernel", "reduce_kernel", "flip_kernel", "diffuse_kernels"};
 std::mutex g_prof_mu;
 int64_t g_prof_launches[F_NFAM] = {0};
 double g_prof_ms[F_NFAM] = {0};
@@ -318,9 +318,9 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
         std::snprintf(line, sizeof line, fmt, args...);
         out += line;
     };
-    auto put_m = [&](const double* m) {
+    auto put_m = [&](const double* m, int count = 8) {
         out += " m=";
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < count; ++i) {
             std::snprintf(line, sizeof line, i ? ",%a" : "%a", m[i]);
             out += line;
         }
@@ -333,7 +333,7 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
         switch (st.kind) {
             case S_TILE: {
                 const TilePass& tp = st.tile;
-                put("tile k=%d sweeps=%d qpos=", tp.k, tp.nsweeps);
+                put("tile k=%d slots=%d sweeps=%d qpos=", tp.k, tp.slots, tp.nsweeps);
                 for (int i = 0; i < tp.k; ++i) put(i ? ",%d" : "%d", tp.qpos[i]);
                 out += "\n";
                 for (int s = 0; s < tp.nsweeps; ++s) {
@@ -341,11 +341,14 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                     put("sweep s=%d,%d,%d,%d ninstr=%d\n", sw.s[0], sw.s[1], sw.s[2], sw.s[3], sw.ninstr);
                     for (int q = 0; q < sw.ninstr; ++q) {
                         const TileInstr& in = hp.instrs[sw.first + q];
-                        put("instr kind=%d slot=%d cslots=%u lmask=%u dlocal=%d mm1=%u mm0=%u omask=%llu obit=%llu",
-                            in.kind, in.slot, in.cslots, in.lmask, in.dlocal, static_cast<unsigned>(in.mm1),
+                        put("instr kind=%d slot=%d cslots=%u lmask=%u lval=%u dlocal=%d mm1=%u mm0=%u omask=%llu "
+                            "oval=%llu obit=%llu",
+                            int(in.kind), int(in.slot), unsigned(in.cslots), in.lmask, in.lval, int(in.dlocal),
+                            static_cast<unsigned>(in.mm1),
                             static_cast<unsigned>(in.mm0), static_cast<unsigned long long>(in.omask),
-                            static_cast<unsigned long long>(in.obit));
-                        put_m(in.m);
+                            static_cast<unsigned long long>(in.oval),
+                            static_cast<unsigned long long>(in.obitpos >= 0 ? 1ull << in.obitpos : 0ull));
+                        put_m(in.m, 16);
                     }
                 }
                 break;
@@ -360,6 +363,13 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
             case S_UNITARY:
                 put("unitary target=%d m=%d controls=%llu uoff=%llu\n", st.a, st.b,
                     static_cast<unsigned long long>(st.controls), static_cast<unsigned long long>(st.uoff));
+                break;
+            case S_FLIP:
+                put("flip mask=%llu value=%llu\n", static_cast<unsigned long long>(st.mask),
+                    static_cast<unsigned long long>(st.controls));
+                break;
+            case S_DIFFUSE:
+                put("diffuse mask=%llu variant=%d\n", static_cast<unsigned long long>(st.mask), st.a);
                 break;
         }
     }
@@ -405,6 +415,14 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
                 prof.begin(F_UNITARY);
                 e = launch_unitary(s->dtype, s->re, s->im, s->n, st.a, st.b, st.controls, p->d_udata + 2 * st.uoff,
                                    s->stream);
+                break;
+            case S_FLIP:
+                prof.begin(F_FLIP);
+                e = launch_flip(s->dtype, s->re, s->im, s->n, st.mask, st.controls, s->stream);
+                break;
+            case S_DIFFUSE:
+                prof.begin(F_DIFFUSE);
+                e = launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, s->stream);
                 break;
         }
         prof.end();
@@ -454,7 +472,7 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
 struct bfly_batch {
     struct Launch {
         int kind;              // S_TILE: grouped tile launch; otherwise one step of one state
-        int dtype = 0, k = 0;
+        int dtype = 0, k = 0, slots = 0;
         size_t item_off = 0;   // S_TILE: first item in d_items
         int nitems = 0;
         uint64_t total_tiles = 0;
@@ -485,6 +503,8 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
         case S_UNITARY:
             return launch_unitary(s->dtype, s->re, s->im, s->n, st.a, st.b, st.controls, p->d_udata + 2 * st.uoff,
                                   stream);
+        case S_FLIP: return launch_flip(s->dtype, s->re, s->im, s->n, st.mask, st.controls, stream);
+        case S_DIFFUSE: return launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, stream);
     }
     return hipErrorInvalidValue;
 }
@@ -496,7 +516,8 @@ namespace {
 hipError_t enqueue_batch(bfly_batch* b) {
     for (const auto& L : b->launches) {
         hipError_t e;
-        if (L.kind == S_TILE) e = launch_tile_batch(L.dtype, L.k, b->d_items + L.item_off, L.nitems, L.total_tiles, b->stream);
+        if (L.kind == S_TILE)
+            e = launch_tile_batch(L.dtype, L.k, L.slots, b->d_items + L.item_off, L.nitems, L.total_tiles, b->stream);
         else e = launch_step(*L.step, L.s, L.p, b->stream);
         if (e != hipSuccess) return e;
     }
@@ -544,15 +565,15 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
         if (rc) { free_batch(b); return rc; }
         max_steps = std::max(max_steps, plans[i]->hp.steps.size());
     }
-    // rounds: step r of every plan; tile steps grouped by (dtype, k)
+    // rounds: step r of every plan; tile steps grouped by (dtype, sweep width, k)
     for (size_t r = 0; r < max_steps; ++r) {
-        std::vector<std::pair<int, std::vector<int>>> groups;  // key = dtype * 64 + k
+        std::vector<std::pair<int, std::vector<int>>> groups;  // key = dtype * 1024 + slots * 64 + k
         for (int i = 0; i < count; ++i) {
             const auto& steps = plans[i]->hp.steps;
             if (r >= steps.size()) continue;
             const Step& st = steps[r];
             if (st.kind == S_TILE) {
-                const int key = states[i]->dtype * 64 + st.tile.k;
+                const int key = states[i]->dtype * 1024 + st.tile.slots * 64 + st.tile.k;
                 auto it = std::find_if(groups.begin(), groups.end(), [&](auto& g) { return g.first == key; });
                 if (it == groups.end()) groups.push_back({key, {i}});
                 else it->second.push_back(i);
@@ -568,7 +589,8 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
         for (auto& g : groups) {
             bfly_batch::Launch L{};
             L.kind = S_TILE;
-            L.dtype = g.first / 64;
+            L.dtype = g.first / 1024;
+            L.slots = (g.first / 64) % 16;
             L.k = g.first % 64;
             L.item_off = b->items.size();
             uint64_t tiles = 0;
@@ -586,7 +608,7 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
             }
             L.nitems = static_cast<int>(g.second.size());
             L.total_tiles = tiles;
-            e = prepare_tile_kernels(L.dtype, L.k);
+            e = prepare_tile_kernels(L.dtype, L.k, L.slots);
             if (e != hipSuccess) { free_batch(b); return hip_fail(e, "tile kernel preparation"); }
             b->launches.push_back(L);
         }

@@ -14,6 +14,8 @@ namespace bfly {
 constexpr int kMaxTileBits = 13;
 constexpr int kMinTileBits = 4;
 constexpr int kMaxQubits = 40;
+// Widest diffuser block handled by one workgroup (wider ones must cover every qubit).
+constexpr int kDiffuseMaxBits = 12;
 
 // Gate application kinds inside a tile sweep or a stand-alone gate launch.
 // The specialised kinds reproduce the reference's specialised scalar kernels
@@ -33,37 +35,58 @@ enum InstrKind : int32_t {
 // signs (Z family, exact) or multiply by a phase.
 enum TileKind : int32_t {
     T_PAIR = 0,    // m[0..7]: 2x2 matrix on slot `slot`
-    T_NEG = 1,     // negate the members selected by mm1 (and the target bit)
+    T_NEG = 1,     // negate the members selected by mm1 (target bit 1) or mm0 (target bit 0;
+                   // phase flips whose pattern has a 0 at the target)
     T_DIAG = 2,    // multiply members selected by mm1 by d1 = m[2..3], by mm0 by d0 = m[0..1]
+    // Factored single-qubit unitaries (no controls): U = sigma * PH(e^{i phi}) *
+    // R * PH(e^{i lambda}) with R real; the pass's sigmas are applied by T_SCALE.
+    T_ROT = 3,     // on slot `slot`: v <- v * m[2..3] (pre-phase), then the scaled real
+                   // rotation with t = m[0], form = m[1]:
+                   // form 0: (u, v) <- (u - t v, v + t u); form 1: (t u - v, u + t v)
+    T_PH = 4,      // members whose slot bit is 1 times m[0..1]
+    T_SCALE = 5,   // every member times m[0..1]
+    T_ROTS = 6,    // (3-slot sweeps) T_ROT on every slot s in the mask `cslots`, in slot order (they
+                   // commute): m[3s] = t (negative, or -0.0, for form 1),
+                   // m[3s + 1 .. 3s + 2] = pre-phase
 };
 
-// Amplitudes per thread in a tile sweep: 2^kSlots, spanned by kSlots local bits.
-constexpr int kSlots = 4;
-constexpr int kGroup = 1 << kSlots;
+// Amplitudes per thread in a tile sweep: 2^S, spanned by S local bits (the
+// slots); S = 4 (16 amplitudes) or 3 (8 amplitudes, twice the threads per
+// tile, tiles of 2^10 amplitudes or more).
+constexpr int kSlots = 4;            // widest sweep
+constexpr int kMinSlots3Bits = 10;   // smallest tile with 3-slot sweeps
+constexpr int kGroup = 1 << kSlots;  // member masks are kGroup bits wide
 
-// One gate inside a tile pass.  144 bytes.
+// One gate inside a tile pass.  240 bytes; the first 20 bytes and the
+// parameters are what every sweep reads (the outer-control words only when
+// `guard` says so).
 struct alignas(16) TileInstr {
-    int32_t kind;      // TileKind
-    int32_t slot;      // target's index in the sweep's slot set, or -1 (diagonal
+    int8_t kind;       // TileKind
+    int8_t slot;       // target's index in the sweep's slot set, or -1 (diagonal
                        // ops only: target outside the slots)
-    uint32_t cslots;   // control bits that are slots, as a mask over slot indices
-    uint32_t lmask;    // control bits local to the tile but not slots (local-index bits)
-    int32_t dlocal;    // diagonal op with target local but not a slot: its local bit, else -1
-    uint16_t mm1;      // diagonal ops: members (of 16) whose slot controls hold and whose
+    uint8_t cslots;    // control bits that are slots, as a mask over slot indices
+    int8_t dlocal;     // diagonal op with target local but not a slot: its local bit, else -1
+    int8_t obitpos;    // diagonal op with target outside the tile: its global bit, else -1
+    uint8_t guard;     // 1: outer controls (omask / oval) must be tested
+    uint16_t pad0;
+    uint16_t mm1;      // diagonal ops: members whose slot controls hold and whose
                        // target bit is 1 (every slot-control member when the target is no slot)
     uint16_t mm0;      // diagonal ops: the same with target bit 0
+    uint32_t lmask;    // control bits local to the tile but not slots (local-index bits)
+    uint32_t lval;     // required values of the lmask bits (lmask for ordinary controls)
+    uint32_t pad1[3];
     uint64_t omask;    // control bits outside the tile, as global-index bits
-    uint64_t obit;     // diagonal op with target outside the tile: its global bit (else 0)
-    double m[8];       // matrix / parameters in FP64
-    float mf[8];       // the same in FP32 (complex64 states)
+    uint64_t oval;     // required values of the omask bits (omask for ordinary controls)
+    double m[16];      // matrix / parameters in FP64 (T_ROTS uses 12)
+    float mf[16];      // the same in FP32 (complex64 states)
 };
-static_assert(sizeof(TileInstr) == 144, "TileInstr layout");
+static_assert(sizeof(TileInstr) == 240, "TileInstr layout");
 
 // A sweep: every group of 16 amplitudes spanned by the kSlots local bits
 // `s` is loaded from LDS into registers, all the sweep's gates run in
 // registers, and the group is written back.  32 bytes.
 struct alignas(16) TileSweep {
-    int32_t s[kSlots]; // local bit positions, distinct
+    int32_t s[kSlots]; // local bit positions, distinct (the first S are used)
     int32_t ninstr;
     int32_t first;     // index of the first instruction
     int32_t pad[2];
@@ -74,6 +97,7 @@ static_assert(sizeof(TileSweep) == 32, "TileSweep layout");
 struct TilePass {
     int32_t n;
     int32_t k;          // tile bits
+    int32_t slots;      // sweep width S (3 or 4)
     int32_t nsweeps;
     int32_t first_sweep;
     int32_t qpos[kMaxTileBits];   // global bit of local bit i (ascending)
@@ -93,15 +117,15 @@ struct GateLaunch {
     float mf[8];
 };
 
-enum StepKind : int32_t { S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4 };
+enum StepKind : int32_t { S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4, S_FLIP = 5, S_DIFFUSE = 6 };
 
 struct Step {
     StepKind kind;
     TilePass tile;       // S_TILE
     GateLaunch gate;     // S_GATE
-    uint64_t mask = 0;   // S_BITREV: target mask (0 = all)
-    int32_t a = 0, b = 0;  // S_SWAP qubits; S_UNITARY: a = target, b = m
-    uint64_t controls = 0; // S_UNITARY control mask
+    uint64_t mask = 0;   // S_BITREV: target mask (0 = all); S_FLIP / S_DIFFUSE: bit mask
+    int32_t a = 0, b = 0;  // S_SWAP qubits; S_UNITARY: a = target, b = m; S_DIFFUSE: a = variant
+    uint64_t controls = 0; // S_UNITARY control mask; S_FLIP: pattern
     uint64_t uoff = 0;     // S_UNITARY: complex offset into the plan's unitary data
 };
 
@@ -128,5 +152,6 @@ int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_le
 // Default tile / low-bit choices for a dtype (also used by the autotuner).
 int default_tile_bits(int32_t dtype);
 int default_low_bits(int32_t dtype);
+int default_sweep_slots(int32_t dtype, int k);
 
 }  // namespace bfly

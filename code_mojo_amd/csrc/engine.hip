@@ -200,7 +200,7 @@ int issue_round(bfly_engine* e) {
         }
         const Step& st = steps[r->next_step];
         if (st.kind == S_TILE) {
-            const int key = r->dtype * 64 + st.tile.k;
+            const int key = r->dtype * 1024 + st.tile.slots * 64 + st.tile.k;
             auto g = std::find_if(groups.begin(), groups.end(), [&](const Group& x) { return x.key == key; });
             if (g == groups.end()) groups.push_back({key, {r}});
             else g->reqs.push_back(r);
@@ -219,7 +219,7 @@ int issue_round(bfly_engine* e) {
         BatchItem* hi = e->h_items + static_cast<size_t>(s) * kSlotItems * 4;
         BatchItem* di = e->d_items + static_cast<size_t>(s) * kSlotItems * 4;
         size_t used = 0;
-        struct L { int dtype, k; size_t off; int n; uint64_t tiles; };
+        struct L { int dtype, slots, k; size_t off; int n; uint64_t tiles; };
         std::vector<L> ls;
         for (auto& g : groups) {
             for (size_t base = 0; base < g.reqs.size(); base += kSlotItems) {
@@ -238,14 +238,14 @@ int issue_round(bfly_engine* e) {
                     it.P = st.tile;
                     tiles += 1ull << (st.tile.n - st.tile.k);
                 }
-                ls.push_back({g.key / 64, g.key % 64, used, static_cast<int>(cnt), tiles});
+                ls.push_back({g.key / 1024, (g.key / 64) % 16, g.key % 64, used, static_cast<int>(cnt), tiles});
                 used += cnt;
             }
         }
         HIP_TRY(hipMemcpyAsync(di, hi, used * sizeof(BatchItem), hipMemcpyHostToDevice, e->stream), "round items");
         for (const L& l : ls) {
-            HIP_TRY(prepare_tile_kernels(l.dtype, l.k), "tile kernel preparation");
-            HIP_TRY(launch_tile_batch(l.dtype, l.k, di + l.off, l.n, l.tiles, e->stream), "batched launch");
+            HIP_TRY(prepare_tile_kernels(l.dtype, l.k, l.slots), "tile kernel preparation");
+            HIP_TRY(launch_tile_batch(l.dtype, l.k, l.slots, di + l.off, l.n, l.tiles, e->stream), "batched launch");
             e->launches += 1;
         }
         HIP_TRY(hipEventRecord(e->slot_ev[s], e->stream), "hipEventRecord");

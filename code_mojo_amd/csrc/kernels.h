@@ -21,13 +21,13 @@ struct BatchItem {
     TilePass P;
 };
 // (b') the next tile pass of many circuits in one persistent launch; all
-// items share the tile size k and the dtype.  d_items == nullptr only
+// items share the tile size k, the sweep width and the dtype.  d_items == nullptr only
 // prepares the kernel (attributes, occupancy) without launching.
-hipError_t launch_tile_batch(int dtype, int k, const BatchItem* d_items, int nitems, uint64_t total_tiles,
+hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_items, int nitems, uint64_t total_tiles,
                              hipStream_t stream);
 // Sets kernel attributes and sizes the persistent grids of the tile kernels
-// for (dtype, k) outside any stream capture.
-hipError_t prepare_tile_kernels(int dtype, int k);
+// for (dtype, k, slots) outside any stream capture.
+hipError_t prepare_tile_kernels(int dtype, int k, int slots);
 // (a) one gate over the whole state (coalesced strided butterfly pairs).
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream);
 // (c) bit reversal of all bits (mask 0) or of the bits in `mask`, in place.
@@ -41,9 +41,16 @@ hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double sca
 // re, im ~ N(0,1) from a SplitMix64 stream (not normalised).
 hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint64_t seed, hipStream_t stream);
 
+// PHASE_FLIP: negate every amplitude with (i & mask) == value.
+hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint64_t value, hipStream_t stream);
+// DIFFUSE over the bits of `mask` (every qubit, or at most kDiffuseMaxBits);
+// uses `scratch` (reduce_scratch_doubles()) when the mask covers every qubit.
+hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, int variant, double* scratch,
+                          hipStream_t stream);
+
 // (d) reductions.  Partials live in `scratch` (at least reduce_scratch_doubles()
 // doubles); the final value(s) are left in scratch[0..nout) on the device.
-enum ReduceMode : int { R_NORM2 = 0, R_PROB_MASK = 1, R_DIFF2 = 2, R_INNER = 3 };
+enum ReduceMode : int { R_NORM2 = 0, R_PROB_MASK = 1, R_DIFF2 = 2, R_INNER = 3, R_SUM = 4 };
 size_t reduce_scratch_doubles();
 hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre,
                          const void* bim, uint64_t size, uint64_t mask, uint64_t value,

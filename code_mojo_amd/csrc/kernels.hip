@@ -163,24 +163,24 @@ __device__ __forceinline__ uint64_t local_to_global(uint32_t j, const TilePass& 
 // (m0, m0 | 2^SLOT).  Pairs whose slot controls do not hold take the
 // identity; the test is on the member index only, so the selection is scalar
 // and no register array is copied.
-template <typename T, int SLOT>
+template <typename T, int G, int SLOT>
 __device__ __forceinline__ void slot_pairs_all(T* ar, T* ai, const T* g) {
 #pragma unroll
-    for (int p = 0; p < kGroup / 2; ++p) {
+    for (int p = 0; p < G / 2; ++p) {
         constexpr int lo_mask = (1 << SLOT) - 1;
         const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
         pair_gen(ar[m0], ai[m0], ar[m0 | (1 << SLOT)], ai[m0 | (1 << SLOT)], g);
     }
 }
 
-template <typename T, int SLOT>
+template <typename T, int G, int SLOT>
 __device__ __forceinline__ void slot_pairs(T* ar, T* ai, uint32_t cslots, const T* g) {
     if (cslots == 0) {  // no slot controls: every pair, no selection
-        slot_pairs_all<T, SLOT>(ar, ai, g);
+        slot_pairs_all<T, G, SLOT>(ar, ai, g);
         return;
     }
 #pragma unroll
-    for (int p = 0; p < kGroup / 2; ++p) {
+    for (int p = 0; p < G / 2; ++p) {
         constexpr int lo_mask = (1 << SLOT) - 1;
         const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
         const int m1 = m0 | (1 << SLOT);
@@ -211,10 +211,10 @@ __device__ __forceinline__ void flip_sign(float& x, uint32_t on) {
 
 // Z family: flip the sign bit of the members selected in sel1 (exactly the
 // product by -1).
-template <typename T>
+template <typename T, int G>
 __device__ __forceinline__ void neg_apply(T* ar, T* ai, uint32_t sel1) {
 #pragma unroll
-    for (int m = 0; m < kGroup; ++m) {
+    for (int m = 0; m < G; ++m) {
         const uint32_t on = (sel1 >> m) & 1u;
         flip_sign(ar[m], on);
         flip_sign(ai[m], on);
@@ -223,10 +223,10 @@ __device__ __forceinline__ void neg_apply(T* ar, T* ai, uint32_t sel1) {
 
 // Phase family: members in sel1 are multiplied by d1, in sel0 by d0, the
 // others keep their value (factor 1 + 0i, exact).
-template <typename T>
+template <typename T, int G>
 __device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t sel0, const T* g) {
 #pragma unroll
-    for (int m = 0; m < kGroup; ++m) {
+    for (int m = 0; m < G; ++m) {
         const bool s1 = (sel1 >> m) & 1u;
         const bool s0 = (sel0 >> m) & 1u;
         const T fr = s1 ? g[2] : (s0 ? g[0] : T(1));
@@ -235,72 +235,203 @@ __device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t
     }
 }
 
+__device__ __forceinline__ double fabs_t(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float fabs_t(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ bool signbit_t(double x) { return __builtin_signbit(x); }
+__device__ __forceinline__ bool signbit_t(float x) { return __builtin_signbitf(x); }
+
+// Fused multiply-add in the state's precision (a single rounding).
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Register copy of the per-sweep part of a TileInstr (parameters in the
+// state's precision): 5 + 8 or 16 scalar registers.
+template <typename T>
+struct InstrRegs {
+    uint32_t w0;       // kind | slot << 8 | cslots << 16 | dlocal << 24 (bytes of the record)
+    uint32_t w1;       // obitpos | guard << 8
+    uint32_t mm;       // mm1 | mm0 << 16
+    uint32_t lmask, lval;
+    T g[8];
+};
+
+template <typename T>
+__device__ __forceinline__ InstrRegs<T> load_instr(const TileInstr* __restrict__ p) {
+    InstrRegs<T> r;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
+    r.w0 = w[0];
+    r.w1 = w[1];
+    r.mm = w[2];
+    r.lmask = w[3];
+    r.lval = w[4];
+    load_params<T>(p->m, p->mf, r.g);
+    return r;
+}
+
+__device__ __forceinline__ int sbyte(uint32_t w, int i) { return static_cast<int8_t>((w >> (8 * i)) & 0xffu); }
+__device__ __forceinline__ uint32_t ubyte(uint32_t w, int i) { return (w >> (8 * i)) & 0xffu; }
+
+// T_ROT on slot SLOT: v <- v * (zr + i zi) (the pre-phase, exact when it
+// is 1 or -1), then the scaled real rotation of a factored one-qubit
+// unitary, one FMA per output component: form 0: (u, v) <- (u - t v, v + t u);
+// form 1: (u, v) <- (t u - v, u + t v).
+template <typename T, int G, int SLOT>
+__device__ __forceinline__ void slot_rot(T* ar, T* ai, T t, bool form1, T zr, T zi) {
+    constexpr int lo_mask = (1 << SLOT) - 1;
+    if (!form1) {
+        const T nt = -t;
+#pragma unroll
+        for (int p = 0; p < G / 2; ++p) {
+            const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
+            const int m1 = m0 | (1 << SLOT);
+            const T ur = ar[m0], ui = ai[m0];
+            const T vr = ar[m1] * zr - ai[m1] * zi, vi = ar[m1] * zi + ai[m1] * zr;
+            ar[m0] = fmat(nt, vr, ur);
+            ai[m0] = fmat(nt, vi, ui);
+            ar[m1] = fmat(t, ur, vr);
+            ai[m1] = fmat(t, ui, vi);
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < G / 2; ++p) {
+            const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
+            const int m1 = m0 | (1 << SLOT);
+            const T ur = ar[m0], ui = ai[m0];
+            const T vr = ar[m1] * zr - ai[m1] * zi, vi = ar[m1] * zi + ai[m1] * zr;
+            ar[m0] = fmat(t, ur, -vr);
+            ai[m0] = fmat(t, ui, -vi);
+            ar[m1] = fmat(t, vr, ur);
+            ai[m1] = fmat(t, vi, ui);
+        }
+    }
+}
+
+// T_PH on slot SLOT: members whose slot bit is 1 times (zr + i zi).
+template <typename T, int G, int SLOT>
+__device__ __forceinline__ void slot_phase(T* ar, T* ai, T zr, T zi) {
+#pragma unroll
+    for (int m = 0; m < G; ++m)
+        if ((m >> SLOT) & 1) cmul(ar[m], ai[m], zr, zi);
+}
+
+// T_SCALE: every member times (zr + i zi); a real factor takes two products.
+template <typename T, int G>
+__device__ __forceinline__ void scale_all(T* ar, T* ai, T zr, T zi) {
+    if (zi == T(0)) {
+#pragma unroll
+        for (int m = 0; m < G; ++m) {
+            ar[m] *= zr;
+            ai[m] *= zr;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < G; ++m) cmul(ar[m], ai[m], zr, zi);
+    }
+}
+
 // Sweeps of one tile pass on the tile staged in LDS (s_re, s_im): each of
 // the 2^(K-4) groups of 16 amplitudes spanned by the sweep's 4 slot bits is
 // loaded into registers, every gate of the sweep runs in registers, and the
 // group is written back.  Ends with a barrier after every sweep.
-template <typename T, int K>
+// Calls CALL with the compile-time slot index SL equal to `slot` (< S).
+#define BFLY_SLOT_SWITCH(S, slot, CALL)                                    \
+    switch (slot) {                                                        \
+        case 0: { constexpr int SL = 0; CALL; } break;                     \
+        case 1: { constexpr int SL = 1; CALL; } break;                     \
+        case 2: { constexpr int SL = 2; if constexpr (SL < S) { CALL; } } break; \
+        default: { constexpr int SL = 3; if constexpr (SL < S) { CALL; } } break; \
+    }
+
+template <typename T, int K, int S>
 __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, const TileSweep* __restrict__ sweeps,
                                            const TileInstr* __restrict__ instrs, uint64_t base, uint32_t tid) {
-    constexpr uint32_t NG = (1u << K) / kGroup;
+    constexpr int G = 1 << S;
+    constexpr uint32_t NG = (1u << K) / G;
     const int nsweeps = P.nsweeps;
     const int first_sweep = P.first_sweep;
     for (int s = 0; s < nsweeps; ++s) {
         const TileSweep sw = sweeps[first_sweep + s];
         if (tid < NG) {
-            int sl[kSlots] = {sw.s[0], sw.s[1], sw.s[2], sw.s[3]};
-            int srt[kSlots] = {sl[0], sl[1], sl[2], sl[3]};
+            int sl[S], srt[S];
 #pragma unroll
-            for (int a = 0; a < kSlots; ++a)
+            for (int a = 0; a < S; ++a) sl[a] = srt[a] = sw.s[a];
 #pragma unroll
-                for (int b2 = a + 1; b2 < kSlots; ++b2)
+            for (int a = 0; a < S; ++a)
+#pragma unroll
+                for (int b2 = a + 1; b2 < S; ++b2)
                     if (srt[b2] < srt[a]) { const int t = srt[a]; srt[a] = srt[b2]; srt[b2] = t; }
             uint32_t jb = tid;
 #pragma unroll
-            for (int a = 0; a < kSlots; ++a) jb = insert_zero32(jb, srt[a]);
+            for (int a = 0; a < S; ++a) jb = insert_zero32(jb, srt[a]);
             // member m of the group sits at local index jb | dep(m); dep is
             // uniform, so it lives in scalar registers
             auto dep = [&](int m) {
                 uint32_t d = 0;
 #pragma unroll
-                for (int a = 0; a < kSlots; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
+                for (int a = 0; a < S; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
                 return d;
             };
-            T ar[kGroup], ai[kGroup];
+            T ar[G], ai[G];
 #pragma unroll
-            for (int m = 0; m < kGroup; ++m) {
+            for (int m = 0; m < G; ++m) {
                 const uint32_t jm = swz<T>(jb | dep(m));
                 ar[m] = s_re[jm];
                 ai[m] = s_im[jm];
             }
-            for (int q = 0; q < sw.ninstr; ++q) {
-                const TileInstr& in = instrs[sw.first + q];
-                if ((base & in.omask) != in.omask) continue;
-                if ((jb & in.lmask) != in.lmask) continue;
-                const int kind = in.kind;
-                T g[8];
-                load_params<T>(in.m, in.mf, g);
-                if (kind == T_PAIR) {
-                    switch (in.slot) {
-                        case 0: slot_pairs<T, 0>(ar, ai, in.cslots, g); break;
-                        case 1: slot_pairs<T, 1>(ar, ai, in.cslots, g); break;
-                        case 2: slot_pairs<T, 2>(ar, ai, in.cslots, g); break;
-                        default: slot_pairs<T, 3>(ar, ai, in.cslots, g); break;
+            const TileInstr* ip = instrs + sw.first;
+            const int ni = sw.ninstr;
+            for (int q = 0; q < ni; ++q) {
+                const InstrRegs<T> in = load_instr<T>(ip + q);
+                if (ubyte(in.w1, 1) && (base & ip[q].omask) != ip[q].oval) continue;
+                if ((jb & in.lmask) != in.lval) continue;
+                const int kind = sbyte(in.w0, 0);
+                const int slot = sbyte(in.w0, 1);
+                const uint32_t cslots = ubyte(in.w0, 2);
+                const int dlocal = sbyte(in.w0, 3);
+                const T* g = in.g;
+                if (kind == T_ROT) {
+                    const bool f1 = g[1] != T(0);
+                    BFLY_SLOT_SWITCH(S, slot, (slot_rot<T, G, SL>(ar, ai, g[0], f1, g[2], g[3])))
+                } else if (S == 3 && kind == T_ROTS) {
+                    const T* h = sizeof(T) == 8 ? reinterpret_cast<const T*>(ip[q].m) : reinterpret_cast<const T*>(ip[q].mf);
+                    if (cslots & 1u) slot_rot<T, G, 0>(ar, ai, fabs_t(h[0]), signbit_t(h[0]), h[1], h[2]);
+                    if (cslots & 2u) slot_rot<T, G, 1>(ar, ai, fabs_t(h[3]), signbit_t(h[3]), h[4], h[5]);
+                    if constexpr (S > 2) {
+                        if (cslots & 4u) slot_rot<T, G, 2>(ar, ai, fabs_t(h[6]), signbit_t(h[6]), h[7], h[8]);
                     }
+                    if constexpr (S > 3) {
+                        if (cslots & 8u) slot_rot<T, G, 3>(ar, ai, fabs_t(h[9]), signbit_t(h[9]), h[10], h[11]);
+                    }
+                } else if (kind == T_PH) {
+                    BFLY_SLOT_SWITCH(S, slot, (slot_phase<T, G, SL>(ar, ai, g[0], g[1])))
+                } else if (kind == T_SCALE) {
+                    scale_all<T, G>(ar, ai, g[0], g[1]);
+                } else if (kind == T_PAIR) {
+                    BFLY_SLOT_SWITCH(S, slot, (slot_pairs<T, G, SL>(ar, ai, cslots, g)))
                 } else {
-                    uint32_t sel1 = in.mm1, sel0 = in.mm0;
-                    if (in.slot < 0) {
+                    uint32_t sel1 = in.mm & 0xffffu, sel0 = in.mm >> 16;
+                    if (slot < 0) {
                         // target outside the slots: one bit per thread (local) or per tile (outer)
-                        const uint32_t bit = in.dlocal >= 0 ? ((jb >> in.dlocal) & 1u) : ((base & in.obit) ? 1u : 0u);
-                        sel1 = bit ? sel1 : 0u;
-                        sel0 = bit ? 0u : sel0;
+                        const int obitpos = sbyte(in.w1, 0);
+                        const uint32_t bit = dlocal >= 0 ? ((jb >> dlocal) & 1u)
+                                                         : static_cast<uint32_t>((base >> obitpos) & 1ull);
+                        if (kind == T_NEG) {
+                            sel1 = bit ? sel1 : sel0;
+                        } else {
+                            sel1 = bit ? sel1 : 0u;
+                            sel0 = bit ? 0u : sel0;
+                        }
                     }
-                    if (kind == T_NEG) neg_apply<T>(ar, ai, sel1);
-                    else diag_apply<T>(ar, ai, sel1, sel0, g);
+                    if (kind == T_NEG) {
+                        if (slot >= 0) neg_apply<T, G>(ar, ai, __builtin_amdgcn_readfirstlane(sel1));
+                        else neg_apply<T, G>(ar, ai, sel1);
+                    } else {
+                        diag_apply<T, G>(ar, ai, sel1, sel0, g);
+                    }
                 }
             }
 #pragma unroll
-            for (int m = 0; m < kGroup; ++m) {
+            for (int m = 0; m < G; ++m) {
                 const uint32_t jm = swz<T>(jb | dep(m));
                 s_re[jm] = ar[m];
                 s_im[jm] = ai[m];
@@ -308,6 +439,24 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
         }
         __syncthreads();
     }
+}
+
+// Threads of a tile block: one per group of 2^S amplitudes (at least a
+// wave).  Minimum waves per SIMD the register budget must allow: the blocks
+// that fit in 160 KiB of LDS (two at most), each spread over 4 SIMDs.
+constexpr int tile_threads(int k, int s) { return k - s >= 6 ? (1 << (k - s)) : 64; }
+constexpr int tile_waves_per_eu(int elem, int k, int s) {
+    const int lds = 2 * elem << k;
+    const int blocks = lds > 80 * 1024 ? 1 : 2;
+    const int w = blocks * (tile_threads(k, s) / 64) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// The batched kernel keeps more per-tile state (item lookup): at most 4 waves
+// per SIMD (2 for complex128) so that it never spills.
+constexpr int batch_waves_per_eu(int elem, int k, int s) {
+    const int cap = elem == 8 ? 2 : 4;
+    return tile_waves_per_eu(elem, k, s) > cap ? cap : tile_waves_per_eu(elem, k, s);
 }
 
 template <int K>
@@ -326,12 +475,15 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const TilePass& P) {
 // next tile's HBM loads are issued into registers before the current tile's
 // sweeps and land in LDS only after its stage-out, so HBM traffic overlaps
 // the arithmetic.
-template <typename T, int K>
-__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_kernel(
+template <typename T, int K, int S>
+__global__ __launch_bounds__(tile_threads(K, S), tile_waves_per_eu(sizeof(T), K, S)) void tile_pass_kernel(
         T* __restrict__ re, T* __restrict__ im, const TilePass P, const TileSweep* __restrict__ sweeps,
         const TileInstr* __restrict__ instrs) {
     constexpr uint32_t TS = 1u << K;
-    constexpr uint32_t NT = K >= 10 ? TS / kGroup : 64;  // threads per block
+    constexpr uint32_t NT = tile_threads(K, S);  // threads per block
+    // register prefetch of the next tile only where two waves per SIMD are
+    // the occupancy anyway; otherwise the co-resident blocks hide the loads
+    constexpr bool kPrefetch = tile_waves_per_eu(sizeof(T), K, S) <= 2;
     constexpr int VN = Vec<T>::N;
     constexpr uint32_t NV = TS / VN;               // 16-byte vectors per tile (re or im)
     constexpr uint32_t PV = (NV + NT - 1) / NT;    // vectors per thread
@@ -381,10 +533,10 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
         uint64_t nbase = 0;
         if (next < ntiles) {
             nbase = tile_base<K>(next, P);
-            load_tile(nbase);  // in flight during the sweeps
+            if (kPrefetch) load_tile(nbase);  // in flight during the sweeps
         }
 
-        run_sweeps<T, K>(s_re, s_im, P, sweeps, instrs, base, tid);
+        run_sweeps<T, K, S>(s_re, s_im, P, sweeps, instrs, base, tid);
 
         // stage out: LDS -> HBM with 16-byte stores
 #pragma unroll
@@ -408,6 +560,7 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
         __syncthreads();  // every stage-out read is done before the next tile lands in LDS
         tile = next;
         base = nbase;
+        if (!kPrefetch) load_tile(base);
     }
 }
 
@@ -415,11 +568,11 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_pass_ke
 // (each with its own state and plan).  Item i owns global tiles
 // [items[i].tile_begin, items[i].tile_begin + 2^(n_i - K)); blocks walk the
 // global tile range persistently and find their item by binary search.
-template <typename T, int K>
-__global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_batch_kernel(
+template <typename T, int K, int S>
+__global__ __launch_bounds__(tile_threads(K, S), batch_waves_per_eu(sizeof(T), K, S)) void tile_batch_kernel(
         const BatchItem* __restrict__ items, int nitems, uint64_t total_tiles) {
     constexpr uint32_t TS = 1u << K;
-    constexpr uint32_t NT = K >= 10 ? TS / kGroup : 64;
+    constexpr uint32_t NT = tile_threads(K, S);
     constexpr int VN = Vec<T>::N;
     constexpr uint32_t NV = TS / VN;
     using V = typename Vec<T>::type;
@@ -452,7 +605,7 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_batch_k
             }
         }
         __syncthreads();
-        run_sweeps<T, K>(s_re, s_im, it.P, it.sweeps, it.instrs, base, tid);
+        run_sweeps<T, K, S>(s_re, s_im, it.P, it.sweeps, it.instrs, base, tid);
         for (uint32_t e = tid; e < NV; e += NT) {
             const uint32_t j = e * VN;
             V vr, vi;
@@ -471,7 +624,6 @@ __global__ __launch_bounds__(K >= 10 ? (1 << (K - 4)) : 64, 2) void tile_batch_k
     }
 }
 
-constexpr int tile_threads(int k) { return k >= 10 ? (1 << (k - 4)) : 64; }
 
 // Kernel attributes (dynamic LDS beyond 64 KiB) and the persistent grid size
 // (blocks that fit on the device at once) for one kernel instance.
@@ -492,65 +644,81 @@ hipError_t persistent_grid(KernelT kernel, int nt, size_t lds, int* grid_cap) {
     return hipSuccess;
 }
 
-template <typename T, int K>
+template <typename T, int K, int S>
 hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
-    constexpr int NT = tile_threads(K);
+    constexpr int NT = tile_threads(K, S);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
     static int grid_cap = 0;
-    hipError_t e = persistent_grid(tile_pass_kernel<T, K>, NT, lds, &grid_cap);
+    hipError_t e = persistent_grid(tile_pass_kernel<T, K, S>, NT, lds, &grid_cap);
     if (e != hipSuccess) return e;
     if (!re) return hipSuccess;  // preparation only
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
-    hipLaunchKernelGGL((tile_pass_kernel<T, K>), dim3(grid), dim3(NT), lds, st, re, im, P, sw, in);
+    hipLaunchKernelGGL((tile_pass_kernel<T, K, S>), dim3(grid), dim3(NT), lds, st, re, im, P, sw, in);
     return hipGetLastError();
 }
 
-template <typename T, int K>
+template <typename T, int K, int S>
 hipError_t launch_batch_k(const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
-    constexpr int NT = tile_threads(K);
+    constexpr int NT = tile_threads(K, S);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
     static int grid_cap = 0;
-    hipError_t e = persistent_grid(tile_batch_kernel<T, K>, NT, lds, &grid_cap);
+    hipError_t e = persistent_grid(tile_batch_kernel<T, K, S>, NT, lds, &grid_cap);
     if (e != hipSuccess) return e;
     if (!items) return hipSuccess;  // preparation only
     const unsigned grid =
         static_cast<unsigned>(total_tiles < static_cast<uint64_t>(grid_cap) ? total_tiles : grid_cap);
-    hipLaunchKernelGGL((tile_batch_kernel<T, K>), dim3(grid), dim3(NT), lds, st, items, nitems, total_tiles);
+    hipLaunchKernelGGL((tile_batch_kernel<T, K, S>), dim3(grid), dim3(NT), lds, st, items, nitems, total_tiles);
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_batch_t(int k, const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
+// Sweep widths compiled: 4 slots for every tile size, 3 slots for tiles of
+// 2^10 .. 2^13 amplitudes.
+template <typename T, int S>
+hipError_t launch_batch_s(int k, const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
     switch (k) {
-        case 4: return launch_batch_k<T, 4>(items, nitems, total_tiles, st);
-        case 5: return launch_batch_k<T, 5>(items, nitems, total_tiles, st);
-        case 6: return launch_batch_k<T, 6>(items, nitems, total_tiles, st);
-        case 7: return launch_batch_k<T, 7>(items, nitems, total_tiles, st);
-        case 8: return launch_batch_k<T, 8>(items, nitems, total_tiles, st);
-        case 9: return launch_batch_k<T, 9>(items, nitems, total_tiles, st);
-        case 10: return launch_batch_k<T, 10>(items, nitems, total_tiles, st);
-        case 11: return launch_batch_k<T, 11>(items, nitems, total_tiles, st);
-        case 12: return launch_batch_k<T, 12>(items, nitems, total_tiles, st);
-        case 13: return launch_batch_k<T, 13>(items, nitems, total_tiles, st);
+        case 4: if constexpr (S == 4) return launch_batch_k<T, 4, S>(items, nitems, total_tiles, st); break;
+        case 5: if constexpr (S == 4) return launch_batch_k<T, 5, S>(items, nitems, total_tiles, st); break;
+        case 6: if constexpr (S == 4) return launch_batch_k<T, 6, S>(items, nitems, total_tiles, st); break;
+        case 7: if constexpr (S == 4) return launch_batch_k<T, 7, S>(items, nitems, total_tiles, st); break;
+        case 8: if constexpr (S == 4) return launch_batch_k<T, 8, S>(items, nitems, total_tiles, st); break;
+        case 9: if constexpr (S == 4) return launch_batch_k<T, 9, S>(items, nitems, total_tiles, st); break;
+        case 10: return launch_batch_k<T, 10, S>(items, nitems, total_tiles, st);
+        case 11: return launch_batch_k<T, 11, S>(items, nitems, total_tiles, st);
+        case 12: return launch_batch_k<T, 12, S>(items, nitems, total_tiles, st);
+        case 13: return launch_batch_k<T, 13, S>(items, nitems, total_tiles, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_batch_t(int k, int slots, const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
+    if (slots == 3) return launch_batch_s<T, 3>(k, items, nitems, total_tiles, st);
+    if (slots == 4) return launch_batch_s<T, 4>(k, items, nitems, total_tiles, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int S>
+hipError_t launch_tile_s(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
+    switch (P.k) {
+        case 4: if constexpr (S == 4) return launch_tile_k<T, 4, S>(re, im, P, sw, in, st); break;
+        case 5: if constexpr (S == 4) return launch_tile_k<T, 5, S>(re, im, P, sw, in, st); break;
+        case 6: if constexpr (S == 4) return launch_tile_k<T, 6, S>(re, im, P, sw, in, st); break;
+        case 7: if constexpr (S == 4) return launch_tile_k<T, 7, S>(re, im, P, sw, in, st); break;
+        case 8: if constexpr (S == 4) return launch_tile_k<T, 8, S>(re, im, P, sw, in, st); break;
+        case 9: if constexpr (S == 4) return launch_tile_k<T, 9, S>(re, im, P, sw, in, st); break;
+        case 10: return launch_tile_k<T, 10, S>(re, im, P, sw, in, st);
+        case 11: return launch_tile_k<T, 11, S>(re, im, P, sw, in, st);
+        case 12: return launch_tile_k<T, 12, S>(re, im, P, sw, in, st);
+        case 13: return launch_tile_k<T, 13, S>(re, im, P, sw, in, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_tile_t(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
-    switch (P.k) {
-        case 4: return launch_tile_k<T, 4>(re, im, P, sw, in, st);
-        case 5: return launch_tile_k<T, 5>(re, im, P, sw, in, st);
-        case 6: return launch_tile_k<T, 6>(re, im, P, sw, in, st);
-        case 7: return launch_tile_k<T, 7>(re, im, P, sw, in, st);
-        case 8: return launch_tile_k<T, 8>(re, im, P, sw, in, st);
-        case 9: return launch_tile_k<T, 9>(re, im, P, sw, in, st);
-        case 10: return launch_tile_k<T, 10>(re, im, P, sw, in, st);
-        case 11: return launch_tile_k<T, 11>(re, im, P, sw, in, st);
-        case 12: return launch_tile_k<T, 12>(re, im, P, sw, in, st);
-        case 13: return launch_tile_k<T, 13>(re, im, P, sw, in, st);
-    }
+    if (P.slots == 3) return launch_tile_s<T, 3>(re, im, P, sw, in, st);
+    if (P.slots == 4) return launch_tile_s<T, 4>(re, im, P, sw, in, st);
     return hipErrorInvalidValue;
 }
 
@@ -752,6 +920,31 @@ __global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ re, T* __res
     }
 }
 
+// Scatter the bits of x into the positions of `mask` (ascending).
+__device__ __forceinline__ uint64_t deposit(uint64_t x, uint64_t mask) {
+    uint64_t r = 0;
+    while (mask) {
+        const uint64_t low = mask & (~mask + 1);
+        if (x & 1) r |= low;
+        x >>= 1;
+        mask ^= low;
+    }
+    return r;
+}
+
+// PHASE_FLIP: negate the 2^(n - |mask|) amplitudes with (i & mask) == value
+// (sign-bit flips, exact).
+template <typename T>
+__global__ __launch_bounds__(256) void flip_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t count,
+                                                   uint64_t others, uint64_t value) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint64_t idx = deposit(i, others) | value;
+        re[idx] = -re[idx];
+        im[idx] = -im[idx];
+    }
+}
+
 // ---------------------------------------------------------------- (d) reductions
 // TwoSum accumulation (Knuth): s + c carries the sum with its rounding error.
 __device__ __forceinline__ void two_sum_add(double& s, double& c, double x) {
@@ -774,7 +967,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
                                                              const T* __restrict__ bre, const T* __restrict__ bim,
                                                              uint64_t size, uint64_t mask, uint64_t value,
                                                              double* __restrict__ partial) {
-    constexpr int NC = MODE == R_INNER ? 2 : 1;
+    constexpr int NC = (MODE == R_INNER || MODE == R_SUM) ? 2 : 1;
     __shared__ double sh[NC][2][kRedThreads];
     double s[NC], c[NC];
 #pragma unroll
@@ -786,6 +979,9 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
             two_sum_add(s[0], c[0], ar * ar + ai * ai);
         } else if (MODE == R_PROB_MASK) {
             if ((i & mask) == value) two_sum_add(s[0], c[0], ar * ar + ai * ai);
+        } else if (MODE == R_SUM) {
+            two_sum_add(s[0], c[0], ar);
+            two_sum_add(s[NC - 1], c[NC - 1], ai);
         } else if (MODE == R_DIFF2) {
             const double dr = ar - static_cast<double>(bre[i]), di = ai - static_cast<double>(bim[i]);
             two_sum_add(s[0], c[0], dr * dr + di * di);
@@ -841,6 +1037,91 @@ __global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double*
         }
         if (threadIdx.x == 0) out[q] = sh[0][0] + sh[1][0];
         __syncthreads();
+    }
+}
+
+// DIFFUSE over every qubit: sums (sum_re, sum_im) were left in sums[0..1]
+// by the reduction; variant 0: a <- a - 2 mean; variant 1 (the reference's
+// classical diffuser): re <- 2 mean_re - re, im <- -2 mean_im - im.
+template <typename T>
+__global__ __launch_bounds__(256) void diffuse_all_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size,
+                                                          double inv_count, int variant,
+                                                          const double* __restrict__ sums) {
+    const double ar = 2.0 * (sums[0] * inv_count), ai = 2.0 * (sums[1] * inv_count);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        const double xr = re[i], xi = im[i];
+        if (variant == 0) {
+            re[i] = static_cast<T>(xr - ar);
+            im[i] = static_cast<T>(xi - ai);
+        } else {
+            re[i] = static_cast<T>(ar - xr);
+            im[i] = static_cast<T>(-ai - xi);
+        }
+    }
+}
+
+// DIFFUSE over t <= 12 bits `tmask`: one workgroup per block of the other
+// bits; the block's 2^t amplitudes stay in registers between the
+// compensated sum (fixed tree: deterministic) and the update.
+constexpr int kDiffThreads = 256;
+template <typename T>
+__global__ __launch_bounds__(kDiffThreads) void diffuse_block_kernel(T* __restrict__ re, T* __restrict__ im, int t,
+                                                                     uint64_t tmask, uint64_t others,
+                                                                     uint64_t nblocks, int variant) {
+    constexpr int kMaxE = (1 << kDiffuseMaxBits) / kDiffThreads;
+    __shared__ double sh[2][2][kDiffThreads];
+    const uint32_t count = 1u << t;
+    const uint32_t tid = threadIdx.x;
+    const double inv = 1.0 / static_cast<double>(count);
+    for (uint64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const uint64_t base = deposit(blk, others);
+        T xr[kMaxE], xi[kMaxE];
+        double sr = 0, cr = 0, si = 0, ci = 0;
+#pragma unroll
+        for (int r = 0; r < kMaxE; ++r) {
+            const uint32_t e = tid + r * kDiffThreads;
+            if (e < count) {
+                const uint64_t idx = base | deposit(e, tmask);
+                xr[r] = re[idx];
+                xi[r] = im[idx];
+                two_sum_add(sr, cr, static_cast<double>(xr[r]));
+                two_sum_add(si, ci, static_cast<double>(xi[r]));
+            }
+        }
+        sh[0][0][tid] = sr; sh[0][1][tid] = cr;
+        sh[1][0][tid] = si; sh[1][1][tid] = ci;
+        __syncthreads();
+        for (int w = kDiffThreads / 2; w > 0; w >>= 1) {
+            if (tid < static_cast<uint32_t>(w)) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    double ss = sh[q][0][tid], cc = sh[q][1][tid];
+                    acc_merge(ss, cc, sh[q][0][tid + w], sh[q][1][tid + w]);
+                    sh[q][0][tid] = ss;
+                    sh[q][1][tid] = cc;
+                }
+            }
+            __syncthreads();
+        }
+        const double ar = 2.0 * ((sh[0][0][0] + sh[0][1][0]) * inv);
+        const double ai = 2.0 * ((sh[1][0][0] + sh[1][1][0]) * inv);
+        __syncthreads();  // the totals are read before the next block reuses sh
+#pragma unroll
+        for (int r = 0; r < kMaxE; ++r) {
+            const uint32_t e = tid + r * kDiffThreads;
+            if (e < count) {
+                const uint64_t idx = base | deposit(e, tmask);
+                const double vr = xr[r], vi = xi[r];
+                if (variant == 0) {
+                    re[idx] = static_cast<T>(vr - ar);
+                    im[idx] = static_cast<T>(vi - ai);
+                } else {
+                    re[idx] = static_cast<T>(ar - vr);
+                    im[idx] = static_cast<T>(-ai - vi);
+                }
+            }
+        }
     }
 }
 
@@ -912,19 +1193,20 @@ hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P, co
     return launch_tile_t<double>(static_cast<double*>(re), static_cast<double*>(im), P, d_sweeps, d_instrs, stream);
 }
 
-hipError_t prepare_tile_kernels(int dtype, int k) {
+hipError_t prepare_tile_kernels(int dtype, int k, int slots) {
     TilePass P{};
     P.k = k;
     P.n = k;
+    P.slots = slots;
     hipError_t e = launch_tile_pass(dtype, nullptr, nullptr, P, nullptr, nullptr, nullptr);
     if (e != hipSuccess) return e;
-    return launch_tile_batch(dtype, k, nullptr, 0, 0, nullptr);
+    return launch_tile_batch(dtype, k, slots, nullptr, 0, 0, nullptr);
 }
 
-hipError_t launch_tile_batch(int dtype, int k, const BatchItem* d_items, int nitems, uint64_t total_tiles,
+hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_items, int nitems, uint64_t total_tiles,
                              hipStream_t stream) {
-    if (dtype == BFLY_C64) return launch_batch_t<float>(k, d_items, nitems, total_tiles, stream);
-    return launch_batch_t<double>(k, d_items, nitems, total_tiles, stream);
+    if (dtype == BFLY_C64) return launch_batch_t<float>(k, slots, d_items, nitems, total_tiles, stream);
+    return launch_batch_t<double>(k, slots, d_items, nitems, total_tiles, stream);
 }
 
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream) {
@@ -1032,13 +1314,58 @@ hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, 
                         : launch_reduce_t<double, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
         case R_INNER: f ? launch_reduce_t<float, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
                         : launch_reduce_t<double, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        case R_SUM: f ? launch_reduce_t<float, R_SUM>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
+                      : launch_reduce_t<double, R_SUM>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int nch = mode == R_INNER ? 2 : 1;
+    const int nch = (mode == R_INNER || mode == R_SUM) ? 2 : 1;
     hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, scratch + 8, static_cast<int>(nb), nch,
                        scratch);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint64_t value, hipStream_t stream) {
+    const uint64_t all = (1ull << n) - 1;
+    const uint64_t others = all & ~mask;
+    const uint64_t count = 1ull << (n - __builtin_popcountll(mask));
+    const unsigned grid = grid_for(count);
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(flip_kernel<float>, dim3(grid), dim3(256), 0, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), count, others, value);
+    else
+        hipLaunchKernelGGL(flip_kernel<double>, dim3(grid), dim3(256), 0, stream, static_cast<double*>(re),
+                           static_cast<double*>(im), count, others, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, int variant, double* scratch,
+                          hipStream_t stream) {
+    const uint64_t size = 1ull << n;
+    const uint64_t all = size - 1;
+    const int t = __builtin_popcountll(mask);
+    if (mask == all) {
+        hipError_t e = launch_reduce(dtype, R_SUM, re, im, nullptr, nullptr, size, 0, 0, scratch, stream);
+        if (e != hipSuccess) return e;
+        const double inv = 1.0 / static_cast<double>(size);
+        if (dtype == BFLY_C64)
+            hipLaunchKernelGGL(diffuse_all_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream,
+                               static_cast<float*>(re), static_cast<float*>(im), size, inv, variant, scratch);
+        else
+            hipLaunchKernelGGL(diffuse_all_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream,
+                               static_cast<double*>(re), static_cast<double*>(im), size, inv, variant, scratch);
+        return hipGetLastError();
+    }
+    if (t > kDiffuseMaxBits) return hipErrorInvalidValue;
+    const uint64_t nblocks = 1ull << (n - t);
+    const unsigned grid = static_cast<unsigned>(nblocks < 65536 ? nblocks : 65536);
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(diffuse_block_kernel<float>, dim3(grid), dim3(kDiffThreads), 0, stream,
+                           static_cast<float*>(re), static_cast<float*>(im), t, mask, all & ~mask, nblocks, variant);
+    else
+        hipLaunchKernelGGL(diffuse_block_kernel<double>, dim3(grid), dim3(kDiffThreads), 0, stream,
+                           static_cast<double*>(re), static_cast<double*>(im), t, mask, all & ~mask, nblocks, variant);
     return hipGetLastError();
 }
 

@@ -72,8 +72,8 @@ Classified classify(const bfly_op& op) {
     return c;
 }
 
-void fill_floats(const double* m, float* mf) {
-    for (int i = 0; i < 8; ++i) mf[i] = static_cast<float>(m[i]);
+void fill_floats(const double* m, float* mf, int count = 8) {
+    for (int i = 0; i < count; ++i) mf[i] = static_cast<float>(m[i]);
 }
 
 // Register form of a classified gate for a tile sweep.  Pair kinds become
@@ -101,6 +101,8 @@ uint64_t op_qubits(const bfly_op& op, int n) {
         case BFLY_OP_BITREV: return op.aux ? op.aux : all;
         case BFLY_OP_SWAP: return (1ull << op.target) | (1ull << op.arg_i);
         case BFLY_OP_UNITARY: return (((1ull << op.arg_i) - 1) << op.target) | op.controls;
+        case BFLY_OP_PHASE_FLIP:
+        case BFLY_OP_DIFFUSE: return op.aux;
     }
     return all;
 }
@@ -129,6 +131,7 @@ struct PendingGate {
 }  // namespace
 
 int default_tile_bits(int32_t dtype) { return dtype == BFLY_C64 ? 13 : 12; }
+int default_sweep_slots(int32_t dtype, int k) { return (dtype == BFLY_C128 && k >= kMinSlots3Bits) ? 3 : 4; }
 int default_low_bits(int32_t dtype) { return dtype == BFLY_C64 ? 5 : 4; }
 
 int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_len, std::string& err) {
@@ -161,6 +164,18 @@ int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_le
                 if (op.aux + (1ull << (2 * m)) > udata_len) { err = "Unitary size mismatch"; return BFLY_EINVAL; }
                 break;
             }
+            case BFLY_OP_PHASE_FLIP:
+                if (op.aux == 0 || (op.aux & ~all)) { err = "Target out of bounds"; return BFLY_EINVAL; }
+                if (op.controls & ~op.aux) { err = "phase-flip pattern outside its mask"; return BFLY_EINVAL; }
+                break;
+            case BFLY_OP_DIFFUSE:
+                if (op.aux == 0 || (op.aux & ~all)) { err = "Target out of bounds"; return BFLY_EINVAL; }
+                if (op.arg_i != 0 && op.arg_i != 1) { err = "unknown diffuser variant"; return BFLY_EINVAL; }
+                if (op.aux != all && popc(op.aux) > kDiffuseMaxBits) {
+                    err = "diffusers over more than 12 qubits must cover every qubit";
+                    return BFLY_EUNSUPPORTED;
+                }
+                break;
             default:
                 err = "Unknown transformation type";
                 return BFLY_EUNSUPPORTED;
@@ -174,12 +189,258 @@ namespace {
 // Pack the gates of one pass into register sweeps and append the records.
 // A sweep owns up to kSlots local bits; its gates' control masks are split
 // into slot bits / other local bits / outer bits once the slot set is final.
-void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std::vector<PendingGate>& gates) {
+struct Cx {
+    double r, i;
+};
+Cx cmul(Cx a, Cx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+Cx cdiv_unit(Cx a, Cx b) { return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }  // a / b, |b| = 1
+bool is_one(Cx z) { return z.r == 1.0 && z.i == 0.0; }
+bool is_minus_one(Cx z) { return z.r == -1.0 && z.i == 0.0; }
+
+// Factors a 2x2 unitary as M = sigma * PH(ephi) * R * PH(elam), where
+// PH(z) = diag(1, z) and R is the real rotation [[c, -s], [s, c]] written as
+// c * [[1, -t], [t, 1]] (form 0, t = s/c <= 1) or s * [[t, -1], [1, t]]
+// (form 1, t = c/s < 1); sigma collects e^{i alpha} and c or s.  Returns
+// false when M is not unitary to 1e-12 (then the generic pair product runs).
+bool factor_unitary(const double* m, Cx& sigma, Cx& ephi, Cx& elam, int& form, double& t) {
+    const Cx m00{m[0], m[1]}, m01{m[2], m[3]}, m10{m[4], m[5]}, m11{m[6], m[7]};
+    const double c = std::hypot(m00.r, m00.i), s = std::hypot(m10.r, m10.i);
+    if (!(c + s > 0.0) || !std::isfinite(c + s)) return false;
+    const Cx ea = c > 0.0 ? Cx{m00.r / c, m00.i / c} : Cx{m10.r / s, m10.i / s};
+    ephi = {1.0, 0.0};
+    if (c > 0.0 && s > 0.0) {
+        ephi = cdiv_unit({m10.r / s, m10.i / s}, ea);
+        elam = cdiv_unit({-m01.r / s, -m01.i / s}, ea);
+    } else if (s == 0.0) {
+        elam = cdiv_unit({m11.r / c, m11.i / c}, ea);
+    } else {
+        elam = cdiv_unit({-m01.r / s, -m01.i / s}, ea);
+    }
+    double scale;
+    if (c >= s) { form = 0; t = s / c; scale = c; }
+    else { form = 1; t = c / s; scale = s; }
+    sigma = {ea.r * scale, ea.i * scale};
+    // rebuild ea * [[c, -elam s], [ephi s, ephi elam c]] and compare
+    const Cx el_s = cmul(elam, {s, 0.0}), ep_s = cmul(ephi, {s, 0.0});
+    const Cx rec[4] = {cmul(ea, {c, 0.0}), cmul(ea, {-el_s.r, -el_s.i}), cmul(ea, ep_s),
+                       cmul(ea, cmul(cmul(ephi, elam), {c, 0.0}))};
+    const Cx ref[4] = {m00, m01, m10, m11};
+    for (int e = 0; e < 4; ++e)
+        if (std::fabs(rec[e].r - ref[e].r) > 1e-12 || std::fabs(rec[e].i - ref[e].i) > 1e-12) return false;
+    return std::fabs(c * c + s * s - 1.0) <= 1e-12;
+}
+
+// One entry of a tile pass's instruction list before sweep assignment.
+struct Micro {
+    enum Kind { GATE, ROT, PH, SCALE } kind;
+    int q;                    // target qubit (global); -1 for SCALE
+    const PendingGate* pg;    // GATE
+    Cx z;                     // PH factor on target bit 1; SCALE factor
+    int form;                 // ROT
+    double t;                 // ROT
+};
+
+bool blocks_phase(const Micro& u, int q) {
+    return u.q == q && (u.kind == Micro::ROT || (u.kind == Micro::GATE && !u.pg->c.diag));
+}
+
+// Builds the instruction list of a tile pass: uncontrolled one-qubit unitaries
+// are factored (T_ROT between phases), phases on one qubit are moved forward
+// through everything that commutes with them (diagonal gates, gates on other
+// targets, gates controlled by that qubit) and multiplied together, and the
+// pass's scale factors are applied by T_SCALE whenever their product falls
+// below 2^-16 and at the end.
+std::vector<Micro> micro_ops(const std::vector<PendingGate>& gates, bool factor) {
+    std::vector<Micro> u;
+    u.reserve(gates.size() * 3 + 4);
+    for (const PendingGate& pg : gates) {
+        const bfly_op& op = *pg.op;
+        const bool gate = op.kind == BFLY_OP_GATE && factor;
+        if (gate && op.controls == 0 && pg.c.diag && (pg.c.kind == I_PHASE1 || pg.c.kind == I_NEG1)) {
+            const Cx z = pg.c.kind == I_NEG1 ? Cx{-1.0, 0.0} : Cx{pg.c.m[2], pg.c.m[3]};
+            u.push_back({Micro::PH, op.target, nullptr, z, 0, 0.0});
+            continue;
+        }
+        if (gate && op.controls == 0 && !pg.c.diag) {
+            double m[8];
+            pair_matrix(pg.c, m);
+            Cx sigma, ephi, elam;
+            int form;
+            double t;
+            if (factor_unitary(m, sigma, ephi, elam, form, t)) {
+                if (!is_one(elam)) u.push_back({Micro::PH, op.target, nullptr, elam, 0, 0.0});
+                u.push_back({Micro::ROT, op.target, nullptr, sigma, form, t});
+                if (!is_one(ephi)) u.push_back({Micro::PH, op.target, nullptr, ephi, 0, 0.0});
+                continue;
+            }
+        }
+        u.push_back({Micro::GATE, op.kind == BFLY_OP_GATE ? op.target : -1, &pg, {1.0, 0.0}, 0, 0.0});
+    }
+    const size_t N = u.size();
+    std::vector<char> live(N, 1);
+    // merge each phase into the next phase on its qubit, unless a
+    // non-diagonal action on that qubit comes first
+    for (size_t i = 0; i < N; ++i) {
+        if (u[i].kind != Micro::PH) continue;
+        for (size_t j = i + 1; j < N; ++j) {
+            if (!live[j]) continue;
+            if (u[j].kind == Micro::PH && u[j].q == u[i].q) {
+                u[j].z = cmul(u[j].z, u[i].z);
+                live[i] = 0;
+                break;
+            }
+            if (blocks_phase(u[j], u[i].q)) break;
+        }
+    }
+    // anchor every remaining phase next to a non-diagonal action on its
+    // qubit (just before the next one, else just after the previous one), so
+    // that it runs in a sweep where its qubit is a slot
+    std::vector<std::pair<long, size_t>> order;
+    order.reserve(N);
+    for (size_t i = 0; i < N; ++i) {
+        if (!live[i]) continue;
+        long key = 2 * static_cast<long>(i);
+        if (u[i].kind == Micro::PH) {
+            if (is_one(u[i].z)) continue;
+            long anchor = -1;
+            for (size_t j = i + 1; j < N && anchor < 0; ++j)
+                if (live[j] && blocks_phase(u[j], u[i].q)) anchor = 2 * static_cast<long>(j) - 1;
+            for (size_t j = i; j-- > 0 && anchor < 0;)
+                if (live[j] && blocks_phase(u[j], u[i].q)) anchor = 2 * static_cast<long>(j) + 1;
+            if (anchor >= 0) key = anchor;
+        }
+        order.push_back({key, i});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<long, size_t>& a, const std::pair<long, size_t>& b) { return a.first < b.first; });
+    std::vector<Micro> out;
+    out.reserve(order.size() + 4);
+    Cx acc{1.0, 0.0};
+    for (const auto& e : order) {
+        Micro m = u[e.second];
+        out.push_back(m);
+        if (m.kind == Micro::ROT) {
+            acc = cmul(acc, m.z);
+            if (std::hypot(acc.r, acc.i) < 0x1p-16) {
+                out.push_back({Micro::SCALE, -1, nullptr, acc, 0, 0.0});
+                acc = {1.0, 0.0};
+            }
+        }
+    }
+    if (!is_one(acc)) out.push_back({Micro::SCALE, -1, nullptr, acc, 0, 0.0});
+    return out;
+}
+
+// -- Grover peephole -------------------------------------------------------
+// Recognises, in a gate list, the sequences that the reference's classical
+// replacement rules recognise (grover_diffuser_replacement_rule /
+// grover_inversion_0_replacement_rule, core/quantum_circuit.mojo:298-495) in
+// a more general form, and replaces them with their exact action:
+//   X on S, multi-controlled phase -1 on T (S within T), X on S
+//       -> PHASE_FLIP(mask T, pattern T \ S)   (phase_oracle_match, inversion_0)
+//   H on T, PHASE_FLIP(T, 0), H on T
+//       -> DIFFUSE(T), a <- a - 2 mean_T(a)      (diffuser_circuit)
+bool is_plain(const bfly_op& op, int gate) {
+    return op.kind == BFLY_OP_GATE && op.controls == 0 && op.gate == gate;
+}
+
+// Run of gates `gate` without controls on distinct targets starting at i.
+int plain_run(const bfly_op* ops, uint64_t nops, uint64_t i, int gate, uint64_t& set) {
+    set = 0;
+    int len = 0;
+    while (i + len < nops && is_plain(ops[i + len], gate) && !((set >> ops[i + len].target) & 1)) {
+        set |= 1ull << ops[i + len].target;
+        ++len;
+    }
+    return len;
+}
+
+bool is_sign_gate(const bfly_op& op) {
+    if (op.kind != BFLY_OP_GATE) return false;
+    const Classified c = classify(op);
+    if (!c.diag || c.m[0] != 1.0 || c.m[1] != 0.0) return false;
+    return std::fabs(c.m[2] + 1.0) <= 1e-15 && std::fabs(c.m[3]) <= 1e-15;
+}
+
+// X-conjugated sign gate at i: returns its length (0 if none), mask and pattern.
+int match_flip(const bfly_op* ops, uint64_t nops, uint64_t i, int n, uint64_t& mask, uint64_t& value) {
+    uint64_t S = 0;
+    const int s = plain_run(ops, nops, i, BFLY_GATE_X, S);
+    if (i + s >= nops || !is_sign_gate(ops[i + s])) return 0;
+    const bfly_op& g = ops[i + s];
+    const uint64_t T = g.controls | (1ull << g.target);
+    if (S & ~T) return 0;
+    if (s == 0 && popc(T) < n) return 0;  // a bare controlled Z stays a gate
+    uint64_t S2 = 0;
+    const int s2 = plain_run(ops, nops, i + s + 1, BFLY_GATE_X, S2);
+    if (s2 < s || S2 != S) {
+        // the closing run may be longer than S: only its first s gates belong here
+        uint64_t acc = 0;
+        for (int j = 0; j < s; ++j) {
+            const uint64_t q = i + s + 1 + j;
+            if (q >= nops || !is_plain(ops[q], BFLY_GATE_X)) return 0;
+            acc |= 1ull << ops[q].target;
+        }
+        if (acc != S) return 0;
+    }
+    mask = T;
+    value = T & ~S;
+    return 2 * s + 1;
+}
+
+std::vector<bfly_op> rewrite_grover(const bfly_op* ops, uint64_t nops, int n, bool diffusers) {
+    std::vector<bfly_op> out;
+    out.reserve(nops);
+    uint64_t i = 0;
+    while (i < nops) {
+        uint64_t mask = 0, value = 0;
+        if (diffusers) {
+            uint64_t T = 0;
+            const int t = plain_run(ops, nops, i, BFLY_GATE_H, T);
+            int f = 0;
+            if (t > 0 && (popc(T) <= kDiffuseMaxBits || popc(T) == n))
+                f = match_flip(ops, nops, i + t, n, mask, value);
+            if (f > 0 && mask == T && value == 0 && f == 2 * t + 1) {
+                uint64_t T2 = 0;
+                int t2 = 0;
+                for (; t2 < t && i + t + f + t2 < nops && is_plain(ops[i + t + f + t2], BFLY_GATE_H); ++t2)
+                    T2 |= 1ull << ops[i + t + f + t2].target;
+                if (t2 == t && T2 == T) {
+                    bfly_op d{};
+                    d.kind = BFLY_OP_DIFFUSE;
+                    d.aux = T;
+                    d.arg_i = 0;
+                    out.push_back(d);
+                    i += 2 * t + f;
+                    continue;
+                }
+            }
+        }
+        const int f = match_flip(ops, nops, i, n, mask, value);
+        if (f > 0) {
+            bfly_op p{};
+            p.kind = BFLY_OP_PHASE_FLIP;
+            p.aux = mask;
+            p.controls = value;
+            out.push_back(p);
+            i += f;
+            continue;
+        }
+        out.push_back(ops[i]);
+        ++i;
+    }
+    return out;
+}
+
+void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, const std::vector<PendingGate>& gates,
+                    bool factor) {
     Step st{};
     st.kind = S_TILE;
     TilePass& tp = st.tile;
     tp.n = n;
     tp.k = k;
+    tp.slots = S;
+    const int G = 1 << S;
     int local_of[64];
     int li = 0, oi = 0;
     for (int b = 0; b < n; ++b) {
@@ -189,28 +450,148 @@ void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std:
     tp.nouter = oi;
     tp.first_sweep = static_cast<int32_t>(plan.sweeps.size());
 
-    int slots[kSlots];
+    const std::vector<Micro> micro = micro_ops(gates, factor);
+    int slots[kSlots] = {0, 0, 0, 0};
     int nslots = 0;
-    std::vector<const PendingGate*> cur;
+    std::vector<const Micro*> cur;
     auto slot_of = [&](int lb) {
         for (int s = 0; s < nslots; ++s)
             if (slots[s] == lb) return s;
         return -1;
     };
+    // Sign flips commute and compose by XOR: a Z-family instruction whose
+    // selection depends on slot bits only merges into a preceding one.
+    auto push = [&](TileInstr in, int32_t first) {
+        in.guard = in.omask != 0 ? 1 : 0;
+        const bool uniform_neg = in.kind == T_NEG && in.slot >= 0 && in.lmask == 0 && in.omask == 0;
+        if (uniform_neg && static_cast<int32_t>(plan.instrs.size()) > first) {
+            TileInstr& prev = plan.instrs.back();
+            if (prev.kind == T_NEG && prev.slot >= 0 && prev.lmask == 0 && prev.omask == 0) {
+                prev.mm1 = static_cast<uint16_t>(prev.mm1 ^ in.mm1);
+                return;
+            }
+        }
+        plan.instrs.push_back(in);
+    };
     auto close = [&]() {
-        for (int b = 0; nslots < kSlots && b < k; ++b)
+        for (int b = 0; nslots < S && b < k; ++b)
             if (slot_of(b) < 0) slots[nslots++] = b;
         TileSweep sw{};
-        for (int s = 0; s < kSlots; ++s) sw.s[s] = slots[s];
+        for (int s = 0; s < kSlots; ++s) sw.s[s] = s < S ? slots[s] : -1;
         sw.first = static_cast<int32_t>(plan.instrs.size());
-        for (const PendingGate* pg : cur) {
-            const bfly_op& op = *pg->op;
+        // a phase of a rotation's qubit just before it in the sweep rides on
+        // the rotation's instruction (pre-phase; 1 when there is none)
+        std::vector<char> used(cur.size(), 0);
+        std::vector<TileInstr> rot_instr(cur.size());
+        for (size_t ci = 0; ci < cur.size(); ++ci) {
+            const Micro* mu = cur[ci];
+            if (mu->kind != Micro::ROT) continue;
+            TileInstr& in = rot_instr[ci];
+            in.dlocal = -1;
+            in.obitpos = -1;
+            in.kind = T_ROT;
+            in.slot = slot_of(local_of[mu->q]);
+            in.m[0] = mu->t;
+            in.m[1] = mu->form;
+            in.m[2] = 1.0;
+            in.m[3] = 0.0;
+            if (ci > 0 && !used[ci - 1] && cur[ci - 1]->kind == Micro::PH && cur[ci - 1]->q == mu->q) {
+                in.m[2] = cur[ci - 1]->z.r;
+                in.m[3] = cur[ci - 1]->z.i;
+                used[ci - 1] = 1;
+            }
+            fill_floats(in.m, in.mf);
+        }
+        for (size_t ci = 0; ci < cur.size(); ++ci) {
+            if (used[ci]) continue;
+            const Micro* mu = cur[ci];
+            if (mu->kind == Micro::ROT) {
+                push(rot_instr[ci], sw.first);
+                continue;
+            }
             TileInstr in{};
+            in.dlocal = -1;
+            in.obitpos = -1;
+            if (mu->kind == Micro::SCALE) {
+                in.kind = T_SCALE;
+                in.slot = -1;
+                in.m[0] = mu->z.r;
+                in.m[1] = mu->z.i;
+                fill_floats(in.m, in.mf);
+                push(in, sw.first);
+                continue;
+            }
+            const int lt = mu->q >= 0 ? local_of[mu->q] : -1;
+            in.slot = lt >= 0 ? slot_of(lt) : -1;
+            if (mu->kind == Micro::PH) {
+                if (in.slot >= 0) {
+                    for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m)
+                        if ((m >> in.slot) & 1u) in.mm1 |= static_cast<uint16_t>(1u << m);
+                } else {
+                    if (lt < 0) in.obitpos = static_cast<int8_t>(mu->q);
+                    else in.dlocal = lt;
+                    in.mm1 = 0xFFFF;
+                }
+                if (is_minus_one(mu->z)) {
+                    in.kind = T_NEG;
+                } else {
+                    in.kind = in.slot >= 0 ? T_PH : T_DIAG;
+                    in.m[0] = 1.0;
+                    in.m[1] = 0.0;
+                    in.m[2] = mu->z.r;
+                    in.m[3] = mu->z.i;
+                    if (in.kind == T_PH) { in.m[0] = mu->z.r; in.m[1] = mu->z.i; in.m[2] = 0.0; in.m[3] = 0.0; }
+                }
+                fill_floats(in.m, in.mf);
+                push(in, sw.first);
+                continue;
+            }
+            const PendingGate* pg = mu->pg;
+            const bfly_op& op = *pg->op;
+            if (op.kind == BFLY_OP_PHASE_FLIP) {
+                // negation where (index & mask) == pattern: the pattern's slot
+                // bits select members, the others are tested per lane / tile
+                in.kind = T_NEG;
+                int tq = -1;  // the bit that plays the target: a slot if possible
+                for (int b = 0; b < n && tq < 0; ++b)
+                    if (((op.aux >> b) & 1) && local_of[b] >= 0 && slot_of(local_of[b]) >= 0) tq = b;
+                for (int b = 0; b < n && tq < 0; ++b)
+                    if (((op.aux >> b) & 1) && local_of[b] >= 0) tq = b;
+                for (int b = 0; b < n && tq < 0; ++b)
+                    if ((op.aux >> b) & 1) tq = b;
+                uint32_t cvals = 0;
+                for (int b = 0; b < n; ++b) {
+                    if (!((op.aux >> b) & 1) || b == tq) continue;
+                    const uint64_t v = (op.controls >> b) & 1;
+                    const int lb = local_of[b];
+                    if (lb < 0) { in.omask |= 1ull << b; in.oval |= v << b; }
+                    else if (slot_of(lb) >= 0) { in.cslots |= 1u << slot_of(lb); cvals |= static_cast<uint32_t>(v) << slot_of(lb); }
+                    else { in.lmask |= 1u << lb; in.lval |= static_cast<uint32_t>(v) << lb; }
+                }
+                const uint32_t tv = static_cast<uint32_t>((op.controls >> tq) & 1);
+                const int tl = local_of[tq];
+                in.slot = tl >= 0 ? slot_of(tl) : -1;
+                uint16_t sel = 0;
+                for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m) {
+                    if ((m & in.cslots) != cvals) continue;
+                    if (in.slot >= 0 && ((m >> in.slot) & 1u) != tv) continue;
+                    sel |= static_cast<uint16_t>(1u << m);
+                }
+                if (in.slot >= 0) {
+                    in.mm1 = sel;
+                } else {
+                    if (tl < 0) in.obitpos = static_cast<int8_t>(tq);
+                    else in.dlocal = tl;
+                    (tv ? in.mm1 : in.mm0) = sel;
+                }
+                in.cslots = 0;  // folded into the member sets
+                push(in, sw.first);
+                continue;
+            }
             const int ik = pg->c.kind;
             in.kind = ik == I_NEG1 ? T_NEG : (pg->c.diag ? T_DIAG : T_PAIR);
             pair_matrix(pg->c, in.m);
             fill_floats(in.m, in.mf);
-            in.dlocal = -1;
             for (int b = 0; b < n; ++b) {
                 if (!((op.controls >> b) & 1)) continue;
                 const int lb = local_of[b];
@@ -218,12 +599,10 @@ void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std:
                 else if (slot_of(lb) >= 0) in.cslots |= 1u << slot_of(lb);
                 else in.lmask |= 1u << lb;
             }
-            const int lt = local_of[op.target];
-            in.slot = lt >= 0 ? slot_of(lt) : -1;
             if (pg->c.diag) {
-                if (lt < 0) in.obit = 1ull << op.target;
+                if (lt < 0) in.obitpos = static_cast<int8_t>(op.target);
                 else if (in.slot < 0) in.dlocal = lt;
-                for (uint32_t m = 0; m < static_cast<uint32_t>(kGroup); ++m) {
+                for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m) {
                     if ((m & in.cslots) != in.cslots) continue;
                     if (in.slot < 0 || ((m >> in.slot) & 1u)) in.mm1 |= static_cast<uint16_t>(1u << m);
                     if (in.slot < 0 || !((m >> in.slot) & 1u)) in.mm0 |= static_cast<uint16_t>(1u << m);
@@ -231,32 +610,75 @@ void emit_tile_pass(HostPlan& plan, int n, int k, uint64_t tile_mask, const std:
                 // only I_DIAG touches target-bit-0 members (P / controlled P / Z leave them as is)
                 if (ik != I_DIAG) in.mm0 = 0;
             }
-            // Sign flips commute and compose by XOR: a Z-family gate whose
-            // selection depends on slot bits only merges into a preceding one.
-            const bool uniform_neg = in.kind == T_NEG && in.slot >= 0 && in.lmask == 0 && in.omask == 0;
-            if (uniform_neg && static_cast<int32_t>(plan.instrs.size()) > sw.first) {
-                TileInstr& prev = plan.instrs.back();
-                if (prev.kind == T_NEG && prev.slot >= 0 && prev.lmask == 0 && prev.omask == 0) {
-                    prev.mm1 = static_cast<uint16_t>(prev.mm1 ^ in.mm1);
+            in.lval = in.lmask;
+            in.oval = in.omask;
+            push(in, sw.first);
+        }
+        // rotations on distinct slots that follow each other commute: one
+        // record applies them all (3-slot sweeps; the 4-slot kernels keep one
+        // record per rotation, which holds their register budget)
+        if (S == 3) {
+            std::vector<TileInstr> packed;
+            for (size_t i = static_cast<size_t>(sw.first); i < plan.instrs.size();) {
+                const TileInstr& a = plan.instrs[i];
+                size_t j = i;
+                uint32_t used_slots = 0;
+                while (j < plan.instrs.size() && plan.instrs[j].kind == T_ROT &&
+                       !((used_slots >> plan.instrs[j].slot) & 1u)) {
+                    used_slots |= 1u << plan.instrs[j].slot;
+                    ++j;
+                }
+                if (j - i < 2) {
+                    packed.push_back(a);
+                    ++i;
                     continue;
                 }
+                TileInstr r{};
+                r.kind = T_ROTS;
+                r.slot = -1;
+                r.dlocal = -1;
+                r.obitpos = -1;
+                r.cslots = static_cast<uint8_t>(used_slots);
+                for (size_t q = i; q < j; ++q) {
+                    const TileInstr& x = plan.instrs[q];
+                    const int sl = x.slot;
+                    r.m[3 * sl] = x.m[1] != 0.0 ? -x.m[0] : x.m[0];
+                    if (x.m[1] != 0.0 && x.m[0] == 0.0) r.m[3 * sl] = -0.0;
+                    r.m[3 * sl + 1] = x.m[2];
+                    r.m[3 * sl + 2] = x.m[3];
+                }
+                fill_floats(r.m, r.mf, 16);
+                packed.push_back(r);
+                i = j;
             }
-            plan.instrs.push_back(in);
+            plan.instrs.resize(static_cast<size_t>(sw.first));
+            plan.instrs.insert(plan.instrs.end(), packed.begin(), packed.end());
         }
         sw.ninstr = static_cast<int32_t>(plan.instrs.size()) - sw.first;
         plan.sweeps.push_back(sw);
         cur.clear();
         nslots = 0;
     };
-    for (const PendingGate& pg : gates) {
-        if (!pg.c.diag) {
-            const int lt = local_of[pg.op->target];
-            if (slot_of(lt) < 0) {
-                if (nslots == kSlots) close();
-                slots[nslots++] = lt;
+    auto need_slot = [&](int lt) {
+        if (lt < 0 || slot_of(lt) >= 0) return;
+        if (nslots == S) close();
+        slots[nslots++] = lt;
+    };
+    for (size_t i = 0; i < micro.size(); ++i) {
+        const Micro& mu = micro[i];
+        if (mu.kind == Micro::ROT || (mu.kind == Micro::GATE && !mu.pg->c.diag)) {
+            need_slot(local_of[mu.q]);
+        } else if (mu.kind == Micro::PH && local_of[mu.q] >= 0 && slot_of(local_of[mu.q]) < 0) {
+            // a phase anchored before a rotation of its qubit opens the
+            // rotation's slot now; otherwise it only takes a free slot
+            bool before_rot = false;
+            for (size_t j = i + 1; j < micro.size() && !before_rot; ++j) {
+                if (blocks_phase(micro[j], mu.q)) before_rot = true;
+                else if (micro[j].kind != Micro::PH) break;
             }
+            if (before_rot || nslots < S) need_slot(local_of[mu.q]);
         }
-        cur.push_back(&pg);
+        cur.push_back(&mu);
     }
     close();
     tp.nsweeps = static_cast<int32_t>(plan.sweeps.size()) - tp.first_sweep;
@@ -289,6 +711,18 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
     // first remaining gate always fits and every pass makes progress
     L = std::min(L, k - 1);
     const int max_gates = opts.max_gates_per_pass > 0 ? opts.max_gates_per_pass : (1 << 30);
+    // sweep width: 8-amplitude groups (S = 3) or 16 (S = 4); the threads
+    // option names the workgroup size 2^(k - S)
+    int S = default_sweep_slots(dtype, k);
+    if (opts.threads > 0) {
+        if (k >= kMinSlots3Bits && opts.threads == (1 << (k - 3))) S = 3;
+        else if (opts.threads == (k >= 10 ? (1 << (k - 4)) : 64)) S = 4;
+        else {
+            err = "threads must be 2^(k-3) or 2^(k-4) for tile bits k (2^(k-3) needs k >= 10)";
+            return BFLY_EINVAL;
+        }
+    }
+    if (k < kMinSlots3Bits) S = 4;
     const uint64_t low_mask = (1ull << L) - 1;
 
     // Count gates; emit non-gate ops as their own steps.
@@ -307,10 +741,14 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
             case BFLY_OP_UNITARY:
                 if (op.arg_i <= 0) return;
                 st.kind = S_UNITARY; st.a = op.target; st.b = op.arg_i; st.controls = op.controls; st.uoff = op.aux; break;
+            case BFLY_OP_PHASE_FLIP:
+                st.kind = S_FLIP; st.mask = op.aux; st.controls = op.controls; break;
+            case BFLY_OP_DIFFUSE:
+                st.kind = S_DIFFUSE; st.mask = op.aux; st.a = op.arg_i; break;
             default: return;
         }
         plan.steps.push_back(st);
-        plan.passes += 1;
+        if (st.kind != S_FLIP) plan.passes += 1;  // a phase flip touches 2^(n - |mask|) amplitudes
     };
 
     if (!fused) {
@@ -330,20 +768,33 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
         return BFLY_OK;
     }
 
-    // Fused: greedy commutation-aware pass building.
+    // Fused: Grover sequences become phase flips / diffusers (diffusers only
+    // when the state spans several tiles: a one-tile circuit keeps every gate
+    // in LDS), then greedy commutation-aware pass building.
+    for (uint64_t i = 0; i < nops; ++i)
+        if (ops[i].kind == BFLY_OP_GATE) plan.gates += 1;
+    const bool generic = (opts.flags & BFLY_FLAG_GENERIC_PAIRS) != 0;
+    std::vector<bfly_op> rewritten;
+    if (!generic) {
+        rewritten = rewrite_grover(ops, nops, n, n > k);
+        ops = rewritten.data();
+        nops = rewritten.size();
+    }
     std::vector<uint32_t> remaining(nops);
     for (uint64_t i = 0; i < nops; ++i) remaining[i] = static_cast<uint32_t>(i);
     std::vector<Classified> cls(nops);
     for (uint64_t i = 0; i < nops; ++i) {
-        if (ops[i].kind == BFLY_OP_GATE) { cls[i] = classify(ops[i]); plan.gates += 1; }
+        if (ops[i].kind == BFLY_OP_GATE) cls[i] = classify(ops[i]);
+        else if (ops[i].kind == BFLY_OP_PHASE_FLIP) { cls[i] = Classified{}; cls[i].kind = I_NEG1; cls[i].diag = true; }
     }
+    auto placeable = [](const bfly_op& op) { return op.kind == BFLY_OP_GATE || op.kind == BFLY_OP_PHASE_FLIP; };
     size_t head = 0;  // remaining[head..] are still to schedule (compacted each pass)
     std::vector<uint32_t> rest;
     std::vector<PendingGate> placed;
     constexpr size_t kScanWindow = 8192;  // gates examined past the first deferral
     while (head < remaining.size()) {
         const bfly_op& front = ops[remaining[head]];
-        if (front.kind != BFLY_OP_GATE) {
+        if (!placeable(front)) {
             emit_standalone(front);
             ++head;
             continue;
@@ -360,7 +811,7 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
             const bfly_op& op = ops[oi];
             if (deferred_any && ++scanned_after_defer > kScanWindow) break;
             if ((d_nd & all) == all) break;  // every qubit blocked: nothing more can move
-            if (op.kind != BFLY_OP_GATE) {
+            if (!placeable(op)) {
                 const uint64_t q = op_qubits(op, n);
                 d_nd |= q; d_all |= q;
                 rest.push_back(oi);
@@ -368,7 +819,7 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
                 continue;
             }
             const Classified& c = cls[oi];
-            const uint64_t qall = (1ull << op.target) | op.controls;
+            const uint64_t qall = op.kind == BFLY_OP_PHASE_FLIP ? op.aux : (1ull << op.target) | op.controls;
             const uint64_t qnd = c.diag ? 0 : (1ull << op.target);
             bool ok = !(qnd & d_all) && !(qall & d_nd) && static_cast<int>(placed.size()) < max_gates;
             if (ok && qnd && !(qnd & tile)) {
@@ -387,9 +838,16 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
             err = "internal planner error: a pass placed no gate";
             return BFLY_EINVAL;
         }
-        // complete the tile with the lowest unused qubits (any k-subset works)
-        for (int b = 0; popc(tile) < k && b < n; ++b) tile |= 1ull << b;
-        emit_tile_pass(plan, n, k, tile, placed);
+        bool flips_only = true;
+        for (const PendingGate& pg : placed) flips_only = flips_only && pg.op->kind == BFLY_OP_PHASE_FLIP;
+        if (flips_only) {
+            // a pass of phase flips alone: each touches 2^(n - |mask|) amplitudes
+            for (const PendingGate& pg : placed) emit_standalone(*pg.op);
+        } else {
+            // complete the tile with the lowest unused qubits (any k-subset works)
+            for (int b = 0; popc(tile) < k && b < n; ++b) tile |= 1ull << b;
+            emit_tile_pass(plan, n, k, S, tile, placed, !generic);
+        }
         // compact: deferred ops keep their order, followed by the unscanned tail
         std::vector<uint32_t> next;
         next.reserve(rest.size() + (remaining.size() - idx));

@@ -14,8 +14,12 @@ execute_with_handlers (butterfly/core/executors.mojo:268-319):
   ClassicalTransform "BITREV"        -> BITREV (mask of targets, 0 = all; state.mojo:383-390)
   ClassicalTransform "PERMUTE"       -> the SWAP sequence of apply_permute_qubits (state.mojo:393-431)
   ClassicalTransform "SWAP"          -> SWAP (apply_swap, state.mojo:434-444)
-  ClassicalTransform "sign_flip_oracle" -> X-mask, multi-controlled Z, X-mask per marked
-                                        index (same result as classical_oracle, algos/grover.mojo:64-66)
+  ClassicalTransform "sign_flip_oracle" -> PHASE_FLIP (mask = all, pattern = index) per listed
+                                        index (classical_oracle, algos/grover.mojo:64-66)
+  ClassicalTransform "GROVER_INVERSION_0" -> PHASE_FLIP (mask = targets, pattern 0)
+                                        (apply_grover_inversion_0, algos/grover_classical.mojo:262-281)
+  ClassicalTransform "GROVER_DIFFUSER" -> DIFFUSE (mask = targets, reference variant)
+                                        (apply_grover_diffuser, algos/grover_classical.mojo:284-326)
   Measurement / other classical      -> error: not on the GPU path
 
 Validation follows validate_circuit (butterfly/core/executors.mojo:90-185).
@@ -35,7 +39,7 @@ OP_DTYPE = np.dtype([
 ], align=True)
 assert OP_DTYPE.itemsize == 104
 
-OP_GATE, OP_BITREV, OP_SWAP, OP_UNITARY = 0, 1, 2, 3
+OP_GATE, OP_BITREV, OP_SWAP, OP_UNITARY, OP_PHASE_FLIP, OP_DIFFUSE = 0, 1, 2, 3, 4, 5
 
 
 class LoweringError(ValueError):
@@ -71,6 +75,12 @@ class OpList:
 
     def swap(self, a: int, b: int):
         self.rows.append((OP_SWAP, 0, int(a), int(b), 0, 0, 0.0, [0.0] * 8))
+
+    def phase_flip(self, mask: int, value: int):
+        self.rows.append((OP_PHASE_FLIP, 0, 0, 0, int(value), int(mask), 0.0, [0.0] * 8))
+
+    def diffuse(self, mask: int, variant: int):
+        self.rows.append((OP_DIFFUSE, 0, 0, int(variant), 0, int(mask), 0.0, [0.0] * 8))
 
     def unitary(self, u: np.ndarray, target: int, m: int, controls: List[int]):
         mask = 0
@@ -144,12 +154,22 @@ def _lower_sign_flip(out: OpList, indices: List[int]) -> None:
     for m in indices:
         if m < 0 or m >= (1 << n):
             raise LoweringError(f"Target out of bounds: {m}")
-        zeros = [q for q in range(n) if not (m >> q) & 1]
-        for q in zeros:
-            out.gate(q, [], GateKind.X, X)
-        out.gate(n - 1, list(range(n - 1)), GateKind.Z, Z)
-        for q in zeros:
-            out.gate(q, [], GateKind.X, X)
+        out.phase_flip((1 << n) - 1, m)
+
+
+def _target_mask(targets: List[int], n: int) -> int:
+    """_normalize_targets (algos/grover_classical.mojo:221-245): empty means
+    every qubit; out of range and duplicates are errors."""
+    if not targets:
+        return (1 << n) - 1
+    mask = 0
+    for t in targets:
+        if t < 0 or t >= n:
+            raise LoweringError(f"Target out of bounds: {t}")
+        if (mask >> t) & 1:
+            raise LoweringError(f"Duplicate target: {t}")
+        mask |= 1 << t
+    return mask
 
 
 def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None) -> Tuple[np.ndarray, np.ndarray]:
@@ -219,6 +239,13 @@ def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None) -> Tuple[np
                     out.swap(a, b)
             elif name == "sign_flip_oracle":
                 _lower_sign_flip(out, list(tr.targets))
+            elif name == "GROVER_INVERSION_0":
+                out.phase_flip(_target_mask(list(tr.targets), n), 0)
+            elif name == "GROVER_DIFFUSER":
+                mask = _target_mask(list(tr.targets), n)
+                if bin(mask).count("1") > 12 and mask != (1 << n) - 1:
+                    raise LoweringError("GROVER_DIFFUSER over more than 12 qubits must cover every qubit")
+                out.diffuse(mask, 1)
             else:
                 raise LoweringError(
                     f"classical transform '{tr.name}' cannot be lowered to the GPU strategy")

@@ -24,7 +24,7 @@ from . import _lib, algos, workloads as W
 from ._lib import check
 from .circuit import QuantumCircuit
 from .lowering import lower_circuit
-from .state import GpuState, Plan, QuantumState, dtype_code
+from .state import Batch, GpuState, Plan, QuantumState, dtype_code
 
 
 @dataclass
@@ -293,3 +293,144 @@ def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None) -> di
     wall = time.perf_counter() - t0
     return {"circuits": len(jobs), "wall_s": wall, "circuits_per_s": len(jobs) / wall,
             "p50_ms": percentile(lat, 50) * 1e3, "p99_ms": percentile(lat, 99) * 1e3}
+
+
+# -- benchmarks -----------------------------------------------------------------
+def _init_state(job: Job) -> GpuState:
+    g = GpuState(job.n, job.dtype)
+    if job.init_kind == _lib.INIT_RANDOM:
+        g.fill_random(job.init_seed)
+    elif job.init_kind == _lib.INIT_HOST:
+        g.upload(job.init)
+    return g
+
+
+def _reinit(g: GpuState, job: Job) -> None:
+    if job.init_kind == _lib.INIT_RANDOM:
+        g.fill_random(job.init_seed)
+    elif job.init_kind == _lib.INIT_HOST:
+        g.upload(job.init)
+    else:
+        g.reset()
+
+
+def fidelity(a: QuantumState, b: QuantumState) -> Tuple[float, float]:
+    """(L2 error ||a - b||, fidelity |<a|b>|^2 / (<a|a><b|b>)) in FP64."""
+    ar, ai = a.re.astype(np.float64), a.im.astype(np.float64)
+    br, bi = b.re.astype(np.float64), b.im.astype(np.float64)
+    l2 = math.sqrt(float(np.sum((ar - br) ** 2) + np.sum((ai - bi) ** 2)))
+    ip_re = float(np.dot(ar, br) + np.dot(ai, bi))
+    ip_im = float(np.dot(ar, bi) - np.dot(ai, br))
+    na = float(np.dot(ar, ar) + np.dot(ai, ai))
+    nb = float(np.dot(br, br) + np.dot(bi, bi))
+    return l2, (ip_re * ip_re + ip_im * ip_im) / (na * nb)
+
+
+def bench_batching(jobs: List[Job], graph: bool = True, repeats: int = 1, check: bool = True,
+                   exec_opts: Optional[_lib.ExecOpts] = None, log=None, check_every: int = 1) -> dict:
+    """Run `jobs` (i) as one batch: every round's tile passes of all circuits
+    in one persistent launch per tile size, all rounds captured in one
+    hipGraph, and (ii) one circuit after another, each circuit's launches
+    alone (the per-call pattern of the reference's execute()).  States start
+    from their recipe before every timed run; plans are compiled outside the
+    timed region for both.  With `check`, every final state is compared with
+    a complex128 run of the same circuit (every `check_every`-th circuit
+    against its complex128 run; all of them against the one-by-one run)."""
+    eo = exec_opts or _lib.opts()
+    say = log or (lambda m: None)
+    t0 = time.perf_counter()
+    states = [_init_state(j) for j in jobs]
+    plans = [Plan(j.n, j.dtype, j.ops, j.udata, eo) for j in jobs]
+    say(f"batching: {len(jobs)} states + plans in {time.perf_counter() - t0:.1f}s")
+    passes = sum(p.stats()["launches"] for p in plans)
+    algo_bytes = sum(p.stats()["launches"] * 2 * (1 << j.n) * np.dtype(j.dtype).itemsize * 2
+                     for p, j in zip(plans, jobs))
+    batch = Batch(states, plans, graph=graph)
+    batch.launch()              # first launch captures the graph
+    batch.synchronize()
+    results = [g.download() for g in states] if check else None
+    t_batch = []
+    for _ in range(repeats):
+        for g, j in zip(states, jobs):
+            _reinit(g, j)
+        for g in states:
+            g.synchronize()
+        ts = time.perf_counter()
+        batch.launch()
+        batch.synchronize()
+        t_batch.append(time.perf_counter() - ts)
+    bstats = batch.stats()
+    batch.free()
+    say(f"batching: batch {min(t_batch):.3f}s ({bstats})")
+    # one circuit per launch sequence
+    for g, j in zip(states, jobs):
+        _reinit(g, j)
+    for g in states:
+        g.synchronize()
+    plans[0].execute(states[0])   # warm the kernels
+    states[0].synchronize()
+    _reinit(states[0], jobs[0])
+    states[0].synchronize()
+    ts = time.perf_counter()
+    for g, p in zip(states, plans):
+        p.execute(g)
+        g.synchronize()
+    t_seq = time.perf_counter() - ts
+    say(f"batching: sequential {t_seq:.3f}s")
+    worst = {"max_amp_err_vs_seq": 0.0}
+    if check:
+        for g, r in zip(states, results):
+            s = g.download()
+            worst["max_amp_err_vs_seq"] = max(worst["max_amp_err_vs_seq"],
+                                              float(np.max(np.abs(s.re - r.re))), float(np.max(np.abs(s.im - r.im))))
+    for p in plans:
+        p.free()
+    for g in states:
+        g.free()
+    out = {"circuits": len(jobs), "batch_s": min(t_batch), "sequential_s": t_seq,
+           "circuits_per_s": len(jobs) / min(t_batch), "sequential_circuits_per_s": len(jobs) / t_seq,
+           "speedup": t_seq / min(t_batch), "passes": passes, "algorithmic_bytes": algo_bytes,
+           "rounds": bstats["rounds"], "launches": bstats["launches"], "graph": graph}
+    if check:
+        l2s, infid = [], []
+        for j, r in list(zip(jobs, results))[::check_every]:
+            g = GpuState(j.n, np.float64)
+            p = Plan(j.n, np.float64, j.ops, j.udata, eo)
+            _reinit(g, j)
+            p.execute(g)
+            ref = g.download()
+            p.free()
+            g.free()
+            l2, f = fidelity(r, ref)
+            l2s.append(l2)
+            infid.append(1.0 - f)
+        out.update({"max_l2_vs_c128": max(l2s), "min_fidelity_vs_c128": 1.0 - max(infid),
+                    "checked_vs_c128": len(l2s), **worst})
+        say(f"batching: checked {len(l2s)} results vs complex128 runs")
+    return out
+
+
+def bench_serving(jobs: List[Job], arrivals: Optional[np.ndarray], engine_opts: Optional[dict] = None,
+                  sequential: bool = True, log=None) -> dict:
+    """The same request list through the engine and through one circuit per
+    launch sequence; `arrivals` None means every request at t=0 (saturated
+    throughput), else the Poisson arrival times."""
+    say = log or (lambda m: None)
+    with Engine(**(engine_opts or {})) as eng:
+        # warm-up: one small request of each dtype
+        for dt in (np.float32, np.float64):
+            rid = eng.submit(layered_job(10, 2, 1, dt))
+            eng.wait(rid, 60)
+            eng.release(rid)
+        eng_res = run_stream(jobs, arrivals, eng)
+    say(f"serving: engine {eng_res['circuits_per_s']:.1f} circuits/s p50 {eng_res['p50_ms']:.2f} ms "
+        f"p99 {eng_res['p99_ms']:.2f} ms failed {len(eng_res['failed'])}")
+    out = {"engine": {k: v for k, v in eng_res.items() if k != "results"}}
+    if sequential:
+        run_sequential(jobs[:2])  # warm-up
+        seq = run_sequential(jobs, arrivals)
+        say(f"serving: sequential {seq['circuits_per_s']:.1f} circuits/s p50 {seq['p50_ms']:.2f} ms "
+            f"p99 {seq['p99_ms']:.2f} ms")
+        out["sequential"] = seq
+        out["speedup"] = eng_res["circuits_per_s"] / seq["circuits_per_s"]
+    return out

@@ -56,6 +56,16 @@ extern "C" {
 #define BFLY_OP_SWAP     2    /* exchange qubits `target` and `arg_i`            state.mojo:304-317, executors.mojo:231-265 */
 #define BFLY_OP_UNITARY  3    /* dense 2^m block, m = `arg_i`, on [target, target+m); matrix at udata[2*aux ..]
                                  (re,im interleaved, row-major); optional controls mask  transformations_unitary.mojo:22-139 */
+#define BFLY_OP_PHASE_FLIP 4  /* negate every amplitude i with (i & aux) == controls (controls within aux):
+                                 classical_oracle (aux = all, controls = the marked index)  algos/grover.mojo:64-66,
+                                 apply_grover_inversion_0 (controls = 0)  algos/grover_classical.mojo:262-281 */
+#define BFLY_OP_DIFFUSE  5    /* reflection about the uniform superposition of the bits in `aux`, per block of
+                                 the other bits, with mean = (sum of the block) / 2^popcount(aux):
+                                 arg_i = 0: a <- a - 2 mean, the action of H X MCP(pi) X H on those bits
+                                            (diffuser_circuit, algos/grover.mojo:45-62);
+                                 arg_i = 1: re <- 2 mean_re - re, im <- -2 mean_im - im, exactly
+                                            apply_grover_diffuser (algos/grover_classical.mojo:284-326);
+                                 at most 12 bits unless aux covers every qubit */
 
 /* ---- gate kinds: the reference's GateKind values (gates.mojo:83-91) ---- */
 #define BFLY_GATE_H      0
@@ -98,6 +108,9 @@ typedef struct bfly_exec_opts {
 
 #define BFLY_FLAG_SYNC      1 /* block until the device work is done before returning */
 #define BFLY_FLAG_PROFILE   2 /* bracket every launch with HIP events (see bfly_profile_*) */
+#define BFLY_FLAG_GENERIC_PAIRS 4 /* fused mode: apply every gate as its 2x2 product (no factoring of
+                                     uncontrolled one-qubit gates into phases and a scaled rotation,
+                                     no Grover peephole): the reference's per-gate arithmetic */
 
 typedef struct bfly_state bfly_state;   /* opaque: device-resident SoA state */
 typedef struct bfly_plan  bfly_plan;    /* opaque: compiled pass schedule    */

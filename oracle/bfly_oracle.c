@@ -287,6 +287,66 @@ static void SFX(apply_swap)(int n, real* re, real* im, int a, int b) {
     SFX(kernel_x)(re, im, L, 1ull << b, a);
 }
 
+/* Scatter the bits of x into the set positions of mask (ascending):
+   _base_index_from_bits (algos/grover_classical.mojo:248-259). */
+static uint64_t deposit_bits(uint64_t x, uint64_t mask) {
+    uint64_t r = 0;
+    for (int b = 0; b < 64 && mask; ++b) {
+        if (!((mask >> b) & 1)) continue;
+        if (x & 1) r |= 1ull << b;
+        x >>= 1;
+        mask &= ~(1ull << b);
+    }
+    return r;
+}
+
+/* PHASE_FLIP: classical_oracle (algos/grover.mojo:64-66) negates listed
+   indices; apply_grover_inversion_0 (algos/grover_classical.mojo:262-281)
+   negates the target-bits-zero amplitude of every block.  Both are "negate
+   where (i & mask) == value". */
+static void SFX(apply_flip)(int n, real* re, real* im, uint64_t mask, uint64_t value) {
+    uint64_t all = (1ull << n) - 1, others = all & ~mask;
+    uint64_t blocks = 1ull << (n - popcount64(mask));
+    for (uint64_t b = 0; b < blocks; ++b) {
+        uint64_t idx = deposit_bits(b, others) | value;
+        re[idx] = -re[idx];
+        im[idx] = -im[idx];
+    }
+}
+
+/* DIFFUSE: apply_grover_diffuser (algos/grover_classical.mojo:284-326):
+   per block, naive sums over the target subspace, add = 2/2^t * sum;
+   variant 1 keeps the reference's update (re <- add_re - re,
+   im <- -2/2^t * sum_im - im); variant 0 is the action of the gate
+   diffuser H X MCP(pi) X H (algos/grover.mojo:45-62): a <- a - add. */
+static void SFX(apply_diffuse)(int n, real* re, real* im, uint64_t mask, int variant) {
+    uint64_t all = (1ull << n) - 1, others = all & ~mask;
+    int t = popcount64(mask);
+    uint64_t block_size = 1ull << t, blocks = 1ull << (n - t);
+    real inv_block = (real)1 / (real)block_size;
+    for (uint64_t b = 0; b < blocks; ++b) {
+        uint64_t base = deposit_bits(b, others);
+        real sum_re = 0, sum_im = 0;
+        for (uint64_t e = 0; e < block_size; ++e) {
+            uint64_t idx = base | deposit_bits(e, mask);
+            sum_re += re[idx];
+            sum_im += im[idx];
+        }
+        real add_re = (real)2 * inv_block * sum_re;
+        real add_im = (variant == 1 ? (real)-2 : (real)2) * inv_block * sum_im;
+        for (uint64_t e = 0; e < block_size; ++e) {
+            uint64_t idx = base | deposit_bits(e, mask);
+            if (variant == 1) {
+                re[idx] = add_re - re[idx];
+                im[idx] = add_im - im[idx];
+            } else {
+                re[idx] = re[idx] - add_re;
+                im[idx] = im[idx] - add_im;
+            }
+        }
+    }
+}
+
 /* validate_circuit (executors.mojo:129-185) for the lowered op kinds. */
 static int validate_ops(int n, const bfly_op* ops, uint64_t nops, uint64_t udata_len, char* err, int errlen) {
     char buf[160];
@@ -321,6 +381,16 @@ static int validate_ops(int n, const bfly_op* ops, uint64_t nops, uint64_t udata
                 if (need > udata_len) { set_err(err, errlen, "Unitary size mismatch"); return BFLY_EINVAL; }
                 break;
             }
+            case BFLY_OP_PHASE_FLIP:
+            case BFLY_OP_DIFFUSE:
+                if (op->aux == 0 || (op->aux & ~all)) { set_err(err, errlen, "Target out of bounds"); return BFLY_EINVAL; }
+                if (op->kind == BFLY_OP_PHASE_FLIP && (op->controls & ~op->aux)) {
+                    set_err(err, errlen, "phase-flip pattern outside its mask"); return BFLY_EINVAL;
+                }
+                if (op->kind == BFLY_OP_DIFFUSE && op->arg_i != 0 && op->arg_i != 1) {
+                    set_err(err, errlen, "unknown diffuser variant"); return BFLY_EINVAL;
+                }
+                break;
             default:
                 set_err(err, errlen, "Unknown transformation type");
                 return BFLY_EUNSUPPORTED;
@@ -351,6 +421,8 @@ int SFX(oracle_execute)(int n, real* re, real* im, const bfly_op* ops, uint64_t 
                 rc = SFX(apply_unitary)(n, re, im, udata + 2 * op->aux, op->target, op->arg_i, op->controls);
                 if (rc) { set_err(err, errlen, "unitary allocation failed"); return rc; }
                 break;
+            case BFLY_OP_PHASE_FLIP: SFX(apply_flip)(n, re, im, op->aux, op->controls); break;
+            case BFLY_OP_DIFFUSE: SFX(apply_diffuse)(n, re, im, op->aux, op->arg_i); break;
         }
     }
     return BFLY_OK;

@@ -8,10 +8,12 @@ reordering and its mask translation on the CPU, independently of the GPU
 arithmetic."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 I_GEN, I_H, I_X, I_RY, I_DIAG, I_PHASE1, I_NEG1 = range(7)
-T_PAIR, T_NEG, T_DIAG = range(3)
+T_PAIR, T_NEG, T_DIAG, T_ROT, T_PH, T_SCALE, T_ROTS = range(7)
 SQ_HALF = 0.7071067811865476
 
 
@@ -39,17 +41,18 @@ def parse(text: str):
         kind = parts[0]
         kv = _kv(parts[1:])
         if kind == "tile":
-            cur_tile = {"kind": "tile", "k": int(kv["k"]), "qpos": [int(x) for x in kv["qpos"].split(",")],
-                        "sweeps": []}
+            cur_tile = {"kind": "tile", "k": int(kv["k"]), "slots": int(kv["slots"]),
+                        "qpos": [int(x) for x in kv["qpos"].split(",")], "sweeps": []}
             steps.append(cur_tile)
         elif kind == "sweep":
-            cur_sweep = {"s": [int(x) for x in kv["s"].split(",")], "instrs": []}
+            cur_sweep = {"s": [int(x) for x in kv["s"].split(",")][:cur_tile["slots"]], "instrs": []}
             cur_tile["sweeps"].append(cur_sweep)
         elif kind == "instr":
             cur_sweep["instrs"].append({"kind": int(kv["kind"]), "slot": int(kv["slot"]), "cslots": int(kv["cslots"]),
-                                        "lmask": int(kv["lmask"]), "dlocal": int(kv["dlocal"]),
+                                        "lmask": int(kv["lmask"]), "lval": int(kv["lval"]), "dlocal": int(kv["dlocal"]),
                                         "mm1": int(kv["mm1"]), "mm0": int(kv["mm0"]),
-                                        "omask": int(kv["omask"]), "obit": int(kv["obit"]), "m": _parse_m(parts[-1])})
+                                        "omask": int(kv["omask"]), "oval": int(kv["oval"]), "obit": int(kv["obit"]),
+                                        "m": _parse_m(parts[-1])})
         elif kind == "gate":
             steps.append({"kind": "gate", "gkind": int(kv["kind"]), "target": int(kv["target"]),
                           "cmask": int(kv["cmask"]), "m": _parse_m(parts[-1])})
@@ -57,6 +60,10 @@ def parse(text: str):
             steps.append({"kind": "bitrev", "mask": int(kv["mask"])})
         elif kind == "swap":
             steps.append({"kind": "swap", "a": int(kv["a"]), "b": int(kv["b"])})
+        elif kind == "flip":
+            steps.append({"kind": "flip", "mask": int(kv["mask"]), "value": int(kv["value"])})
+        elif kind == "diffuse":
+            steps.append({"kind": "diffuse", "mask": int(kv["mask"]), "variant": int(kv["variant"])})
         elif kind == "unitary":
             steps.append({"kind": "unitary", "target": int(kv["target"]), "m": int(kv["m"]),
                           "controls": int(kv["controls"]), "uoff": int(kv["uoff"])})
@@ -101,17 +108,52 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
             G = base[:, None] + l2g[None, :]
             A = psi[G]
             for sw in st["sweeps"]:
-                assert len(set(sw["s"])) == 4 and all(0 <= x < k for x in sw["s"])
+                S = st["slots"]
+                assert S in (3, 4) and len(set(sw["s"])) == S and all(0 <= x < k for x in sw["s"])
                 for ins in sw["instrs"]:
-                    tile_ok = (base & ins["omask"]) == ins["omask"]
-                    lm = ins["lmask"]
-                    lane_ok = (j & lm) == lm
+                    tile_ok = (base & ins["omask"]) == ins["oval"]
+                    lane_ok = (j & ins["lmask"]) == ins["lval"]
                     # member index of every local index within its 16-group
                     member = np.zeros(1 << k, dtype=np.int64)
-                    for si in range(4):
+                    for si in range(S):
                         member |= ((j >> sw["s"][si]) & 1) << si
                     m = ins["m"]
-                    if ins["kind"] == T_PAIR:
+                    if ins["kind"] == T_ROTS:
+                        for sl in range(S):
+                            if not (ins["cslots"] >> sl) & 1:
+                                continue
+                            tf, zr, zi = m[3 * sl], m[3 * sl + 1], m[3 * sl + 2]
+                            lb = sw["s"][sl]
+                            j0 = j[((j >> lb) & 1) == 0]
+                            j1 = j0 | (1 << lb)
+                            u, v = A[:, j0], A[:, j1] * complex(zr, zi)
+                            t = abs(tf)
+                            if math.copysign(1.0, tf) < 0:
+                                u, v = t * u - v, u + t * v
+                            else:
+                                u, v = u - t * v, v + t * u
+                            A[:, j0], A[:, j1] = u, v
+                        continue
+                    if ins["kind"] in (T_ROT, T_PH, T_SCALE):
+                        # factored one-qubit unitaries: never controlled
+                        assert ins["lmask"] == 0 and ins["omask"] == 0 and ins["cslots"] == 0
+                        if ins["kind"] == T_SCALE:
+                            A = A * complex(m[0], m[1])
+                            continue
+                        lb = sw["s"][ins["slot"]]
+                        j0 = j[((j >> lb) & 1) == 0]
+                        j1 = j0 | (1 << lb)
+                        u, v = A[:, j0], A[:, j1]
+                        if ins["kind"] == T_PH:
+                            A[:, j1] = v * complex(m[0], m[1])
+                        else:
+                            v = v * complex(m[2], m[3])   # pre-phase
+                            if m[1]:
+                                u, v = m[0] * u - v, u + m[0] * v
+                            else:
+                                u, v = u - m[0] * v, v + m[0] * u
+                            A[:, j0], A[:, j1] = u, v
+                    elif ins["kind"] == T_PAIR:
                         cs = ins["cslots"]
                         lb = sw["s"][ins["slot"]]
                         j0 = j[((j >> lb) & 1) == 0]
@@ -130,8 +172,13 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
                                 bit = ((j >> ins["dlocal"]) & 1)[None, :] * np.ones((ntiles, 1), np.int64)
                             else:
                                 bit = ((base & ins["obit"]) != 0).astype(np.int64)[:, None] * np.ones((1, 1 << k), np.int64)
-                            s1 = sel1[None, :] * bit
-                            s0 = sel0[None, :] * (1 - bit)
+                            if ins["kind"] == T_NEG:
+                                # sign flips: mm1 where the target bit is 1, mm0 where it is 0
+                                s1 = sel1[None, :] * bit + sel0[None, :] * (1 - bit)
+                                s0 = 0 * s1
+                            else:
+                                s1 = sel1[None, :] * bit
+                                s0 = sel0[None, :] * (1 - bit)
                         else:
                             s1 = sel1[None, :] * np.ones((ntiles, 1), np.int64)
                             s0 = sel0[None, :] * np.ones((ntiles, 1), np.int64)
@@ -156,6 +203,19 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
             for i, b in enumerate(pos):
                 perm |= ((idx >> b) & 1) << pos[len(pos) - 1 - i]
             psi = psi[perm]
+        elif st["kind"] == "flip":
+            psi = np.where((idx & st["mask"]) == st["value"], -psi, psi)
+        elif st["kind"] == "diffuse":
+            mask = st["mask"]
+            others = ((1 << n) - 1) & ~mask
+            key = idx & others
+            sums = np.zeros(1 << n, dtype=np.complex128)
+            np.add.at(sums, key, psi)
+            mean = sums[key] / (1 << bin(mask).count("1"))
+            if st["variant"] == 0:
+                psi = psi - 2 * mean
+            else:
+                psi = (2 * mean.real - psi.real) + 1j * (-2 * mean.imag - psi.imag)
         elif st["kind"] == "swap":
             a, b = st["a"], st["b"]
             x = ((idx >> a) & 1) ^ ((idx >> b) & 1)
