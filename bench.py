@@ -132,6 +132,13 @@ def bench_serving(args) -> int:
     sat = serve.bench_serving(jobs, None, log=log)
     live = serve.bench_serving(jobs, arrivals, log=log)
     eng = sat["engine"]
+    # the same mix restricted to 12-20 qubits: every request is launch-bound
+    # alone, which is where batching across requests pays
+    small_spec = serve.StreamSpec(count=spec.count, rate=spec.rate, seed=spec.seed, lo=12, hi=20)
+    small_jobs = serve.serving_jobs(small_spec)
+    small = serve.bench_serving(small_jobs, None, log=log)
+    small_live = serve.bench_serving(small_jobs, serve.poisson_arrivals(len(small_jobs), spec.rate, spec.seed),
+                                     log=log)
     out = {
         "metric": "circuits/s through the serving engine (saturated), p99 latency at the stated rate",
         "value": round(eng["circuits_per_s"], 2), "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -151,6 +158,14 @@ def bench_serving(args) -> int:
                     "one_circuit_per_launch": {k: round(live["sequential"][k], 3)
                                                for k in ("circuits_per_s", "p50_ms", "p99_ms")}},
         "engine_stats": eng["engine"],
+        "small_requests_12_20q": {
+            "circuits_per_s": round(small["engine"]["circuits_per_s"], 2),
+            "one_circuit_per_launch_circuits_per_s": round(small["sequential"]["circuits_per_s"], 2),
+            "speedup_vs_one_circuit_per_launch": round(small["speedup"], 3),
+            "saturated_p99_ms": round(small["engine"]["p99_ms"], 3),
+            "at_rate": {"engine": {k: round(small_live["engine"][k], 3) for k in ("circuits_per_s", "p50_ms", "p99_ms")},
+                        "one_circuit_per_launch": {k: round(small_live["sequential"][k], 3)
+                                                   for k in ("circuits_per_s", "p50_ms", "p99_ms")}}},
         "roofline": None,
         "cpu_baseline": None,
     }

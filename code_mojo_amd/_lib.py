@@ -42,7 +42,8 @@ class ExecOpts(ct.Structure):
 
 class EngineOpts(ct.Structure):
     _fields_ = [("budget_bytes", ct.c_uint64), ("max_inflight_rounds", ct.c_int32), ("large_qubits", ct.c_int32),
-                ("max_round_requests", ct.c_int32), ("flags", ct.c_int32)]
+                ("max_round_requests", ct.c_int32), ("flags", ct.c_int32), ("planner_threads", ct.c_int32),
+                ("reserved", ct.c_int32)]
 
 
 class RequestDesc(ct.Structure):

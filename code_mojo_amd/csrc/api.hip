@@ -113,6 +113,43 @@ int upload_plan(bfly_plan* p, int device) {
     return BFLY_OK;
 }
 
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+int fill_random_on(bfly_state* s, uint64_t seed, hipStream_t stream) {
+    // N(0,1) draws, then the compensated norm and the scaling, all on the
+    // device (the sum of 2^(n+1) squared normals is never 0 in practice)
+    const uint64_t size = 1ull << s->n;
+    HIP_TRY(launch_random_fill(s->dtype, s->re, s->im, size, seed, stream), "random fill launch");
+    HIP_TRY(launch_reduce(s->dtype, R_NORM2, s->re, s->im, nullptr, nullptr, size, 0, 0, s->scratch, stream),
+            "norm launch");
+    HIP_TRY(launch_normalize(s->dtype, s->re, s->im, size, s->scratch, stream), "normalize launch");
+    return BFLY_OK;
+}
+
+size_t plan_device_bytes(const bfly_plan* p) {
+    const HostPlan& hp = p->hp;
+    return align256(hp.sweeps.size() * sizeof(TileSweep)) + align256(hp.instrs.size() * sizeof(TileInstr)) +
+           align256(hp.udata.size() * sizeof(double));
+}
+
+int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream) {
+    HostPlan& hp = p->hp;
+    char* c = static_cast<char*>(buf);
+    const size_t bs = hp.sweeps.size() * sizeof(TileSweep), bi = hp.instrs.size() * sizeof(TileInstr),
+                 bu = hp.udata.size() * sizeof(double);
+    p->d_sweeps = bs ? reinterpret_cast<TileSweep*>(c) : nullptr;
+    p->d_instrs = bi ? reinterpret_cast<TileInstr*>(c + align256(bs)) : nullptr;
+    p->d_udata = bu ? reinterpret_cast<double*>(c + align256(bs) + align256(bi)) : nullptr;
+    p->borrowed = true;
+    p->device = device;
+    if (bs) HIP_TRY(hipMemcpyAsync(p->d_sweeps, hp.sweeps.data(), bs, hipMemcpyHostToDevice, stream), "plan sweeps");
+    if (bi) HIP_TRY(hipMemcpyAsync(p->d_instrs, hp.instrs.data(), bi, hipMemcpyHostToDevice, stream), "plan instrs");
+    if (bu) HIP_TRY(hipMemcpyAsync(p->d_udata, hp.udata.data(), bu, hipMemcpyHostToDevice, stream), "plan unitaries");
+    return BFLY_OK;
+}
+
 }  // namespace bfly
 
 namespace {
@@ -294,10 +331,12 @@ int bfly_plan_create(bfly_plan** out, int32_t num_qubits, int32_t dtype, const b
 
 int bfly_plan_destroy(bfly_plan* p) {
     if (!p) return BFLY_OK;
-    if (p->device >= 0) (void)hipSetDevice(p->device);
-    if (p->d_sweeps) (void)hipFree(p->d_sweeps);
-    if (p->d_instrs) (void)hipFree(p->d_instrs);
-    if (p->d_udata) (void)hipFree(p->d_udata);
+    if (!p->borrowed) {
+        if (p->device >= 0) (void)hipSetDevice(p->device);
+        if (p->d_sweeps) (void)hipFree(p->d_sweeps);
+        if (p->d_instrs) (void)hipFree(p->d_instrs);
+        if (p->d_udata) (void)hipFree(p->d_udata);
+    }
     delete p;
     return BFLY_OK;
 }
@@ -471,9 +510,10 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
 // ---- batched execution -------------------------------------------------------
 struct bfly_batch {
     struct Launch {
-        int kind;              // S_TILE: grouped tile launch; otherwise one step of one state
+        int kind;              // S_TILE / S_FLIP / S_DIFFUSE (grouped) or one step of one state
+        bool grouped = false;
         int dtype = 0, k = 0, slots = 0;
-        size_t item_off = 0;   // S_TILE: first item in d_items
+        size_t item_off = 0;   // grouped: first item in d_items (tiles) or d_ops (flips, diffusers)
         int nitems = 0;
         uint64_t total_tiles = 0;
         bfly_state* s = nullptr;
@@ -487,6 +527,8 @@ struct bfly_batch {
     std::vector<Launch> launches;
     std::vector<BatchItem> items;
     BatchItem* d_items = nullptr;
+    std::vector<OpItem> ops;     // flip / diffuser items of grouped launches
+    OpItem* d_ops = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int64_t rounds = 0;
@@ -516,9 +558,13 @@ namespace {
 hipError_t enqueue_batch(bfly_batch* b) {
     for (const auto& L : b->launches) {
         hipError_t e;
-        if (L.kind == S_TILE)
+        if (!L.grouped) e = launch_step(*L.step, L.s, L.p, b->stream);
+        else if (L.kind == S_TILE)
             e = launch_tile_batch(L.dtype, L.k, L.slots, b->d_items + L.item_off, L.nitems, L.total_tiles, b->stream);
-        else e = launch_step(*L.step, L.s, L.p, b->stream);
+        else if (L.kind == S_FLIP)
+            e = launch_flip_items(L.dtype, OpItem{}, b->d_ops + L.item_off, L.nitems, L.total_tiles, b->stream);
+        else
+            e = launch_diffuse_items(L.dtype, OpItem{}, b->d_ops + L.item_off, L.nitems, L.total_tiles, b->stream);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -528,6 +574,7 @@ void free_batch(bfly_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_ops) (void)hipFree(b->d_ops);
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -565,17 +612,25 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
         if (rc) { free_batch(b); return rc; }
         max_steps = std::max(max_steps, plans[i]->hp.steps.size());
     }
-    // rounds: step r of every plan; tile steps grouped by (dtype, sweep width, k)
+    // rounds: step r of every plan; tile steps grouped by (dtype, sweep width,
+    // k), phase flips and whole-state diffusers by (dtype): one launch each
     for (size_t r = 0; r < max_steps; ++r) {
         std::vector<std::pair<int, std::vector<int>>> groups;  // key = dtype * 1024 + slots * 64 + k
+        std::vector<std::pair<int, std::vector<int>>> op_groups;  // key = kind * 16 + dtype
         for (int i = 0; i < count; ++i) {
             const auto& steps = plans[i]->hp.steps;
             if (r >= steps.size()) continue;
             const Step& st = steps[r];
+            const bool whole_diffuse = st.kind == S_DIFFUSE && st.mask == (1ull << states[i]->n) - 1;
             if (st.kind == S_TILE) {
                 const int key = states[i]->dtype * 1024 + st.tile.slots * 64 + st.tile.k;
                 auto it = std::find_if(groups.begin(), groups.end(), [&](auto& g) { return g.first == key; });
                 if (it == groups.end()) groups.push_back({key, {i}});
+                else it->second.push_back(i);
+            } else if (st.kind == S_FLIP || whole_diffuse) {
+                const int key = st.kind * 16 + states[i]->dtype;
+                auto it = std::find_if(op_groups.begin(), op_groups.end(), [&](auto& g) { return g.first == key; });
+                if (it == op_groups.end()) op_groups.push_back({key, {i}});
                 else it->second.push_back(i);
             } else {
                 bfly_batch::Launch L{};
@@ -586,9 +641,30 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
                 b->launches.push_back(L);
             }
         }
+        for (auto& g : op_groups) {
+            bfly_batch::Launch L{};
+            L.kind = g.first / 16;
+            L.dtype = g.first % 16;
+            L.grouped = true;
+            L.item_off = b->ops.size();
+            uint64_t units = 0;
+            for (int i : g.second) {
+                const Step& st = plans[i]->hp.steps[r];
+                bfly_state* s = states[i];
+                OpItem it = L.kind == S_FLIP ? flip_item(s->re, s->im, s->n, st.mask, st.controls)
+                                             : diffuse_item(s->re, s->im, s->n, st.a, s->scratch);
+                it.begin = units;
+                units += it.count;
+                b->ops.push_back(it);
+            }
+            L.nitems = static_cast<int>(g.second.size());
+            L.total_tiles = units;
+            b->launches.push_back(L);
+        }
         for (auto& g : groups) {
             bfly_batch::Launch L{};
             L.kind = S_TILE;
+            L.grouped = true;
             L.dtype = g.first / 1024;
             L.slots = (g.first / 64) % 16;
             L.k = g.first % 64;
@@ -619,6 +695,12 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
         if (e == hipSuccess)
             e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
         if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch items"); }
+    }
+    if (!b->ops.empty()) {
+        e = hipMalloc(&b->d_ops, b->ops.size() * sizeof(OpItem));
+        if (e == hipSuccess)
+            e = hipMemcpy(b->d_ops, b->ops.data(), b->ops.size() * sizeof(OpItem), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch op items"); }
     }
     if (flags & BFLY_BATCH_GRAPH) {
         e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
@@ -757,12 +839,7 @@ int bfly_state_fill_random(bfly_state* s, uint64_t seed) {
     int rc = check_state(s);
     if (rc) return rc;
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
-    HIP_TRY(launch_random_fill(s->dtype, s->re, s->im, 1ull << s->n, seed, s->stream), "random fill launch");
-    double nrm = 0;
-    if ((rc = run_reduce(s, nullptr, R_NORM2, 0, 0, &nrm, 1))) return rc;
-    if (!(nrm > 0)) return fail(BFLY_ENUMERIC, "random state has zero norm");
-    HIP_TRY(launch_scale(s->dtype, s->re, s->im, 1ull << s->n, 1.0 / std::sqrt(nrm), s->stream), "scale launch");
-    return BFLY_OK;
+    return fill_random_on(s, seed, s->stream);
 }
 
 int bfly_scale(bfly_state* s, double scale) {

@@ -26,6 +26,7 @@ struct bfly_plan {
     bfly::TileSweep* d_sweeps = nullptr;
     bfly::TileInstr* d_instrs = nullptr;
     double* d_udata = nullptr;
+    bool borrowed = false;   // device records live in a buffer the plan does not own
 };
 
 namespace bfly {
@@ -36,6 +37,13 @@ int hip_fail(hipError_t e, const char* what);
 int check_state(const bfly_state* s);
 // Copy a plan's sweep / instruction / unitary records to `device` (once).
 int upload_plan(bfly_plan* p, int device);
+// Device bytes of a plan's records, and their asynchronous upload into a
+// caller-owned buffer of at least that size (the host records must stay
+// alive until the copy has run on `stream`).
+size_t plan_device_bytes(const bfly_plan* p);
+// Seeded N(0,1) state normalised to 1, enqueued on `stream`.
+int fill_random_on(bfly_state* s, uint64_t seed, hipStream_t stream);
+int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream);
 // Enqueue one plan step on `stream`.
 hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t stream);
 

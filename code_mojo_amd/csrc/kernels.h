@@ -38,10 +38,34 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
                           const double* d_u, hipStream_t stream);
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
+// Scales to unit norm using the squared norm at d_norm2 (on the device).
+hipError_t launch_normalize(int dtype, void* re, void* im, uint64_t size, const double* d_norm2, hipStream_t stream);
 // re, im ~ N(0,1) from a SplitMix64 stream (not normalised).
 hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint64_t seed, hipStream_t stream);
 
-// PHASE_FLIP: negate every amplitude with (i & mask) == value.
+// One state's share of a phase-flip or whole-state diffuser launch (alone, or
+// batched over many states).
+struct OpItem {
+    void* re;
+    void* im;
+    uint64_t others;      // flip: bits outside the mask
+    uint64_t value;       // flip: pattern
+    uint64_t count;       // flip: amplitudes negated; diffuse: chunks
+    uint64_t begin;       // first work unit of this item in the launch
+    double* partial;      // diffuse: 4 doubles per chunk (the state's scratch)
+    int32_t chunk_bits;   // diffuse: log2 of the amplitudes per chunk
+    int32_t variant;      // diffuse: 0 gate action, 1 the reference's classical update
+};
+OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value);
+OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch);
+// PHASE_FLIP: negate every amplitude with (i & mask) == value.  d_items ==
+// nullptr runs `one` alone.
+hipError_t launch_flip_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
+                             hipStream_t stream);
+// DIFFUSE over every qubit: compensated chunk sums, then the update of every
+// chunk from the same ordered total (identical bits alone or batched).
+hipError_t launch_diffuse_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
+                                hipStream_t stream);
 hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint64_t value, hipStream_t stream);
 // DIFFUSE over the bits of `mask` (every qubit, or at most kDiffuseMaxBits);
 // uses `scratch` (reduce_scratch_doubles()) when the mask covers every qubit.
@@ -50,7 +74,7 @@ hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, i
 
 // (d) reductions.  Partials live in `scratch` (at least reduce_scratch_doubles()
 // doubles); the final value(s) are left in scratch[0..nout) on the device.
-enum ReduceMode : int { R_NORM2 = 0, R_PROB_MASK = 1, R_DIFF2 = 2, R_INNER = 3, R_SUM = 4 };
+enum ReduceMode : int { R_NORM2 = 0, R_PROB_MASK = 1, R_DIFF2 = 2, R_INNER = 3 };
 size_t reduce_scratch_doubles();
 hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre,
                          const void* bim, uint64_t size, uint64_t mask, uint64_t value,

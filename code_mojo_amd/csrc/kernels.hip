@@ -910,6 +910,18 @@ __global__ __launch_bounds__(256) void random_fill_kernel(T* __restrict__ re, T*
     }
 }
 
+// Scale to unit norm with the squared norm left on the device by a reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void normalize_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size,
+                                                        const double* __restrict__ norm2) {
+    const T sc = static_cast<T>(1.0 / sqrt(norm2[0]));
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        re[i] *= sc;
+        im[i] *= sc;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size, double s) {
     const T sc = static_cast<T>(s);
@@ -932,14 +944,30 @@ __device__ __forceinline__ uint64_t deposit(uint64_t x, uint64_t mask) {
     return r;
 }
 
+// Item of global work unit u: the last item whose `begin` is <= u.
+__device__ __forceinline__ const OpItem& find_item(const OpItem& one, const OpItem* __restrict__ items, int nitems,
+                                                   uint64_t u) {
+    if (!items) return one;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].begin <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return items[lo];
+}
+
 // PHASE_FLIP: negate the 2^(n - |mask|) amplitudes with (i & mask) == value
-// (sign-bit flips, exact).
+// (sign-bit flips, exact), for every item of the launch.
 template <typename T>
-__global__ __launch_bounds__(256) void flip_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t count,
-                                                   uint64_t others, uint64_t value) {
+__global__ __launch_bounds__(256) void flip_kernel(const OpItem one, const OpItem* __restrict__ items, int nitems,
+                                                   uint64_t total) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += stride) {
-        const uint64_t idx = deposit(i, others) | value;
+    for (uint64_t u = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; u < total; u += stride) {
+        const OpItem& it = find_item(one, items, nitems, u);
+        const uint64_t idx = deposit(u - it.begin, it.others) | it.value;
+        T* re = static_cast<T*>(it.re);
+        T* im = static_cast<T*>(it.im);
         re[idx] = -re[idx];
         im[idx] = -im[idx];
     }
@@ -967,7 +995,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
                                                              const T* __restrict__ bre, const T* __restrict__ bim,
                                                              uint64_t size, uint64_t mask, uint64_t value,
                                                              double* __restrict__ partial) {
-    constexpr int NC = (MODE == R_INNER || MODE == R_SUM) ? 2 : 1;
+    constexpr int NC = MODE == R_INNER ? 2 : 1;
     __shared__ double sh[NC][2][kRedThreads];
     double s[NC], c[NC];
 #pragma unroll
@@ -979,9 +1007,6 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(const T* __restrict
             two_sum_add(s[0], c[0], ar * ar + ai * ai);
         } else if (MODE == R_PROB_MASK) {
             if ((i & mask) == value) two_sum_add(s[0], c[0], ar * ar + ai * ai);
-        } else if (MODE == R_SUM) {
-            two_sum_add(s[0], c[0], ar);
-            two_sum_add(s[NC - 1], c[NC - 1], ai);
         } else if (MODE == R_DIFF2) {
             const double dr = ar - static_cast<double>(bre[i]), di = ai - static_cast<double>(bim[i]);
             two_sum_add(s[0], c[0], dr * dr + di * di);
@@ -1040,23 +1065,101 @@ __global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double*
     }
 }
 
-// DIFFUSE over every qubit: sums (sum_re, sum_im) were left in sums[0..1]
-// by the reduction; variant 0: a <- a - 2 mean; variant 1 (the reference's
-// classical diffuser): re <- 2 mean_re - re, im <- -2 mean_im - im.
+// DIFFUSE over every qubit, pass 1: block b sums chunk b of its item (a
+// fixed 2^chunk_bits amplitudes) with TwoSum in a fixed order and leaves
+// (s_re, c_re, s_im, c_im) in the item's partials.
+constexpr int kDiffChunkThreads = 256;
 template <typename T>
-__global__ __launch_bounds__(256) void diffuse_all_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size,
-                                                          double inv_count, int variant,
-                                                          const double* __restrict__ sums) {
-    const double ar = 2.0 * (sums[0] * inv_count), ai = 2.0 * (sums[1] * inv_count);
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
-        const double xr = re[i], xi = im[i];
-        if (variant == 0) {
-            re[i] = static_cast<T>(xr - ar);
-            im[i] = static_cast<T>(xi - ai);
-        } else {
-            re[i] = static_cast<T>(ar - xr);
-            im[i] = static_cast<T>(-ai - xi);
+__global__ __launch_bounds__(kDiffChunkThreads) void diffuse_partial_kernel(const OpItem one,
+                                                                           const OpItem* __restrict__ items,
+                                                                           int nitems, uint64_t total) {
+    __shared__ double sh[2][2][kDiffChunkThreads];
+    for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const OpItem& it = find_item(one, items, nitems, u);
+        const uint64_t c = u - it.begin;
+        const uint64_t len = 1ull << it.chunk_bits;
+        const T* re = static_cast<const T*>(it.re) + (c << it.chunk_bits);
+        const T* im = static_cast<const T*>(it.im) + (c << it.chunk_bits);
+        double sr = 0, cr = 0, si = 0, ci = 0;
+        for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
+            two_sum_add(sr, cr, static_cast<double>(re[i]));
+            two_sum_add(si, ci, static_cast<double>(im[i]));
+        }
+        sh[0][0][threadIdx.x] = sr; sh[0][1][threadIdx.x] = cr;
+        sh[1][0][threadIdx.x] = si; sh[1][1][threadIdx.x] = ci;
+        __syncthreads();
+        for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < static_cast<unsigned>(w)) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
+                    acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
+                    sh[q][0][threadIdx.x] = ss;
+                    sh[q][1][threadIdx.x] = cc;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            it.partial[4 * c] = sh[0][0][0];
+            it.partial[4 * c + 1] = sh[0][1][0];
+            it.partial[4 * c + 2] = sh[1][0][0];
+            it.partial[4 * c + 3] = sh[1][1][0];
+        }
+        __syncthreads();
+    }
+}
+
+// Pass 2: every block of an item merges the item's chunk partials in the
+// same order (so all chunks use the same mean), then updates its chunk:
+// variant 0: a <- a - 2 mean; variant 1 (the reference's classical
+// diffuser): re <- 2 mean_re - re, im <- -2 mean_im - im.
+template <typename T>
+__global__ __launch_bounds__(kDiffChunkThreads) void diffuse_apply_kernel(const OpItem one,
+                                                                         const OpItem* __restrict__ items,
+                                                                         int nitems, uint64_t total) {
+    __shared__ double sh[2][2][kDiffChunkThreads];
+    for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const OpItem& it = find_item(one, items, nitems, u);
+        const uint64_t c = u - it.begin;
+        double sr = 0, cr = 0, si = 0, ci = 0;
+        for (uint64_t i = threadIdx.x; i < it.count; i += kDiffChunkThreads) {
+            acc_merge(sr, cr, it.partial[4 * i], it.partial[4 * i + 1]);
+            acc_merge(si, ci, it.partial[4 * i + 2], it.partial[4 * i + 3]);
+        }
+        sh[0][0][threadIdx.x] = sr; sh[0][1][threadIdx.x] = cr;
+        sh[1][0][threadIdx.x] = si; sh[1][1][threadIdx.x] = ci;
+        __syncthreads();
+        for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < static_cast<unsigned>(w)) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
+                    acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
+                    sh[q][0][threadIdx.x] = ss;
+                    sh[q][1][threadIdx.x] = cc;
+                }
+            }
+            __syncthreads();
+        }
+        // 2^-n is exact: mean = sum * 2^-n
+        const int n = it.chunk_bits + (63 - __builtin_clzll(it.count));
+        const double inv = ldexp(1.0, -n);
+        const double ar = 2.0 * ((sh[0][0][0] + sh[0][1][0]) * inv);
+        const double ai = 2.0 * ((sh[1][0][0] + sh[1][1][0]) * inv);
+        __syncthreads();
+        const uint64_t len = 1ull << it.chunk_bits;
+        T* re = static_cast<T*>(it.re) + (c << it.chunk_bits);
+        T* im = static_cast<T*>(it.im) + (c << it.chunk_bits);
+        for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
+            const double xr = re[i], xi = im[i];
+            if (it.variant == 0) {
+                re[i] = static_cast<T>(xr - ar);
+                im[i] = static_cast<T>(xi - ai);
+            } else {
+                re[i] = static_cast<T>(ar - xr);
+                im[i] = static_cast<T>(-ai - xi);
+            }
         }
     }
 }
@@ -1284,6 +1387,16 @@ hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double sca
     return hipGetLastError();
 }
 
+hipError_t launch_normalize(int dtype, void* re, void* im, uint64_t size, const double* d_norm2, hipStream_t stream) {
+    if (dtype == BFLY_C64)
+        hipLaunchKernelGGL(normalize_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<float*>(re),
+                           static_cast<float*>(im), size, d_norm2);
+    else
+        hipLaunchKernelGGL(normalize_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream,
+                           static_cast<double*>(re), static_cast<double*>(im), size, d_norm2);
+    return hipGetLastError();
+}
+
 size_t reduce_scratch_doubles() {
     const size_t a = 2ull * 2 * kRedBlocks + 8;
     const size_t b = 2ull * (1 << kMargMaxBits) * kRedBlocks + (1 << kMargMaxBits);
@@ -1314,30 +1427,69 @@ hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, 
                         : launch_reduce_t<double, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
         case R_INNER: f ? launch_reduce_t<float, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
                         : launch_reduce_t<double, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
-        case R_SUM: f ? launch_reduce_t<float, R_SUM>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
-                      : launch_reduce_t<double, R_SUM>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int nch = (mode == R_INNER || mode == R_SUM) ? 2 : 1;
+    const int nch = mode == R_INNER ? 2 : 1;
     hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, scratch + 8, static_cast<int>(nb), nch,
                        scratch);
     return hipGetLastError();
 }
 
-hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint64_t value, hipStream_t stream) {
-    const uint64_t all = (1ull << n) - 1;
-    const uint64_t others = all & ~mask;
-    const uint64_t count = 1ull << (n - __builtin_popcountll(mask));
-    const unsigned grid = grid_for(count);
+OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value) {
+    OpItem it{};
+    it.re = re;
+    it.im = im;
+    it.others = ((1ull << n) - 1) & ~mask;
+    it.value = value;
+    it.count = 1ull << (n - __builtin_popcountll(mask));
+    return it;
+}
+
+// Chunks of at least 2^14 amplitudes and at most 2^14 chunks: the partials
+// (4 doubles per chunk) fit the state's reduction scratch.
+OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch) {
+    OpItem it{};
+    it.re = re;
+    it.im = im;
+    it.chunk_bits = n <= 14 ? n : (n - 14 > 14 ? n - 14 : 14);
+    it.count = 1ull << (n - it.chunk_bits);
+    it.partial = scratch;
+    it.variant = variant;
+    return it;
+}
+
+hipError_t launch_flip_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
+                             hipStream_t stream) {
+    const unsigned grid = grid_for(total);
     if (dtype == BFLY_C64)
-        hipLaunchKernelGGL(flip_kernel<float>, dim3(grid), dim3(256), 0, stream, static_cast<float*>(re),
-                           static_cast<float*>(im), count, others, value);
+        hipLaunchKernelGGL(flip_kernel<float>, dim3(grid), dim3(256), 0, stream, one, d_items, nitems, total);
     else
-        hipLaunchKernelGGL(flip_kernel<double>, dim3(grid), dim3(256), 0, stream, static_cast<double*>(re),
-                           static_cast<double*>(im), count, others, value);
+        hipLaunchKernelGGL(flip_kernel<double>, dim3(grid), dim3(256), 0, stream, one, d_items, nitems, total);
     return hipGetLastError();
+}
+
+hipError_t launch_diffuse_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
+                                hipStream_t stream) {
+    const unsigned grid = static_cast<unsigned>(total < 65536 ? total : 65536);
+    if (dtype == BFLY_C64) {
+        hipLaunchKernelGGL(diffuse_partial_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one, d_items,
+                           nitems, total);
+        hipLaunchKernelGGL(diffuse_apply_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one, d_items,
+                           nitems, total);
+    } else {
+        hipLaunchKernelGGL(diffuse_partial_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
+                           d_items, nitems, total);
+        hipLaunchKernelGGL(diffuse_apply_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
+                           d_items, nitems, total);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint64_t value, hipStream_t stream) {
+    const OpItem it = flip_item(re, im, n, mask, value);
+    return launch_flip_items(dtype, it, nullptr, 0, it.count, stream);
 }
 
 hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, int variant, double* scratch,
@@ -1346,16 +1498,8 @@ hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, i
     const uint64_t all = size - 1;
     const int t = __builtin_popcountll(mask);
     if (mask == all) {
-        hipError_t e = launch_reduce(dtype, R_SUM, re, im, nullptr, nullptr, size, 0, 0, scratch, stream);
-        if (e != hipSuccess) return e;
-        const double inv = 1.0 / static_cast<double>(size);
-        if (dtype == BFLY_C64)
-            hipLaunchKernelGGL(diffuse_all_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream,
-                               static_cast<float*>(re), static_cast<float*>(im), size, inv, variant, scratch);
-        else
-            hipLaunchKernelGGL(diffuse_all_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream,
-                               static_cast<double*>(re), static_cast<double*>(im), size, inv, variant, scratch);
-        return hipGetLastError();
+        const OpItem it = diffuse_item(re, im, n, variant, scratch);
+        return launch_diffuse_items(dtype, it, nullptr, 0, it.count, stream);
     }
     if (t > kDiffuseMaxBits) return hipErrorInvalidValue;
     const uint64_t nblocks = 1ull << (n - t);

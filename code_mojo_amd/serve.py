@@ -46,9 +46,10 @@ class Engine:
     """Python handle of a bfly_engine."""
 
     def __init__(self, budget_bytes: int = 0, max_inflight_rounds: int = 0, large_qubits: int = 0,
-                 max_round_requests: int = 0):
+                 max_round_requests: int = 0, planner_threads: int = 0):
         lib = _lib.load()
-        opts = _lib.EngineOpts(budget_bytes, max_inflight_rounds, large_qubits, max_round_requests, 0)
+        opts = _lib.EngineOpts(budget_bytes, max_inflight_rounds, large_qubits, max_round_requests, 0,
+                               planner_threads, 0)
         h = ct.c_void_p()
         check(lib.bfly_engine_create(ct.byref(h), ct.byref(opts)))
         self._h = h
@@ -266,18 +267,29 @@ def run_stream(jobs: List[Job], arrivals: Optional[np.ndarray], engine: Engine, 
     return out
 
 
-def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None) -> dict:
+def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None, reuse_states: bool = True) -> dict:
     """One circuit per launch sequence: each circuit planned and executed
-    alone, one after the other (the reference's execute() call pattern)."""
+    alone, one after the other (the reference's execute() call pattern).
+    With `reuse_states` one device state per (qubits, dtype) is kept and
+    re-initialised for every circuit, so the comparison with the engine
+    (which pools states) is not about allocation cost."""
     t0 = time.perf_counter()
     lat = []
+    cache: Dict[Tuple[int, type], GpuState] = {}
     for i, job in enumerate(jobs):
         if arrivals is not None:
             delay = arrivals[i] - (time.perf_counter() - t0)
             if delay > 0:
                 time.sleep(delay)
         ts = time.perf_counter()
-        g = GpuState(job.n, job.dtype)
+        key = (job.n, job.dtype)
+        g = cache.get(key) if reuse_states else None
+        if g is None:
+            g = GpuState(job.n, job.dtype)
+            if reuse_states:
+                cache[key] = g
+        elif job.init_kind == _lib.INIT_ZERO:
+            g.reset()
         if job.init_kind == _lib.INIT_RANDOM:
             g.fill_random(job.init_seed)
         elif job.init_kind == _lib.INIT_HOST:
@@ -289,8 +301,11 @@ def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None) -> di
         ref = ts if arrivals is None else t0 + arrivals[i]
         lat.append(t_done - ref)
         p.free()
-        g.free()
+        if not reuse_states:
+            g.free()
     wall = time.perf_counter() - t0
+    for g in cache.values():
+        g.free()
     return {"circuits": len(jobs), "wall_s": wall, "circuits_per_s": len(jobs) / wall,
             "p50_ms": percentile(lat, 50) * 1e3, "p99_ms": percentile(lat, 99) * 1e3}
 

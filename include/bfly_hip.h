@@ -196,9 +196,11 @@ typedef struct bfly_engine bfly_engine;
 typedef struct bfly_engine_opts {
     uint64_t budget_bytes;        /* 0 = 90 % of the device memory free at creation */
     int32_t max_inflight_rounds;  /* rounds enqueued ahead of completion (0 = 3) */
-    int32_t large_qubits;         /* requests with n >= this run on their own stream (0 = 26) */
+    int32_t large_qubits;         /* requests with n >= this run on their own stream, two at a time (0 = 21) */
     int32_t max_round_requests;   /* requests per round (0 = unlimited) */
     int32_t flags;
+    int32_t planner_threads;      /* host threads compiling submitted circuits (0 = 8) */
+    int32_t reserved;
 } bfly_engine_opts;
 
 #define BFLY_INIT_ZERO   0        /* |0..0> */

@@ -27,6 +27,12 @@ def mixed_jobs(seed, count=12):
             ops, ud = random_ops(rng, n, 40)
         st = W.normal_state(n, seed + 100 + i, dtype)
         jobs.append((n, dtype, ops, ud, st))
+    # Grover circuits above one tile: their phase flips and diffusers run as
+    # batched launches; several of them share rounds
+    for i, (n, dtype) in enumerate(((14, np.float64), (15, np.float32), (14, np.float32), (13, np.float64))):
+        qc, _ = W.grover_search(n, seed + 50 + i)
+        ops, ud = lower_circuit(qc)
+        jobs.append((n, dtype, ops, ud, W.normal_state(n, seed + 200 + i, dtype)))
     return jobs
 
 

@@ -42,6 +42,9 @@ def mixed_jobs():
             job.init = W.normal_state(n, 7 + i, dtype)
             jobs.append(job)
     jobs.append(serve.ae_job(3, np.float64))
+    jobs.append(serve.grover_job(15, 21, np.float32))   # phase flips + diffusers (batched)
+    jobs.append(serve.grover_job(14, 22, np.float32))
+    jobs.append(serve.grover_job(13, 23, np.float64))
     jobs.append(serve.layered_job(26, 2, 99, np.float32))  # runs on its own stream (large)
     return jobs
 
@@ -52,10 +55,14 @@ def test_results_equal_alone_and_in_the_mix(gpu_lib):
         res = serve.run_stream(jobs, None, eng, keep_results=True)
         assert res["completed"] == len(jobs), res["failed"]
         assert eng.stats()["rounds"] >= 1
-    for job, got in zip(jobs, res["results"]):
+        # the second pass reuses the pooled states of the first (re-initialised)
+        res2 = serve.run_stream(list(reversed(jobs)), None, eng, keep_results=True)
+        assert res2["completed"] == len(jobs), res2["failed"]
+    for job, got, got2 in zip(jobs, res["results"], reversed(res2["results"])):
         ref = solo(job)
-        assert np.array_equal(got.re.view(np.uint8), ref.re.view(np.uint8)), job.name
-        assert np.array_equal(got.im.view(np.uint8), ref.im.view(np.uint8)), job.name
+        for g in (got, got2):
+            assert np.array_equal(g.re.view(np.uint8), ref.re.view(np.uint8)), job.name
+            assert np.array_equal(g.im.view(np.uint8), ref.im.view(np.uint8)), job.name
 
 
 def test_random_fill_matches_host_generator(gpu_lib):
