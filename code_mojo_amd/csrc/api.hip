@@ -41,9 +41,10 @@ namespace {
 size_t elem_size(int dtype) { return dtype == BFLY_C64 ? sizeof(float) : sizeof(double); }
 
 // ---- profiling ------------------------------------------------------------
-enum Family { F_TILE = 0, F_GATE, F_BITREV, F_SWAP, F_UNITARY, F_REDUCE, F_FLIP, F_DIFFUSE, F_NFAM };
+enum Family { F_TILE = 0, F_GATE, F_BITREV, F_SWAP, F_UNITARY, F_REDUCE, F_FLIP, F_DIFFUSE, F_BLOCK, F_NFAM };
 const char* kFamilyNames[F_NFAM] = {"tile_pass_kernel", "gate_kernel", "bitrev_kernel", "swap_kernel",
-                                    "unitary_kernel", "reduce_kernel", "flip_kernel", "diffuse_kernels"};
+                                    "unitary_kernel", "reduce_kernel", "flip_kernel", "diffuse_kernels",
+                                    "block_mfma_kernel"};
 std::mutex g_prof_mu;
 int64_t g_prof_launches[F_NFAM] = {0};
 double g_prof_ms[F_NFAM] = {0};
@@ -109,6 +110,16 @@ int upload_plan(bfly_plan* p, int device) {
         HIP_TRY(hipMemcpy(p->d_udata, hp.udata.data(), hp.udata.size() * sizeof(double), hipMemcpyHostToDevice),
                 "hipMemcpy(plan unitaries)");
     }
+    if (!hp.bmat.empty()) {
+        HIP_TRY(hipMalloc(&p->d_bmat, hp.bmat.size() * sizeof(float)), "hipMalloc(plan blocks)");
+        HIP_TRY(hipMemcpy(p->d_bmat, hp.bmat.data(), hp.bmat.size() * sizeof(float), hipMemcpyHostToDevice),
+                "hipMemcpy(plan blocks)");
+    }
+    if (!hp.bphase.empty()) {
+        HIP_TRY(hipMalloc(&p->d_bphase, hp.bphase.size() * sizeof(BlockPhase)), "hipMalloc(plan phases)");
+        HIP_TRY(hipMemcpy(p->d_bphase, hp.bphase.data(), hp.bphase.size() * sizeof(BlockPhase), hipMemcpyHostToDevice),
+                "hipMemcpy(plan phases)");
+    }
     p->device = device;
     return BFLY_OK;
 }
@@ -131,7 +142,8 @@ int fill_random_on(bfly_state* s, uint64_t seed, hipStream_t stream) {
 size_t plan_device_bytes(const bfly_plan* p) {
     const HostPlan& hp = p->hp;
     return align256(hp.sweeps.size() * sizeof(TileSweep)) + align256(hp.instrs.size() * sizeof(TileInstr)) +
-           align256(hp.udata.size() * sizeof(double));
+           align256(hp.udata.size() * sizeof(double)) + align256(hp.bmat.size() * sizeof(float)) +
+           align256(hp.bphase.size() * sizeof(BlockPhase));
 }
 
 int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream) {
@@ -142,11 +154,17 @@ int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream) {
     p->d_sweeps = bs ? reinterpret_cast<TileSweep*>(c) : nullptr;
     p->d_instrs = bi ? reinterpret_cast<TileInstr*>(c + align256(bs)) : nullptr;
     p->d_udata = bu ? reinterpret_cast<double*>(c + align256(bs) + align256(bi)) : nullptr;
+    const size_t bm = hp.bmat.size() * sizeof(float), bp = hp.bphase.size() * sizeof(BlockPhase);
+    const size_t om = align256(bs) + align256(bi) + align256(bu);
+    p->d_bmat = bm ? reinterpret_cast<float*>(c + om) : nullptr;
+    p->d_bphase = bp ? reinterpret_cast<BlockPhase*>(c + om + align256(bm)) : nullptr;
     p->borrowed = true;
     p->device = device;
     if (bs) HIP_TRY(hipMemcpyAsync(p->d_sweeps, hp.sweeps.data(), bs, hipMemcpyHostToDevice, stream), "plan sweeps");
     if (bi) HIP_TRY(hipMemcpyAsync(p->d_instrs, hp.instrs.data(), bi, hipMemcpyHostToDevice, stream), "plan instrs");
     if (bu) HIP_TRY(hipMemcpyAsync(p->d_udata, hp.udata.data(), bu, hipMemcpyHostToDevice, stream), "plan unitaries");
+    if (bm) HIP_TRY(hipMemcpyAsync(p->d_bmat, hp.bmat.data(), bm, hipMemcpyHostToDevice, stream), "plan blocks");
+    if (bp) HIP_TRY(hipMemcpyAsync(p->d_bphase, hp.bphase.data(), bp, hipMemcpyHostToDevice, stream), "plan phases");
     return BFLY_OK;
 }
 
@@ -274,9 +292,15 @@ int bfly_state_copy(bfly_state* dst, const bfly_state* src) {
     int rc = check_state(dst);
     if (rc) return rc;
     if ((rc = check_state(src))) return rc;
-    if (dst->n != src->n || dst->dtype != src->dtype) return fail(BFLY_EINVAL, "state shape mismatch");
+    if (dst->n != src->n) return fail(BFLY_EINVAL, "state shape mismatch");
     HIP_TRY(hipSetDevice(dst->device), "hipSetDevice");
     HIP_TRY(hipStreamSynchronize(src->stream), "hipStreamSynchronize");
+    if (dst->dtype != src->dtype) {
+        // precision conversion (complex64 <-> complex128), element by element
+        HIP_TRY(launch_convert(src->dtype, src->re, src->im, dst->re, dst->im, 1ull << dst->n, dst->stream),
+                "convert launch");
+        return BFLY_OK;
+    }
     const size_t bytes = (1ull << dst->n) * elem_size(dst->dtype);
     HIP_TRY(hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy re)");
     HIP_TRY(hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy im)");
@@ -336,6 +360,8 @@ int bfly_plan_destroy(bfly_plan* p) {
         if (p->d_sweeps) (void)hipFree(p->d_sweeps);
         if (p->d_instrs) (void)hipFree(p->d_instrs);
         if (p->d_udata) (void)hipFree(p->d_udata);
+        if (p->d_bmat) (void)hipFree(p->d_bmat);
+        if (p->d_bphase) (void)hipFree(p->d_bphase);
     }
     delete p;
     return BFLY_OK;
@@ -410,6 +436,29 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
             case S_DIFFUSE:
                 put("diffuse mask=%llu variant=%d\n", static_cast<unsigned long long>(st.mask), st.a);
                 break;
+            case S_BLOCK: {
+                const TilePass& tp = st.tile;
+                const BlockPass& bp = st.block;
+                put("block k=%d renorm=%d qloc=%d,%d,%d,%d,%d qpos=", tp.k, bp.renorm, bp.qloc[0], bp.qloc[1],
+                    bp.qloc[2], bp.qloc[3], bp.qloc[4]);
+                for (int i = 0; i < tp.k; ++i) put(i ? ",%d" : "%d", tp.qpos[i]);
+                out += " mat=";
+                for (int i = 0; i < 64 * 64; ++i) {
+                    std::snprintf(line, sizeof line, i ? ",%a" : "%a", static_cast<double>(hp.bmat[bp.mat_off + i]));
+                    out += line;
+                }
+                out += "\n";
+                for (int w = 0; w < 2; ++w) {
+                    const int first = w ? bp.post_first : bp.pre_first, count = w ? bp.post_count : bp.pre_count;
+                    for (int i = 0; i < count; ++i) {
+                        const BlockPhase& q = hp.bphase[first + i];
+                        put("phase when=%s omask=%llu slot=%d theta=%a\n", w ? "post" : "pre",
+                            static_cast<unsigned long long>(q.omask), q.slot, q.theta);
+                    }
+                }
+                break;
+            }
+            case S_NORM: put("norm\n"); break;
         }
     }
     if (needed) *needed = out.size() + 1;
@@ -462,6 +511,14 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
             case S_DIFFUSE:
                 prof.begin(F_DIFFUSE);
                 e = launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, s->stream);
+                break;
+            case S_BLOCK:
+                prof.begin(F_BLOCK);
+                e = launch_step(st, s, p, s->stream);
+                break;
+            case S_NORM:
+                prof.begin(F_REDUCE);
+                e = launch_step(st, s, p, s->stream);
                 break;
         }
         prof.end();
@@ -547,6 +604,12 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
                                   stream);
         case S_FLIP: return launch_flip(s->dtype, s->re, s->im, s->n, st.mask, st.controls, stream);
         case S_DIFFUSE: return launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, stream);
+        case S_BLOCK:
+            return launch_block_pass(s->dtype, s->re, s->im, st.tile, st.block, p->d_bmat + st.block.mat_off,
+                                     p->d_bphase, s->scratch, stream);
+        case S_NORM:
+            return launch_reduce(s->dtype, R_NORM2, s->re, s->im, nullptr, nullptr, 1ull << s->n, 0, 0, s->scratch,
+                                 stream);
     }
     return hipErrorInvalidValue;
 }

@@ -117,11 +117,39 @@ struct GateLaunch {
     float mf[8];
 };
 
-enum StepKind : int32_t { S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4, S_FLIP = 5, S_DIFFUSE = 6 };
+enum StepKind : int32_t {
+    S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4, S_FLIP = 5, S_DIFFUSE = 6,
+    S_BLOCK = 7,   // dense 2^5-amplitude block on FP32 MFMA (complex64 BLOCK mode)
+    S_NORM = 8,    // compensated squared norm into the state's scratch (BLOCK mode check)
+};
+
+// BLOCK mode: a block is a 32 x 32 complex unitary U on 5 qubits (the
+// block's slots), applied to every group of 32 amplitudes that share the
+// other bits, between phase factors from controlled-phase gates that couple
+// at most one slot to qubits outside the block: amplitude i is multiplied by
+// exp(i theta) when (i & omask) == omask and, for slot >= 0, the slot bit is 1.
+constexpr int kBlockQubits = 5;
+constexpr int kBlockDim = 1 << kBlockQubits;
+struct BlockPhase {
+    uint64_t omask;    // bits outside the block that must be 1 (global index bits)
+    int32_t slot;      // block slot that must be 1, or -1
+    int32_t pad;
+    double theta;      // radians, FP64 (exact for the dyadic pi / 2^k ladders)
+};
+static_assert(sizeof(BlockPhase) == 24, "BlockPhase layout");
+
+struct BlockPass {
+    int32_t qloc[kBlockQubits];  // tile-local bit of each slot
+    int32_t renorm;              // 1: scale by 1 / sqrt(scratch[0]) on load
+    uint64_t mat_off;            // first float of the 64 x 64 real form of U in HostPlan::bmat
+    int32_t pre_first, pre_count;
+    int32_t post_first, post_count;
+};
 
 struct Step {
     StepKind kind;
-    TilePass tile;       // S_TILE
+    TilePass tile;       // S_TILE, S_BLOCK (tile bits = the block's slots + column bits)
+    BlockPass block;     // S_BLOCK
     GateLaunch gate;     // S_GATE
     uint64_t mask = 0;   // S_BITREV: target mask (0 = all); S_FLIP / S_DIFFUSE: bit mask
     int32_t a = 0, b = 0;  // S_SWAP qubits; S_UNITARY: a = target, b = m; S_DIFFUSE: a = variant
@@ -137,6 +165,8 @@ struct HostPlan {
     std::vector<TileSweep> sweeps;
     std::vector<TileInstr> instrs;
     std::vector<double> udata;   // complex, interleaved
+    std::vector<float> bmat;     // BLOCK mode: 64 x 64 real forms of the blocks' U, row-major
+    std::vector<BlockPhase> bphase;
     int64_t gates = 0;
     int64_t passes = 0;          // HBM sweeps over the state
 };

@@ -26,6 +26,8 @@ struct bfly_plan {
     bfly::TileSweep* d_sweeps = nullptr;
     bfly::TileInstr* d_instrs = nullptr;
     double* d_udata = nullptr;
+    float* d_bmat = nullptr;             // BLOCK mode block matrices
+    bfly::BlockPhase* d_bphase = nullptr;  // BLOCK mode phase records
     bool borrowed = false;   // device records live in a buffer the plan does not own
 };
 

@@ -28,6 +28,12 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
 // Sets kernel attributes and sizes the persistent grids of the tile kernels
 // for (dtype, k, slots) outside any stream capture.
 hipError_t prepare_tile_kernels(int dtype, int k, int slots);
+// (b'') BLOCK mode pass (complex64): a 32 x 32 block on FP32 MFMA between
+// FP64-generated phase factors; `mat` is the block's 64 x 64 real form,
+// `ph` the plan's phase records, `norm2` the squared norm to divide out
+// (B.renorm) left on the device by a reduction.
+hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
+                             const BlockPhase* ph, const double* norm2, hipStream_t stream);
 // (a) one gate over the whole state (coalesced strided butterfly pairs).
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream);
 // (c) bit reversal of all bits (mask 0) or of the bits in `mask`, in place.
@@ -38,6 +44,9 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
                           const double* d_u, hipStream_t stream);
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
+// Element-wise precision conversion from a src_dtype state to the other dtype.
+hipError_t launch_convert(int src_dtype, const void* sre, const void* sim, void* dre, void* dim, uint64_t size,
+                          hipStream_t stream);
 // Scales to unit norm using the squared norm at d_norm2 (on the device).
 hipError_t launch_normalize(int dtype, void* re, void* im, uint64_t size, const double* d_norm2, hipStream_t stream);
 // re, im ~ N(0,1) from a SplitMix64 stream (not normalised).

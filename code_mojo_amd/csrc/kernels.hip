@@ -722,6 +722,208 @@ hipError_t launch_tile_t(T* re, T* im, const TilePass& P, const TileSweep* sw, c
     return hipErrorInvalidValue;
 }
 
+// ------------------------------------------------ (b'') dense blocks on MFMA
+// BLOCK mode (complex64): a tile of 2^K amplitudes (K = 9 .. 12) spanned by
+// the block's 5 slots and K - 5 column bits is staged in LDS.  Each wave takes
+// 16 columns at a time: the 32 amplitudes of a column, multiplied by their
+// pre-phase, form the B operand (64 x 16: re over im) of
+//   [out_re; out_im] = [[Ur, -Ui], [Ui, Ur]] [in_re; in_im]
+// computed by 64 v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation);
+// the 64 x 64 A operand stays in registers for the whole pass.  Phase angles
+// are summed in FP64 per column (exact for dyadic pi / 2^k ladders), turned
+// into factors by FP64 sincos and products, and rounded to FP32 once.
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+// exp(i (c0 + sum_j c_j bit_j)) coefficients of one column (FP64).
+__device__ __forceinline__ void block_phase_coeffs(const BlockPhase* __restrict__ ph, int count, uint64_t colglobal,
+                                                   double* c) {
+#pragma unroll
+    for (int j = 0; j <= kBlockQubits; ++j) c[j] = 0.0;
+    for (int i = 0; i < count; ++i) {
+        const BlockPhase q = ph[i];
+        if ((colglobal & q.omask) == q.omask) c[q.slot + 1] += q.theta;
+    }
+}
+
+struct Z64 {
+    double r, i;
+};
+__device__ __forceinline__ Z64 zmul(Z64 a, Z64 b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ Z64 zexp(double t) {
+    double sn, cs;
+    sincos(t, &sn, &cs);
+    return {cs, sn};
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re, float* __restrict__ im,
+                                                         const TilePass P, const BlockPass B,
+                                                         const float* __restrict__ mat,
+                                                         const BlockPhase* __restrict__ ph,
+                                                         const double* __restrict__ norm2) {
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NT = 256;
+    constexpr int VN = 4;
+    constexpr uint32_t NV = TS / VN;
+    constexpr int NGROUP = static_cast<int>(TS / kBlockDim / 16);  // 16-column groups per tile
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* s_re = reinterpret_cast<float*>(lds_raw);
+    float* s_im = s_re + TS;
+    const uint32_t tid = threadIdx.x;
+    const int lane = static_cast<int>(tid & 63u), wave = static_cast<int>(tid >> 6);
+    const int g = lane >> 4;  // k offset of the lane's B / A fragments, row quad of its output
+
+    float afrag[4][16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int s2 = 0; s2 < 16; ++s2) afrag[r][s2] = mat[(16 * r + (lane & 15)) * 64 + 4 * s2 + g];
+
+    uint32_t qmask = 0;
+#pragma unroll
+    for (int j = 0; j < kBlockQubits; ++j) qmask |= 1u << B.qloc[j];
+    const uint32_t wmask = (TS - 1) & ~qmask;
+    auto eloc = [&](int e) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < kBlockQubits; ++j) x |= static_cast<uint32_t>((e >> j) & 1) << B.qloc[j];
+        return x;
+    };
+    const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
+
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile_base<K>(tile, P);
+        for (uint32_t v = tid; v < NV; v += NT) {
+            const uint32_t j = v * VN;
+            const uint64_t gi = base + local_to_global<K>(j, P);
+            const float4 a = *reinterpret_cast<const float4*>(re + gi);
+            const float4 b = *reinterpret_cast<const float4*>(im + gi);
+            s_re[swz<float>(j)] = a.x; s_re[swz<float>(j + 1)] = a.y;
+            s_re[swz<float>(j + 2)] = a.z; s_re[swz<float>(j + 3)] = a.w;
+            s_im[swz<float>(j)] = b.x; s_im[swz<float>(j + 1)] = b.y;
+            s_im[swz<float>(j + 2)] = b.z; s_im[swz<float>(j + 3)] = b.w;
+        }
+        __syncthreads();
+        for (int cg = wave; cg < NGROUP; cg += NT / 64) {
+            const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
+            uint32_t colloc = 0;
+            {
+                uint32_t x = col, m = wmask;
+                while (m) {
+                    const uint32_t low = m & (~m + 1);
+                    if (x & 1) colloc |= low;
+                    x >>= 1;
+                    m ^= low;
+                }
+            }
+            const uint64_t colglobal = base + local_to_global<K>(colloc, P);
+            // pre-phase factors of the lane's 8 elements e = g + 4 m
+            float fr[8], fi[8];
+            {
+                Z64 f[8];
+                f[0] = {scale, 0.0};
+                if (B.pre_count > 0) {
+                    double c[kBlockQubits + 1];
+                    block_phase_coeffs(ph + B.pre_first, B.pre_count, colglobal, c);
+                    Z64 base_f = zexp(c[0]);
+                    if (g & 1) base_f = zmul(base_f, zexp(c[1]));
+                    if (g & 2) base_f = zmul(base_f, zexp(c[2]));
+                    f[0] = {base_f.r * scale, base_f.i * scale};
+                    const Z64 z2 = zexp(c[3]), z3 = zexp(c[4]), z4 = zexp(c[5]);
+                    f[1] = zmul(f[0], z2);
+                    f[2] = zmul(f[0], z3);
+                    f[3] = zmul(f[1], z3);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) f[4 + m] = zmul(f[m], z4);
+                } else {
+#pragma unroll
+                    for (int m = 1; m < 8; ++m) f[m] = f[0];
+                }
+#pragma unroll
+                for (int m = 0; m < 8; ++m) { fr[m] = static_cast<float>(f[m].r); fi[m] = static_cast<float>(f[m].i); }
+            }
+            float bfrag[16];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const uint32_t j = swz<float>(colloc | eloc(g + 4 * m));
+                const float xr = s_re[j], xi = s_im[j];
+                bfrag[m] = xr * fr[m] - xi * fi[m];
+                bfrag[m + 8] = xr * fi[m] + xi * fr[m];
+            }
+            f4_t acc[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[r] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s2 = 0; s2 < 16; ++s2)
+                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[r][s2], bfrag[s2], acc[r], 0, 0, 0);
+            }
+            // lane holds rows 16 r + 4 g + v: re of e = 16 h + 4 g + v (r = h), im (r = 2 + h)
+            float pr[8], pi2[8];
+            {
+                Z64 f[8];
+                f[0] = {1.0, 0.0};
+                if (B.post_count > 0) {
+                    double c[kBlockQubits + 1];
+                    block_phase_coeffs(ph + B.post_first, B.post_count, colglobal, c);
+                    Z64 base_f = zexp(c[0]);
+                    if (g & 1) base_f = zmul(base_f, zexp(c[3]));
+                    if (g & 2) base_f = zmul(base_f, zexp(c[4]));
+                    const Z64 z0 = zexp(c[1]), z1 = zexp(c[2]), z4 = zexp(c[5]);
+                    f[0] = base_f;          // index h * 4 + v
+                    f[1] = zmul(f[0], z0);
+                    f[2] = zmul(f[0], z1);
+                    f[3] = zmul(f[1], z1);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f[4 + v] = zmul(f[v], z4);
+                } else {
+#pragma unroll
+                    for (int v = 1; v < 8; ++v) f[v] = f[0];
+                }
+#pragma unroll
+                for (int v = 0; v < 8; ++v) { pr[v] = static_cast<float>(f[v].r); pi2[v] = static_cast<float>(f[v].i); }
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float yr = acc[h][v], yi = acc[2 + h][v];
+                    const int q = 4 * h + v;
+                    const uint32_t j = swz<float>(colloc | eloc(16 * h + 4 * g + v));
+                    s_re[j] = yr * pr[q] - yi * pi2[q];
+                    s_im[j] = yr * pi2[q] + yi * pr[q];
+                }
+        }
+        __syncthreads();
+        for (uint32_t v = tid; v < NV; v += NT) {
+            const uint32_t j = v * VN;
+            float4 a, b;
+            a.x = s_re[swz<float>(j)]; a.y = s_re[swz<float>(j + 1)];
+            a.z = s_re[swz<float>(j + 2)]; a.w = s_re[swz<float>(j + 3)];
+            b.x = s_im[swz<float>(j)]; b.y = s_im[swz<float>(j + 1)];
+            b.z = s_im[swz<float>(j + 2)]; b.w = s_im[swz<float>(j + 3)];
+            const uint64_t gi = base + local_to_global<K>(j, P);
+            *reinterpret_cast<float4*>(re + gi) = a;
+            *reinterpret_cast<float4*>(im + gi) = b;
+        }
+        __syncthreads();
+    }
+}
+
+template <int K>
+hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
+                          const BlockPhase* ph, const double* norm2, hipStream_t st) {
+    const size_t lds = 2ull * (1ull << K) * sizeof(float);
+    static int grid_cap = 0;
+    hipError_t e = persistent_grid(block_mfma_kernel<K>, 256, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
+    hipLaunchKernelGGL(block_mfma_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, ph, norm2);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- (c) permutations
 __device__ __forceinline__ uint32_t rev_bits32(uint32_t x, int nbits) {
     return nbits == 0 ? 0u : (__brev(x) >> (32 - nbits));
@@ -907,6 +1109,16 @@ __global__ __launch_bounds__(256) void random_fill_kernel(T* __restrict__ re, T*
         const double r = sqrt(-2.0 * log(u1));
         re[i] = static_cast<T>(r * cos(two_pi * u2));
         im[i] = static_cast<T>(r * sin(two_pi * u2));
+    }
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(const S* __restrict__ sre, const S* __restrict__ sim,
+                                                      D* __restrict__ dre, D* __restrict__ dim, uint64_t size) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        dre[i] = static_cast<D>(sre[i]);
+        dim[i] = static_cast<D>(sim[i]);
     }
 }
 
@@ -1312,6 +1524,20 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
     return launch_batch_t<double>(k, slots, d_items, nitems, total_tiles, stream);
 }
 
+hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
+                             const BlockPhase* ph, const double* norm2, hipStream_t stream) {
+    if (dtype != BFLY_C64) return hipErrorInvalidValue;
+    float* r = static_cast<float*>(re);
+    float* i = static_cast<float*>(im);
+    switch (P.k) {
+        case 9: return launch_block_k<9>(r, i, P, B, mat, ph, norm2, stream);
+        case 10: return launch_block_k<10>(r, i, P, B, mat, ph, norm2, stream);
+        case 11: return launch_block_k<11>(r, i, P, B, mat, ph, norm2, stream);
+        case 12: return launch_block_k<12>(r, i, P, B, mat, ph, norm2, stream);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream) {
     if (dtype == BFLY_C64) return launch_gate_t<float>(static_cast<float*>(re), static_cast<float*>(im), G, stream);
     return launch_gate_t<double>(static_cast<double*>(re), static_cast<double*>(im), G, stream);
@@ -1384,6 +1610,19 @@ hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double sca
     else
         hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<double*>(re),
                            static_cast<double*>(im), size, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(int src_dtype, const void* sre, const void* sim, void* dre, void* dim, uint64_t size,
+                          hipStream_t stream) {
+    if (src_dtype == BFLY_C64)
+        hipLaunchKernelGGL((convert_kernel<float, double>), dim3(grid_for(size)), dim3(256), 0, stream,
+                           static_cast<const float*>(sre), static_cast<const float*>(sim), static_cast<double*>(dre),
+                           static_cast<double*>(dim), size);
+    else
+        hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(size)), dim3(256), 0, stream,
+                           static_cast<const double*>(sre), static_cast<const double*>(sim), static_cast<float*>(dre),
+                           static_cast<float*>(dim), size);
     return hipGetLastError();
 }
 

@@ -15,6 +15,7 @@
 // gate left behind, so each pass is exact, not approximate.
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 
 #include "bfly_plan.h"
@@ -686,6 +687,278 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
     plan.passes += 1;
 }
 
+// -- BLOCK mode -------------------------------------------------------------
+using Cplx = std::complex<double>;
+
+// U <- G U for a gate on block slot t with slot controls cm (rows with the
+// controls set mix pairwise; diagonal gates scale rows).
+void fold_into_block(std::vector<Cplx>& U, const Classified& c, int t, uint32_t cm) {
+    double g[8];
+    pair_matrix(c, g);
+    const Cplx a(g[0], g[1]), b(g[2], g[3]), cc(g[4], g[5]), d(g[6], g[7]);
+    const Cplx d0(c.m[0], c.m[1]), d1(c.m[2], c.m[3]);
+    for (int i = 0; i < kBlockDim; ++i) {
+        if ((i >> t) & 1) continue;
+        if ((static_cast<uint32_t>(i) & cm) != cm) continue;
+        const int i1 = i | (1 << t);
+        for (int col = 0; col < kBlockDim; ++col) {
+            Cplx& u0 = U[i * kBlockDim + col];
+            Cplx& u1 = U[i1 * kBlockDim + col];
+            if (c.diag) {
+                if (c.kind == I_DIAG) u0 *= d0;
+                u1 *= (c.kind == I_NEG1 ? Cplx(-1.0, 0.0) : d1);
+            } else {
+                const Cplx v0 = u0, v1 = u1;
+                u0 = a * v0 + b * v1;
+                u1 = cc * v0 + d * v1;
+            }
+        }
+    }
+}
+
+// Phase angle of a controlled-phase gate diag(1, e^{i theta}): the exact
+// angle the circuit gave for P gates (FP64), pi for Z.
+double phase_angle(const bfly_op& op, const Classified& c) {
+    if (op.kind == BFLY_OP_GATE && op.gate == BFLY_GATE_P) return op.theta;
+    if (c.kind == I_NEG1) return M_PI;
+    return std::atan2(c.m[3], c.m[2]);
+}
+
+int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, const bfly_exec_opts& opts,
+                     std::string& err) {
+    const int b = kBlockQubits;
+    const int K = n < 12 ? n : 12;
+    const uint64_t all = (1ull << n) - 1;
+    const int renorm = opts.renorm_every == 0 ? 8 : opts.renorm_every;
+    std::vector<uint32_t> remaining(nops);
+    for (uint64_t i = 0; i < nops; ++i) remaining[i] = static_cast<uint32_t>(i);
+    std::vector<Classified> cls(nops);
+    for (uint64_t i = 0; i < nops; ++i)
+        if (ops[i].kind == BFLY_OP_GATE) { cls[i] = classify(ops[i]); plan.gates += 1; }
+    auto emit_other = [&](const bfly_op& op) {
+        Step st{};
+        switch (op.kind) {
+            case BFLY_OP_BITREV: {
+                uint64_t mask = op.aux;
+                if (mask == all) mask = 0;
+                if (mask != 0 && popc(mask) <= 1) return;
+                st.kind = S_BITREV; st.mask = mask; break;
+            }
+            case BFLY_OP_SWAP:
+                if (op.target == op.arg_i) return;
+                st.kind = S_SWAP; st.a = op.target; st.b = op.arg_i; break;
+            case BFLY_OP_UNITARY:
+                if (op.arg_i <= 0) return;
+                st.kind = S_UNITARY; st.a = op.target; st.b = op.arg_i; st.controls = op.controls; st.uoff = op.aux; break;
+            case BFLY_OP_PHASE_FLIP:
+                st.kind = S_FLIP; st.mask = op.aux; st.controls = op.controls; break;
+            case BFLY_OP_DIFFUSE:
+                st.kind = S_DIFFUSE; st.mask = op.aux; st.a = op.arg_i; break;
+            default: return;
+        }
+        plan.steps.push_back(st);
+        if (st.kind != S_FLIP) plan.passes += 1;
+    };
+    int blocks = 0;
+    bool renorm_next = false;
+    size_t head = 0;
+    constexpr size_t kScanWindow = 8192;
+    std::vector<uint32_t> rest;
+    while (head < remaining.size()) {
+        const bfly_op& front = ops[remaining[head]];
+        if (front.kind != BFLY_OP_GATE) {
+            emit_other(front);
+            ++head;
+            continue;
+        }
+        uint64_t Q = 0, frozen = 0, nd_done = 0;  // block qubits; no more non-diagonal; had non-diagonal
+        int slot_of[64];
+        int qubit_of[kBlockQubits];
+        int nq = 0;
+        for (int q = 0; q < 64; ++q) slot_of[q] = -1;
+        std::vector<Cplx> U(kBlockDim * kBlockDim, Cplx(0.0, 0.0));
+        for (int i = 0; i < kBlockDim; ++i) U[i * kBlockDim + i] = 1.0;
+        std::vector<BlockPhase> pre, post;
+        uint64_t d_nd = 0, d_all = 0;
+        size_t placed = 0, scanned_after_defer = 0;
+        bool deferred_any = false;
+        rest.clear();
+        size_t idx = head;
+        for (; idx < remaining.size(); ++idx) {
+            const uint32_t oi = remaining[idx];
+            const bfly_op& op = ops[oi];
+            if (deferred_any && ++scanned_after_defer > kScanWindow) break;
+            if ((d_nd & all) == all) break;
+            if (op.kind != BFLY_OP_GATE) {
+                const uint64_t q = op_qubits(op, n);
+                d_nd |= q; d_all |= q;
+                rest.push_back(oi);
+                deferred_any = true;
+                continue;
+            }
+            const Classified& c = cls[oi];
+            const uint64_t qall = (1ull << op.target) | op.controls;
+            const uint64_t qnd = c.diag ? 0 : (1ull << op.target);
+            bool ok = !(qnd & d_all) && !(qall & d_nd);
+            // A qubit may join the block only if the phase records already
+            // placed stay linear in the slots: a pre-phase naming it outside
+            // the block moves it to its slot (at most one slot per record);
+            // post-phases (applied after the whole block) forbid it.
+            auto can_join = [&](uint64_t need) {
+                if (nq + popc(need) > b) return false;
+                for (const BlockPhase& ph : pre)
+                    if ((ph.omask & need) && (ph.slot >= 0 || popc(ph.omask & need) > 1)) return false;
+                for (const BlockPhase& ph : post)
+                    if (ph.omask & need) return false;
+                return true;
+            };
+            // Outside qubits that phase records name cannot fill an empty
+            // slot later (the kernel tests a record's omask on the column
+            // index, where the slot bits are zero): a new record must leave
+            // enough unnamed qubits for the slots still empty.
+            auto named_mask = [&]() {
+                uint64_t m = 0;
+                for (const BlockPhase& ph : pre) m |= ph.omask;
+                for (const BlockPhase& ph : post) m |= ph.omask;
+                return m;
+            };
+            auto filler_room = [&](uint64_t omask) {
+                const uint64_t free = all & ~Q & ~(named_mask() | omask);
+                return popc(free) >= b - nq;
+            };
+            auto join = [&](uint64_t need) {
+                for (int q = 0; q < n; ++q)
+                    if ((need >> q) & 1) { slot_of[q] = nq; qubit_of[nq++] = q; }
+                Q |= need;
+                for (BlockPhase& ph : pre)
+                    if (ph.omask & need) {
+                        ph.slot = slot_of[__builtin_ctzll(ph.omask & need)];
+                        ph.omask &= ~need;
+                    }
+            };
+            if (ok && !c.diag) {
+                const uint64_t need = qall & ~Q;
+                if ((qnd & frozen) || !can_join(need)) {
+                    ok = false;
+                } else {
+                    join(need);
+                    uint32_t cm = 0;
+                    for (int q = 0; q < n; ++q)
+                        if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
+                    fold_into_block(U, c, slot_of[op.target], cm);
+                    nd_done |= qnd;
+                }
+            } else if (ok) {
+                if ((qall & ~Q) == 0) {
+                    uint32_t cm = 0;
+                    for (int q = 0; q < n; ++q)
+                        if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
+                    fold_into_block(U, c, slot_of[op.target], cm);
+                } else if ((c.kind == I_PHASE1 || c.kind == I_NEG1) && popc(qall & Q) <= 1 &&
+                           filler_room(qall & ~Q)) {
+                    const uint64_t a = qall & Q;
+                    BlockPhase ph{};
+                    ph.omask = qall & ~Q;
+                    ph.slot = a ? slot_of[__builtin_ctzll(a)] : -1;
+                    ph.theta = phase_angle(op, c);
+                    if (!(a & nd_done)) {
+                        pre.push_back(ph);
+                    } else {
+                        post.push_back(ph);
+                        frozen |= a;
+                    }
+                } else if (can_join(qall & ~Q)) {
+                    // any other diagonal joins the block's unitary
+                    join(qall & ~Q);
+                    uint32_t cm = 0;
+                    for (int q = 0; q < n; ++q)
+                        if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
+                    fold_into_block(U, c, slot_of[op.target], cm);
+                } else {
+                    ok = false;
+                }
+            }
+            if (!ok) {
+                d_nd |= qnd; d_all |= qall;
+                rest.push_back(oi);
+                deferred_any = true;
+                continue;
+            }
+            ++placed;
+        }
+        if (placed == 0) {
+            // a gate on more qubits than a block holds runs alone
+            Step gst{};
+            gst.kind = S_GATE;
+            gst.gate = make_gate_launch(front, n);
+            plan.steps.push_back(gst);
+            plan.passes += 1;
+            remaining.erase(remaining.begin() + static_cast<long>(head));
+            continue;
+        }
+        // complete the slots with qubits no phase record names, then the
+        // tile with the lowest bits
+        uint64_t named = 0;
+        for (const BlockPhase& ph : pre) named |= ph.omask;
+        for (const BlockPhase& ph : post) named |= ph.omask;
+        for (int q = 0; nq < b && q < n; ++q)
+            if (!(((Q | named) >> q) & 1)) { slot_of[q] = nq; qubit_of[nq++] = q; Q |= 1ull << q; }
+        if (nq < b) {
+            err = "internal: no free qubit to complete a block";
+            return BFLY_EINVAL;
+        }
+        uint64_t tile = Q;
+        for (int q = 0; popc(tile) < K && q < n; ++q) tile |= 1ull << q;
+        Step st{};
+        st.kind = S_BLOCK;
+        TilePass& tp = st.tile;
+        tp.n = n;
+        tp.k = K;
+        tp.slots = kSlots;
+        int li = 0, oi2 = 0, local_of[64];
+        for (int q = 0; q < n; ++q) {
+            if ((tile >> q) & 1) { local_of[q] = li; tp.qpos[li++] = q; }
+            else { local_of[q] = -1; tp.opos[oi2++] = q; }
+        }
+        tp.nouter = oi2;
+        BlockPass& bp = st.block;
+        for (int j = 0; j < b; ++j) bp.qloc[j] = local_of[qubit_of[j]];
+        bp.renorm = renorm_next ? 1 : 0;
+        renorm_next = false;
+        bp.mat_off = plan.bmat.size();
+        for (int r = 0; r < 2 * kBlockDim; ++r)
+            for (int k2 = 0; k2 < 2 * kBlockDim; ++k2) {
+                const Cplx u = U[(r & 31) * kBlockDim + (k2 & 31)];
+                const double v = (r < 32) == (k2 < 32) ? u.real() : (r < 32 ? -u.imag() : u.imag());
+                plan.bmat.push_back(static_cast<float>(v));
+            }
+        bp.pre_first = static_cast<int32_t>(plan.bphase.size());
+        bp.pre_count = static_cast<int32_t>(pre.size());
+        plan.bphase.insert(plan.bphase.end(), pre.begin(), pre.end());
+        bp.post_first = static_cast<int32_t>(plan.bphase.size());
+        bp.post_count = static_cast<int32_t>(post.size());
+        plan.bphase.insert(plan.bphase.end(), post.begin(), post.end());
+        plan.steps.push_back(st);
+        plan.passes += 1;
+        ++blocks;
+        if (renorm > 0 && blocks % renorm == 0) {
+            Step nst{};
+            nst.kind = S_NORM;
+            plan.steps.push_back(nst);
+            renorm_next = true;
+        }
+        std::vector<uint32_t> next;
+        next.reserve(rest.size() + (remaining.size() - idx));
+        next.insert(next.end(), rest.begin(), rest.end());
+        next.insert(next.end(), remaining.begin() + static_cast<long>(idx), remaining.end());
+        remaining.swap(next);
+        head = 0;
+    }
+    // a trailing check normalises nothing: drop it
+    if (!plan.steps.empty() && plan.steps.back().kind == S_NORM) plan.steps.pop_back();
+    return BFLY_OK;
+}
+
 }  // namespace
 
 int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
@@ -701,7 +974,8 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
     const uint64_t all = (1ull << n) - 1;
     int k = opts.tile_bits > 0 ? opts.tile_bits : default_tile_bits(dtype);
     k = std::min(std::max(k, kMinTileBits), kMaxTileBits);
-    const bool fused = opts.mode == BFLY_MODE_FUSED && n >= kMinTileBits;
+    // BLOCK mode below 9 qubits (fewer than 16 columns of 32) runs fused tiles
+    const bool fused = (opts.mode == BFLY_MODE_FUSED || opts.mode == BFLY_MODE_BLOCK) && n >= kMinTileBits;
     if (k > n) k = n;
     int L = opts.low_bits > 0 ? opts.low_bits : default_low_bits(dtype);
     // the tile kernel stages 16-byte vectors: 2 (f64) or 4 (f32) consecutive
@@ -751,6 +1025,13 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
         if (st.kind != S_FLIP) plan.passes += 1;  // a phase flip touches 2^(n - |mask|) amplitudes
     };
 
+    if (opts.mode == BFLY_MODE_BLOCK && n >= 9) {
+        if (dtype != BFLY_C64) {
+            err = "BLOCK mode runs complex64 states (FP32 MFMA); use FUSED for complex128";
+            return BFLY_EUNSUPPORTED;
+        }
+        return build_block_plan(plan, n, ops, nops, opts, err);
+    }
     if (!fused) {
         for (uint64_t i = 0; i < nops; ++i) {
             const bfly_op& op = ops[i];

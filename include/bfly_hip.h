@@ -134,7 +134,8 @@ int bfly_state_reset(bfly_state* s);
 int bfly_state_upload(bfly_state* s, const void* re, const void* im, uint64_t len);
 int bfly_state_download(bfly_state* s, void* re, void* im, uint64_t len);
 /* Copy one state into another of the same shape and dtype (device to device). */
-int bfly_state_copy(bfly_state* dst, const bfly_state* src);
+int bfly_state_copy(bfly_state* dst, const bfly_state* src);   /* same qubit count; converts
+                                                                  complex64 <-> complex128 */
 int bfly_state_info(const bfly_state* s, int32_t* num_qubits, int32_t* dtype);
 /* Device pointers of the re / im arrays (for callers that share memory). */
 int bfly_state_device_ptrs(bfly_state* s, void** re, void** im);

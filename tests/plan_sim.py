@@ -60,6 +60,17 @@ def parse(text: str):
             steps.append({"kind": "bitrev", "mask": int(kv["mask"])})
         elif kind == "swap":
             steps.append({"kind": "swap", "a": int(kv["a"]), "b": int(kv["b"])})
+        elif kind == "block":
+            cur_tile = None
+            steps.append({"kind": "block", "k": int(kv["k"]), "renorm": int(kv["renorm"]),
+                          "qloc": [int(x) for x in kv["qloc"].split(",")],
+                          "qpos": [int(x) for x in kv["qpos"].split(",")],
+                          "mat": np.array([float.fromhex(x) for x in kv["mat"].split(",")]).reshape(64, 64),
+                          "pre": [], "post": []})
+        elif kind == "phase":
+            steps[-1][kv["when"]].append((int(kv["omask"]), int(kv["slot"]), float.fromhex(kv["theta"])))
+        elif kind == "norm":
+            steps.append({"kind": "norm"})
         elif kind == "flip":
             steps.append({"kind": "flip", "mask": int(kv["mask"]), "value": int(kv["value"])})
         elif kind == "diffuse":
@@ -203,6 +214,38 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
             for i, b in enumerate(pos):
                 perm |= ((idx >> b) & 1) << pos[len(pos) - 1 - i]
             psi = psi[perm]
+        elif st["kind"] == "block":
+            qbits = [st["qpos"][l] for l in st["qloc"]]
+            if st["renorm"]:
+                psi = psi / np.sqrt(np.vdot(psi, psi).real)
+
+            qmask = sum(1 << q for q in qbits)
+
+            def phases(lst):
+                # a record's omask is tested on the column index (slot bits
+                # cleared), as the block kernel does
+                ang = np.zeros(1 << n)
+                for om, slot, th in lst:
+                    assert om & qmask == 0, "phase record names a block qubit"
+                    hit = (idx & ~qmask & om) == om
+                    if slot >= 0:
+                        hit &= ((idx >> qbits[slot]) & 1) == 1
+                    ang += np.where(hit, th, 0.0)
+                return np.exp(1j * ang)
+
+            psi = psi * phases(st["pre"])
+            m = st["mat"]
+            U = m[:32, :32] + 1j * m[32:, :32]
+            bases = idx[(idx & qmask) == 0]
+            e = np.arange(32)
+            offs = np.zeros(32, dtype=np.int64)
+            for jslot, q in enumerate(qbits):
+                offs |= ((e >> jslot) & 1) << q
+            cols = bases[:, None] + offs[None, :]
+            psi[cols] = psi[cols] @ U.T
+            psi = psi * phases(st["post"])
+        elif st["kind"] == "norm":
+            pass
         elif st["kind"] == "flip":
             psi = np.where((idx & st["mask"]) == st["value"], -psi, psi)
         elif st["kind"] == "diffuse":

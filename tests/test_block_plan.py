@@ -1,0 +1,76 @@
+"""BLOCK mode schedules (complex64 on FP32 MFMA) on the CPU: the compiled
+blocks (32 x 32 unitaries in FP32, FP64 phase records), replayed by the plan
+interpreter, against the oracle; dense-block count and phase routing for the
+value-encoding circuit; refusal for complex128."""
+import math
+
+import numpy as np
+import pytest
+
+from code_mojo_amd import _lib, algos, lower_circuit
+from code_mojo_amd import workloads as W
+from code_mojo_amd.state import Plan
+from oracle import oracle as O
+
+from .helpers import random_ops
+from .plan_sim import run_plan
+
+BLOCK = _lib.MODE_BLOCK
+# the blocks' matrices are rounded to FP32 once per block
+TOL = 2e-6
+
+
+def oracle(n, ops, ud, psi):
+    re, im = psi.real.copy(), psi.imag.copy()
+    O.execute(n, re, im, ops, ud)
+    return re + 1j * im
+
+
+@pytest.mark.parametrize("n,seed", [(9, 1), (10, 2), (12, 3), (13, 4)])
+def test_random_gates_block_schedule(n, seed):
+    rng = np.random.default_rng(seed)
+    ops, ud = random_ops(rng, n, 70, with_blocks=False)
+    psi = W.normal_state(n, seed).to_complex()
+    plan = Plan(n, np.float32, ops, ud, _lib.opts(mode=BLOCK, renorm_every=-1))
+    text = plan.dump()
+    assert text.count("block ") > 0
+    got = run_plan(text, psi, ud)
+    assert np.abs(got - oracle(n, ops, ud, psi)).max() < TOL
+
+
+@pytest.mark.parametrize("n", [10, 14])
+def test_value_encoding_blocks(n):
+    qc, v = W.value_encoding_precision(n, 11 + n)
+    ops, ud = lower_circuit(qc)
+    plan = Plan(n, np.float32, ops, ud, _lib.opts(mode=BLOCK))
+    text = plan.dump()
+    nblocks = text.count("\nblock ")
+    # H layer + phases + inverse QFT: a few blocks per 5 qubits, not one per gate
+    assert 0 < nblocks <= 3 * math.ceil(n / 5) + 2, nblocks
+    assert text.count("phase when=") > 0          # ladder phases across blocks, in FP64
+    z = np.zeros(1 << n, complex)
+    z[0] = 1
+    got = run_plan(text, z, ud)
+    rev = int(format(v, f"0{n}b")[::-1], 2)
+    assert abs(abs(got[rev]) ** 2 - 1) < 1e-5
+    assert np.abs(got - oracle(n, ops, ud, z)).max() < TOL
+
+
+def test_renorm_steps_every_eight_blocks():
+    n = 12
+    ops, ud = lower_circuit(W.random_layered(n, 12, 5))
+    text = Plan(n, np.float32, ops, ud, _lib.opts(mode=BLOCK)).dump()
+    lines = [ln.split()[0] for ln in text.split("\n")[1:] if ln.startswith(("block", "norm"))]
+    nb = lines.count("block")
+    assert nb > 8 and lines.count("norm") == (nb - 1) // 8
+    assert "norm" not in lines[-1:]
+    assert text.count("renorm=1") == lines.count("norm")
+
+
+def test_block_mode_complex128_refused_and_small_states_fused():
+    ops, ud = lower_circuit(W.qft_circuit(12))
+    with pytest.raises(_lib.BflyError, match="complex64"):
+        Plan(12, np.float64, ops, ud, _lib.opts(mode=BLOCK))
+    ops, ud = lower_circuit(W.qft_circuit(7))
+    text = Plan(7, np.float32, ops, ud, _lib.opts(mode=BLOCK)).dump()
+    assert "block " not in text and "tile " in text
