@@ -142,8 +142,14 @@ struct BlockPass {
     int32_t qloc[kBlockQubits];  // tile-local bit of each slot
     int32_t renorm;              // 1: scale by 1 / sqrt(scratch[0]) on load
     uint64_t mat_off;            // first float of the 64 x 64 real form of U in HostPlan::bmat
-    int32_t pre_first, pre_count;
+    int32_t pre_first, pre_count;    // phase records whose omask has 2 or more bits
     int32_t post_first, post_count;
+    // angle tables (HostPlan::btab), pre then post, n - 4 rows of 6 angles
+    // (c0 and the 5 slot coefficients): row 0 constant, rows 1 .. k - 5 the
+    // column bits (ascending local position), then the outer bits (tile
+    // index bits); a row adds its angles where its bit is set
+    uint64_t tab_off;
+    int32_t pre_tab, post_tab;       // 1: that table has a nonzero angle
 };
 
 struct Step {
@@ -167,6 +173,7 @@ struct HostPlan {
     std::vector<double> udata;   // complex, interleaved
     std::vector<float> bmat;     // BLOCK mode: 64 x 64 real forms of the blocks' U, row-major
     std::vector<BlockPhase> bphase;
+    std::vector<double> btab;          // BLOCK mode angle tables
     int64_t gates = 0;
     int64_t passes = 0;          // HBM sweeps over the state
 };

@@ -28,6 +28,7 @@ struct bfly_plan {
     double* d_udata = nullptr;
     float* d_bmat = nullptr;             // BLOCK mode block matrices
     bfly::BlockPhase* d_bphase = nullptr;  // BLOCK mode phase records
+    double* d_btab = nullptr;              // BLOCK mode angle tables
     bool borrowed = false;   // device records live in a buffer the plan does not own
 };
 

@@ -729,30 +729,39 @@ hipError_t launch_tile_t(T* re, T* im, const TilePass& P, const TileSweep* sw, c
 // pre-phase, form the B operand (64 x 16: re over im) of
 //   [out_re; out_im] = [[Ur, -Ui], [Ui, Ur]] [in_re; in_im]
 // computed by 64 v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation);
-// the 64 x 64 A operand stays in registers for the whole pass.  Phase angles
-// are summed in FP64 per column (exact for dyadic pi / 2^k ladders), turned
-// into factors by FP64 sincos and products, and rounded to FP32 once.
+// the 64 x 64 A operand stays in registers for the whole pass.
+// Phase records contribute exp(i (c0 + sum_j c_j bit_j)) per column: the
+// angles c are summed in FP64 (exact for dyadic pi / 2^k ladders) once per
+// tile and column, turned into factors by FP64 sincos, kept in LDS, and each
+// lane forms its 8 element factors by FP64 products rounded to FP32 once.
+// The next tile's amplitudes are loaded into registers while the current
+// tile is multiplied.
 typedef float f4_t __attribute__((ext_vector_type(4)));
-
-// exp(i (c0 + sum_j c_j bit_j)) coefficients of one column (FP64).
-__device__ __forceinline__ void block_phase_coeffs(const BlockPhase* __restrict__ ph, int count, uint64_t colglobal,
-                                                   double* c) {
-#pragma unroll
-    for (int j = 0; j <= kBlockQubits; ++j) c[j] = 0.0;
-    for (int i = 0; i < count; ++i) {
-        const BlockPhase q = ph[i];
-        if ((colglobal & q.omask) == q.omask) c[q.slot + 1] += q.theta;
-    }
-}
 
 struct Z64 {
     double r, i;
 };
 __device__ __forceinline__ Z64 zmul(Z64 a, Z64 b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
 __device__ __forceinline__ Z64 zexp(double t) {
+    if (t == 0.0) return {1.0, 0.0};
     double sn, cs;
     sincos(t, &sn, &cs);
     return {cs, sn};
+}
+
+// The 8 factors of elements e = base_bits + {0, s1, s2, s1 + s2} + {0, s3}
+// (four bit positions of e): f[m + 4 h] for m over (s1, s2), h over s3.
+__device__ __forceinline__ void element_factors(const Z64 z0, const Z64 z1, const Z64 z2, const Z64 z3, float* fr,
+                                                float* fi) {
+    Z64 f[8];
+    f[0] = z0;
+    f[1] = zmul(f[0], z1);
+    f[2] = zmul(f[0], z2);
+    f[3] = zmul(f[1], z2);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) f[4 + m] = zmul(f[m], z3);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) { fr[m] = static_cast<float>(f[m].r); fi[m] = static_cast<float>(f[m].i); }
 }
 
 template <int K>
@@ -760,17 +769,26 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
                                                          const TilePass P, const BlockPass B,
                                                          const float* __restrict__ mat,
                                                          const BlockPhase* __restrict__ ph,
+                                                         const double* __restrict__ tab,
                                                          const double* __restrict__ norm2) {
     constexpr uint32_t TS = 1u << K;
     constexpr int NT = 256;
     constexpr int VN = 4;
     constexpr uint32_t NV = TS / VN;
-    constexpr int NGROUP = static_cast<int>(TS / kBlockDim / 16);  // 16-column groups per tile
+    constexpr int NPF = static_cast<int>((NV + NT - 1) / NT);   // float4 per thread and part
+    constexpr int NCOL = static_cast<int>(TS / kBlockDim);       // 32-amplitude columns per tile
+    constexpr int NGROUP = NCOL / 16;                            // 16-column groups per tile
+    constexpr int NCB = K - kBlockQubits;                        // column bits
+    constexpr int NZ = kBlockQubits + 1;
+    static_assert(NT * VN >= 1024 || NPF == 1, "part stride");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* s_re = reinterpret_cast<float*>(lds_raw);
     float* s_im = s_re + TS;
+    double2* s_z = reinterpret_cast<double2*>(s_im + TS);        // [2][NCOL][NZ]: pre, post factors
+    double* s_tp = reinterpret_cast<double*>(s_z + 2 * NCOL * NZ);  // [2][NZ]: the tile's angles
     const uint32_t tid = threadIdx.x;
-    const int lane = static_cast<int>(tid & 63u), wave = static_cast<int>(tid >> 6);
+    const int lane = static_cast<int>(tid & 63u);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
     const int g = lane >> 4;  // k offset of the lane's B / A fragments, row quad of its output
 
     float afrag[4][16];
@@ -789,67 +807,131 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
         for (int j = 0; j < kBlockQubits; ++j) x |= static_cast<uint32_t>((e >> j) & 1) << B.qloc[j];
         return x;
     };
+    auto col_local = [&](uint32_t col) {   // deposit of the column bits into wmask
+        uint32_t colloc = 0, x = col, m = wmask;
+        while (m) {
+            const uint32_t low = m & (~m + 1);
+            if (x & 1) colloc |= low;
+            x >>= 1;
+            m ^= low;
+        }
+        return colloc;
+    };
+    // Deposits and the LDS swizzle are linear over GF(2), so every address
+    // splits into a per-lane part computed here and a uniform part.
+    const uint32_t lane_j = VN * tid;                       // first local index of the thread's float4 (part 0)
+    const uint64_t off_lane = local_to_global<K>(lane_j & (TS - 1), P);
+    const uint32_t lds_lane = swz<float>(lane_j & (TS - 1));
+    const uint32_t dl = col_local(static_cast<uint32_t>(lane & 15));
+    const uint32_t rl_in = swz<float>(dl ^ eloc(g));        // B-fragment rows e = g + 4 m
+    const uint32_t rl_out = swz<float>(dl ^ eloc(4 * g));   // output rows e = 16 h + 4 g + v
     const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
+    const bool pre_any = B.pre_tab || B.pre_count > 0;
+    const bool post_any = B.post_tab || B.post_count > 0;
+    const int nrows = P.n - kBlockQubits + 1;
 
     const uint64_t ntiles = 1ull << (P.n - P.k);
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    float4 pa[NPF], pb[NPF];
+    auto fetch = [&](uint64_t base) {
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint64_t gi = base + off_lane + local_to_global<K>(static_cast<uint32_t>(c * NT * VN), P);
+                pa[c] = *reinterpret_cast<const float4*>(re + gi);
+                pb[c] = *reinterpret_cast<const float4*>(im + gi);
+            }
+        }
+    };
+    uint64_t tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile_base<K>(tile, P));
+    for (; tile < ntiles; tile += gridDim.x) {
         const uint64_t base = tile_base<K>(tile, P);
-        for (uint32_t v = tid; v < NV; v += NT) {
-            const uint32_t j = v * VN;
-            const uint64_t gi = base + local_to_global<K>(j, P);
-            const float4 a = *reinterpret_cast<const float4*>(re + gi);
-            const float4 b = *reinterpret_cast<const float4*>(im + gi);
-            s_re[swz<float>(j)] = a.x; s_re[swz<float>(j + 1)] = a.y;
-            s_re[swz<float>(j + 2)] = a.z; s_re[swz<float>(j + 3)] = a.w;
-            s_im[swz<float>(j)] = b.x; s_im[swz<float>(j + 1)] = b.y;
-            s_im[swz<float>(j + 2)] = b.z; s_im[swz<float>(j + 3)] = b.w;
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint32_t a = lds_lane + static_cast<uint32_t>(c * NT * VN);
+                s_re[a] = pa[c].x; s_re[a ^ 1u] = pa[c].y; s_re[a ^ 2u] = pa[c].z; s_re[a ^ 3u] = pa[c].w;
+                s_im[a] = pb[c].x; s_im[a ^ 1u] = pb[c].y; s_im[a ^ 2u] = pb[c].z; s_im[a ^ 3u] = pb[c].w;
+            }
+        }
+        if (pre_any || post_any) {
+            // the tile's share of the angles: constant row plus its outer bits
+            if (tid < 2 * NZ) {
+                const int w = static_cast<int>(tid) / NZ, j = static_cast<int>(tid) % NZ;
+                const double* t = tab + B.tab_off + static_cast<size_t>(w) * nrows * NZ;
+                double a = t[j];
+                for (int i = 0; i < P.nouter; ++i)
+                    if ((tile >> i) & 1ull) a += t[(1 + NCB + i) * NZ + j];
+                s_tp[tid] = a;
+            }
+            __syncthreads();
+            // per column: the column bits' rows, multi-bit records, FP64 sincos
+            for (uint32_t w = tid; w < 2u * NCOL; w += NT) {
+                const uint32_t col = w % NCOL;
+                const int post = w >= static_cast<uint32_t>(NCOL) ? 1 : 0;
+                if (post ? !post_any : !pre_any) continue;
+                const double* t = tab + B.tab_off + static_cast<size_t>(post) * nrows * NZ;
+                double c[NZ];
+#pragma unroll
+                for (int j = 0; j < NZ; ++j) c[j] = s_tp[post * NZ + j];
+                if (post ? B.post_tab : B.pre_tab) {
+#pragma unroll
+                    for (int i = 0; i < NCB; ++i)
+                        if ((col >> i) & 1u) {
+#pragma unroll
+                            for (int j = 0; j < NZ; ++j) c[j] += t[(1 + i) * NZ + j];
+                        }
+                }
+                const int count = post ? B.post_count : B.pre_count;
+                if (count > 0) {
+                    const int first = post ? B.post_first : B.pre_first;
+                    const uint64_t colglobal = base + local_to_global<K>(col_local(col), P);
+                    for (int i = 0; i < count; ++i) {
+                        const BlockPhase q = ph[first + i];
+                        if ((colglobal & q.omask) == q.omask) c[q.slot + 1] += q.theta;
+                    }
+                }
+                double2* out = s_z + post * NCOL * NZ + col * NZ;
+#pragma unroll
+                for (int j = 0; j < NZ; ++j) {
+                    Z64 z = zexp(c[j]);
+                    if (j == 0 && !post) { z.r *= scale; z.i *= scale; }
+                    out[j] = make_double2(z.r, z.i);
+                }
+            }
         }
         __syncthreads();
+        {
+            const uint64_t nt = tile + gridDim.x;
+            if (nt < ntiles) fetch(tile_base<K>(nt, P));
+        }
         for (int cg = wave; cg < NGROUP; cg += NT / 64) {
             const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
-            uint32_t colloc = 0;
-            {
-                uint32_t x = col, m = wmask;
-                while (m) {
-                    const uint32_t low = m & (~m + 1);
-                    if (x & 1) colloc |= low;
-                    x >>= 1;
-                    m ^= low;
-                }
-            }
-            const uint64_t colglobal = base + local_to_global<K>(colloc, P);
-            // pre-phase factors of the lane's 8 elements e = g + 4 m
-            float fr[8], fi[8];
-            {
-                Z64 f[8];
-                f[0] = {scale, 0.0};
-                if (B.pre_count > 0) {
-                    double c[kBlockQubits + 1];
-                    block_phase_coeffs(ph + B.pre_first, B.pre_count, colglobal, c);
-                    Z64 base_f = zexp(c[0]);
-                    if (g & 1) base_f = zmul(base_f, zexp(c[1]));
-                    if (g & 2) base_f = zmul(base_f, zexp(c[2]));
-                    f[0] = {base_f.r * scale, base_f.i * scale};
-                    const Z64 z2 = zexp(c[3]), z3 = zexp(c[4]), z4 = zexp(c[5]);
-                    f[1] = zmul(f[0], z2);
-                    f[2] = zmul(f[0], z3);
-                    f[3] = zmul(f[1], z3);
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) f[4 + m] = zmul(f[m], z4);
-                } else {
-#pragma unroll
-                    for (int m = 1; m < 8; ++m) f[m] = f[0];
-                }
-#pragma unroll
-                for (int m = 0; m < 8; ++m) { fr[m] = static_cast<float>(f[m].r); fi[m] = static_cast<float>(f[m].i); }
-            }
+            const uint32_t ucg = swz<float>(col_local(static_cast<uint32_t>(cg * 16)));
             float bfrag[16];
+            if (pre_any) {
+                // pre-phase factors of the lane's 8 elements e = g + 4 m
+                float fr[8], fi[8];
+                const double2* z = s_z + col * NZ;
+                Z64 b0 = {z[0].x, z[0].y};
+                if (g & 1) b0 = zmul(b0, Z64{z[1].x, z[1].y});
+                if (g & 2) b0 = zmul(b0, Z64{z[2].x, z[2].y});
+                element_factors(b0, Z64{z[3].x, z[3].y}, Z64{z[4].x, z[4].y}, Z64{z[5].x, z[5].y}, fr, fi);
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const uint32_t j = swz<float>(colloc | eloc(g + 4 * m));
-                const float xr = s_re[j], xi = s_im[j];
-                bfrag[m] = xr * fr[m] - xi * fi[m];
-                bfrag[m + 8] = xr * fi[m] + xi * fr[m];
+                for (int m = 0; m < 8; ++m) {
+                    const uint32_t j = rl_in ^ ucg ^ swz<float>(eloc(4 * m));
+                    const float xr = s_re[j], xi = s_im[j];
+                    bfrag[m] = xr * fr[m] - xi * fi[m];
+                    bfrag[m + 8] = xr * fi[m] + xi * fr[m];
+                }
+            } else {
+                const float sc = static_cast<float>(scale);
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const uint32_t j = rl_in ^ ucg ^ swz<float>(eloc(4 * m));
+                    bfrag[m] = s_re[j] * sc;
+                    bfrag[m + 8] = s_im[j] * sc;
+                }
             }
             f4_t acc[4];
 #pragma unroll
@@ -860,52 +942,46 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
                     acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[r][s2], bfrag[s2], acc[r], 0, 0, 0);
             }
             // lane holds rows 16 r + 4 g + v: re of e = 16 h + 4 g + v (r = h), im (r = 2 + h)
-            float pr[8], pi2[8];
-            {
-                Z64 f[8];
-                f[0] = {1.0, 0.0};
-                if (B.post_count > 0) {
-                    double c[kBlockQubits + 1];
-                    block_phase_coeffs(ph + B.post_first, B.post_count, colglobal, c);
-                    Z64 base_f = zexp(c[0]);
-                    if (g & 1) base_f = zmul(base_f, zexp(c[3]));
-                    if (g & 2) base_f = zmul(base_f, zexp(c[4]));
-                    const Z64 z0 = zexp(c[1]), z1 = zexp(c[2]), z4 = zexp(c[5]);
-                    f[0] = base_f;          // index h * 4 + v
-                    f[1] = zmul(f[0], z0);
-                    f[2] = zmul(f[0], z1);
-                    f[3] = zmul(f[1], z1);
+            if (post_any) {
+                float pr[8], pi2[8];
+                const double2* z = s_z + NCOL * NZ + col * NZ;
+                Z64 b0 = {z[0].x, z[0].y};
+                if (g & 1) b0 = zmul(b0, Z64{z[3].x, z[3].y});
+                if (g & 2) b0 = zmul(b0, Z64{z[4].x, z[4].y});
+                element_factors(b0, Z64{z[1].x, z[1].y}, Z64{z[2].x, z[2].y}, Z64{z[5].x, z[5].y}, pr, pi2);
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) f[4 + v] = zmul(f[v], z4);
-                } else {
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-                    for (int v = 1; v < 8; ++v) f[v] = f[0];
-                }
+                    for (int v = 0; v < 4; ++v) {
+                        const float yr = acc[h][v], yi = acc[2 + h][v];
+                        const int q = 4 * h + v;
+                        const uint32_t j = rl_out ^ ucg ^ swz<float>(eloc(16 * h + v));
+                        s_re[j] = yr * pr[q] - yi * pi2[q];
+                        s_im[j] = yr * pi2[q] + yi * pr[q];
+                    }
+            } else {
 #pragma unroll
-                for (int v = 0; v < 8; ++v) { pr[v] = static_cast<float>(f[v].r); pi2[v] = static_cast<float>(f[v].i); }
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const uint32_t j = rl_out ^ ucg ^ swz<float>(eloc(16 * h + v));
+                        s_re[j] = acc[h][v];
+                        s_im[j] = acc[2 + h][v];
+                    }
             }
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float yr = acc[h][v], yi = acc[2 + h][v];
-                    const int q = 4 * h + v;
-                    const uint32_t j = swz<float>(colloc | eloc(16 * h + 4 * g + v));
-                    s_re[j] = yr * pr[q] - yi * pi2[q];
-                    s_im[j] = yr * pi2[q] + yi * pr[q];
-                }
         }
         __syncthreads();
-        for (uint32_t v = tid; v < NV; v += NT) {
-            const uint32_t j = v * VN;
-            float4 a, b;
-            a.x = s_re[swz<float>(j)]; a.y = s_re[swz<float>(j + 1)];
-            a.z = s_re[swz<float>(j + 2)]; a.w = s_re[swz<float>(j + 3)];
-            b.x = s_im[swz<float>(j)]; b.y = s_im[swz<float>(j + 1)];
-            b.z = s_im[swz<float>(j + 2)]; b.w = s_im[swz<float>(j + 3)];
-            const uint64_t gi = base + local_to_global<K>(j, P);
-            *reinterpret_cast<float4*>(re + gi) = a;
-            *reinterpret_cast<float4*>(im + gi) = b;
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint32_t a = lds_lane + static_cast<uint32_t>(c * NT * VN);
+                float4 x, y;
+                x.x = s_re[a]; x.y = s_re[a ^ 1u]; x.z = s_re[a ^ 2u]; x.w = s_re[a ^ 3u];
+                y.x = s_im[a]; y.y = s_im[a ^ 1u]; y.z = s_im[a ^ 2u]; y.w = s_im[a ^ 3u];
+                const uint64_t gi = base + off_lane + local_to_global<K>(static_cast<uint32_t>(c * NT * VN), P);
+                *reinterpret_cast<float4*>(re + gi) = x;
+                *reinterpret_cast<float4*>(im + gi) = y;
+            }
         }
         __syncthreads();
     }
@@ -913,14 +989,16 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
 
 template <int K>
 hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
-                          const BlockPhase* ph, const double* norm2, hipStream_t st) {
-    const size_t lds = 2ull * (1ull << K) * sizeof(float);
+                          const BlockPhase* ph, const double* tab, const double* norm2, hipStream_t st) {
+    // the tile (re, im), the per-column pre / post factors (6 complex FP64
+    // each) and the tile's 2 x 6 angles
+    const size_t lds = 2ull * (1ull << K) * sizeof(float) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 6 * 8;
     static int grid_cap = 0;
     hipError_t e = persistent_grid(block_mfma_kernel<K>, 256, lds, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
-    hipLaunchKernelGGL(block_mfma_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, ph, norm2);
+    hipLaunchKernelGGL(block_mfma_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, ph, tab, norm2);
     return hipGetLastError();
 }
 
@@ -1525,15 +1603,15 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
 }
 
 hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
-                             const BlockPhase* ph, const double* norm2, hipStream_t stream) {
+                             const BlockPhase* ph, const double* tab, const double* norm2, hipStream_t stream) {
     if (dtype != BFLY_C64) return hipErrorInvalidValue;
     float* r = static_cast<float*>(re);
     float* i = static_cast<float*>(im);
     switch (P.k) {
-        case 9: return launch_block_k<9>(r, i, P, B, mat, ph, norm2, stream);
-        case 10: return launch_block_k<10>(r, i, P, B, mat, ph, norm2, stream);
-        case 11: return launch_block_k<11>(r, i, P, B, mat, ph, norm2, stream);
-        case 12: return launch_block_k<12>(r, i, P, B, mat, ph, norm2, stream);
+        case 9: return launch_block_k<9>(r, i, P, B, mat, ph, tab, norm2, stream);
+        case 10: return launch_block_k<10>(r, i, P, B, mat, ph, tab, norm2, stream);
+        case 11: return launch_block_k<11>(r, i, P, B, mat, ph, tab, norm2, stream);
+        case 12: return launch_block_k<12>(r, i, P, B, mat, ph, tab, norm2, stream);
     }
     return hipErrorInvalidValue;
 }

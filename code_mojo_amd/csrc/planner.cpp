@@ -759,6 +759,16 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
         plan.steps.push_back(st);
         if (st.kind != S_FLIP) plan.passes += 1;
     };
+    // does a non-diagonal gate on one of `need` follow within the next ops?
+    constexpr size_t kLookahead = 256;
+    auto upcoming_nd = [&](uint64_t need, size_t from) {
+        const size_t lim = std::min(remaining.size(), from + 1 + kLookahead);
+        for (size_t j = from + 1; j < lim; ++j) {
+            const bfly_op& o = ops[remaining[j]];
+            if (o.kind == BFLY_OP_GATE && !cls[remaining[j]].diag && ((need >> o.target) & 1)) return true;
+        }
+        return false;
+    };
     int blocks = 0;
     bool renorm_next = false;
     size_t head = 0;
@@ -854,6 +864,15 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
                     for (int q = 0; q < n; ++q)
                         if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
                     fold_into_block(U, c, slot_of[op.target], cm);
+                } else if (upcoming_nd(qall & ~Q, idx) && can_join(qall & ~Q)) {
+                    // a qubit that a non-diagonal gate needs soon joins now,
+                    // so the phase stays in U instead of pinning the qubit
+                    // outside the block
+                    join(qall & ~Q);
+                    uint32_t cm = 0;
+                    for (int q = 0; q < n; ++q)
+                        if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
+                    fold_into_block(U, c, slot_of[op.target], cm);
                 } else if ((c.kind == I_PHASE1 || c.kind == I_NEG1) && popc(qall & Q) <= 1 &&
                            filler_room(qall & ~Q)) {
                     const uint64_t a = qall & Q;
@@ -932,12 +951,36 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
                 const double v = (r < 32) == (k2 < 32) ? u.real() : (r < 32 ? -u.imag() : u.imag());
                 plan.bmat.push_back(static_cast<float>(v));
             }
-        bp.pre_first = static_cast<int32_t>(plan.bphase.size());
-        bp.pre_count = static_cast<int32_t>(pre.size());
-        plan.bphase.insert(plan.bphase.end(), pre.begin(), pre.end());
-        bp.post_first = static_cast<int32_t>(plan.bphase.size());
-        bp.post_count = static_cast<int32_t>(post.size());
-        plan.bphase.insert(plan.bphase.end(), post.begin(), post.end());
+        // records with at most one outside bit go to the angle tables (row of
+        // their bit), the others stay records
+        {
+            int row_of[64];
+            int nrows = 1;
+            uint32_t qm = 0;
+            for (int j = 0; j < b; ++j) qm |= 1u << bp.qloc[j];
+            for (int i = 0; i < K; ++i)
+                if (!((qm >> i) & 1)) row_of[tp.qpos[i]] = nrows++;
+            for (int i = 0; i < tp.nouter; ++i) row_of[tp.opos[i]] = nrows++;
+            bp.tab_off = plan.btab.size();
+            plan.btab.resize(plan.btab.size() + 2 * static_cast<size_t>(nrows) * 6, 0.0);
+            for (int w = 0; w < 2; ++w) {
+                const std::vector<BlockPhase>& recs = w ? post : pre;
+                double* t = plan.btab.data() + bp.tab_off + static_cast<size_t>(w) * nrows * 6;
+                int first = static_cast<int32_t>(plan.bphase.size()), count = 0, tab = 0;
+                for (const BlockPhase& r : recs) {
+                    if (popc(r.omask) <= 1) {
+                        const int row = r.omask ? row_of[__builtin_ctzll(r.omask)] : 0;
+                        t[row * 6 + r.slot + 1] += r.theta;
+                        tab = 1;
+                    } else {
+                        plan.bphase.push_back(r);
+                        ++count;
+                    }
+                }
+                if (w) { bp.post_first = first; bp.post_count = count; bp.post_tab = tab; }
+                else { bp.pre_first = first; bp.pre_count = count; bp.pre_tab = tab; }
+            }
+        }
         plan.steps.push_back(st);
         plan.passes += 1;
         ++blocks;
