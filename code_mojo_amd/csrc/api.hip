@@ -115,11 +115,6 @@ int upload_plan(bfly_plan* p, int device) {
         HIP_TRY(hipMemcpy(p->d_bmat, hp.bmat.data(), hp.bmat.size() * sizeof(float), hipMemcpyHostToDevice),
                 "hipMemcpy(plan blocks)");
     }
-    if (!hp.bphase.empty()) {
-        HIP_TRY(hipMalloc(&p->d_bphase, hp.bphase.size() * sizeof(BlockPhase)), "hipMalloc(plan phases)");
-        HIP_TRY(hipMemcpy(p->d_bphase, hp.bphase.data(), hp.bphase.size() * sizeof(BlockPhase), hipMemcpyHostToDevice),
-                "hipMemcpy(plan phases)");
-    }
     if (!hp.btab.empty()) {
         HIP_TRY(hipMalloc(&p->d_btab, hp.btab.size() * sizeof(double)), "hipMalloc(plan angle tables)");
         HIP_TRY(hipMemcpy(p->d_btab, hp.btab.data(), hp.btab.size() * sizeof(double), hipMemcpyHostToDevice),
@@ -148,7 +143,7 @@ size_t plan_device_bytes(const bfly_plan* p) {
     const HostPlan& hp = p->hp;
     return align256(hp.sweeps.size() * sizeof(TileSweep)) + align256(hp.instrs.size() * sizeof(TileInstr)) +
            align256(hp.udata.size() * sizeof(double)) + align256(hp.bmat.size() * sizeof(float)) +
-           align256(hp.bphase.size() * sizeof(BlockPhase)) + align256(hp.btab.size() * sizeof(double));
+           align256(hp.btab.size() * sizeof(double));
 }
 
 int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream) {
@@ -159,19 +154,16 @@ int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream) {
     p->d_sweeps = bs ? reinterpret_cast<TileSweep*>(c) : nullptr;
     p->d_instrs = bi ? reinterpret_cast<TileInstr*>(c + align256(bs)) : nullptr;
     p->d_udata = bu ? reinterpret_cast<double*>(c + align256(bs) + align256(bi)) : nullptr;
-    const size_t bm = hp.bmat.size() * sizeof(float), bp = hp.bphase.size() * sizeof(BlockPhase);
+    const size_t bm = hp.bmat.size() * sizeof(float), bt = hp.btab.size() * sizeof(double);
     const size_t om = align256(bs) + align256(bi) + align256(bu);
     p->d_bmat = bm ? reinterpret_cast<float*>(c + om) : nullptr;
-    p->d_bphase = bp ? reinterpret_cast<BlockPhase*>(c + om + align256(bm)) : nullptr;
-    const size_t bt = hp.btab.size() * sizeof(double);
-    p->d_btab = bt ? reinterpret_cast<double*>(c + om + align256(bm) + align256(bp)) : nullptr;
+    p->d_btab = bt ? reinterpret_cast<double*>(c + om + align256(bm)) : nullptr;
     p->borrowed = true;
     p->device = device;
     if (bs) HIP_TRY(hipMemcpyAsync(p->d_sweeps, hp.sweeps.data(), bs, hipMemcpyHostToDevice, stream), "plan sweeps");
     if (bi) HIP_TRY(hipMemcpyAsync(p->d_instrs, hp.instrs.data(), bi, hipMemcpyHostToDevice, stream), "plan instrs");
     if (bu) HIP_TRY(hipMemcpyAsync(p->d_udata, hp.udata.data(), bu, hipMemcpyHostToDevice, stream), "plan unitaries");
     if (bm) HIP_TRY(hipMemcpyAsync(p->d_bmat, hp.bmat.data(), bm, hipMemcpyHostToDevice, stream), "plan blocks");
-    if (bp) HIP_TRY(hipMemcpyAsync(p->d_bphase, hp.bphase.data(), bp, hipMemcpyHostToDevice, stream), "plan phases");
     if (bt) HIP_TRY(hipMemcpyAsync(p->d_btab, hp.btab.data(), bt, hipMemcpyHostToDevice, stream), "plan angle tables");
     return BFLY_OK;
 }
@@ -369,7 +361,6 @@ int bfly_plan_destroy(bfly_plan* p) {
         if (p->d_instrs) (void)hipFree(p->d_instrs);
         if (p->d_udata) (void)hipFree(p->d_udata);
         if (p->d_bmat) (void)hipFree(p->d_bmat);
-        if (p->d_bphase) (void)hipFree(p->d_bphase);
         if (p->d_btab) (void)hipFree(p->d_btab);
     }
     delete p;
@@ -471,18 +462,12 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                 for (int w = 0; w < 2; ++w) {
                     const char* when = w ? "post" : "pre";
                     if (w ? bp.post_tab : bp.pre_tab) {
-                        const double* t = hp.btab.data() + bp.tab_off + static_cast<size_t>(w) * nrows * 6;
+                        const double* t = hp.brows.data() + bp.rows_off + static_cast<size_t>(w) * nrows * 6;
                         for (int r = 0; r < nrows; ++r)
                             for (int j = 0; j < 6; ++j)
                                 if (t[r * 6 + j] != 0.0)
                                     put("phase when=%s omask=%llu slot=%d theta=%a\n", when,
                                         static_cast<unsigned long long>(rowmask[r]), j - 1, t[r * 6 + j]);
-                    }
-                    const int first = w ? bp.post_first : bp.pre_first, count = w ? bp.post_count : bp.pre_count;
-                    for (int i = 0; i < count; ++i) {
-                        const BlockPhase& q = hp.bphase[first + i];
-                        put("phase when=%s omask=%llu slot=%d theta=%a\n", when,
-                            static_cast<unsigned long long>(q.omask), q.slot, q.theta);
                     }
                 }
                 break;
@@ -635,7 +620,7 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
         case S_DIFFUSE: return launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, stream);
         case S_BLOCK:
             return launch_block_pass(s->dtype, s->re, s->im, st.tile, st.block, p->d_bmat + st.block.mat_off,
-                                     p->d_bphase, p->d_btab, s->scratch, stream);
+                                     p->d_btab, s->scratch, stream);
         case S_NORM:
             return launch_reduce(s->dtype, R_NORM2, s->re, s->im, nullptr, nullptr, 1ull << s->n, 0, 0, s->scratch,
                                  stream);

@@ -130,26 +130,28 @@ enum StepKind : int32_t {
 // exp(i theta) when (i & omask) == omask and, for slot >= 0, the slot bit is 1.
 constexpr int kBlockQubits = 5;
 constexpr int kBlockDim = 1 << kBlockQubits;
-struct BlockPhase {
-    uint64_t omask;    // bits outside the block that must be 1 (global index bits)
+struct BlockPhase {     // planner record of one phase gate
+    uint64_t omask;    // bits outside the block that must be 1 (global index bits; at most one)
     int32_t slot;      // block slot that must be 1, or -1
     int32_t pad;
     double theta;      // radians, FP64 (exact for the dyadic pi / 2^k ladders)
 };
-static_assert(sizeof(BlockPhase) == 24, "BlockPhase layout");
 
+// Phase records of a block, pre (applied before U) and post, summed per
+// table row in FP64: row 0 is constant, rows 1 .. k - 5 belong to the
+// column bits (ascending local position), the rest to the outer bits (tile
+// index bits); a row adds its 6 angles (c0 and one per slot) where its bit
+// is 1.  Amplitude e of a column then takes exp(i (c0 + sum_j c_j e_j)).
+// The device gets, per block, the column factors exp(i c) of rows 0 and the
+// column bits as [2][2^(k-5)][6] complex FP64 (FP64 sincos of FP64 sums),
+// then the outer-bit angles as [2][n - k][6] FP64.
 struct BlockPass {
     int32_t qloc[kBlockQubits];  // tile-local bit of each slot
     int32_t renorm;              // 1: scale by 1 / sqrt(scratch[0]) on load
     uint64_t mat_off;            // first float of the 64 x 64 real form of U in HostPlan::bmat
-    int32_t pre_first, pre_count;    // phase records whose omask has 2 or more bits
-    int32_t post_first, post_count;
-    // angle tables (HostPlan::btab), pre then post, n - 4 rows of 6 angles
-    // (c0 and the 5 slot coefficients): row 0 constant, rows 1 .. k - 5 the
-    // column bits (ascending local position), then the outer bits (tile
-    // index bits); a row adds its angles where its bit is set
-    uint64_t tab_off;
-    int32_t pre_tab, post_tab;       // 1: that table has a nonzero angle
+    uint64_t tab_off;            // first double of the block's device tables in HostPlan::btab
+    uint64_t rows_off;           // first double of the block's row angles in HostPlan::brows
+    int32_t pre_tab, post_tab;   // 1: that side has a nonzero angle
 };
 
 struct Step {
@@ -172,8 +174,8 @@ struct HostPlan {
     std::vector<TileInstr> instrs;
     std::vector<double> udata;   // complex, interleaved
     std::vector<float> bmat;     // BLOCK mode: 64 x 64 real forms of the blocks' U, row-major
-    std::vector<BlockPhase> bphase;
-    std::vector<double> btab;          // BLOCK mode angle tables
+    std::vector<double> btab;          // BLOCK mode device tables (column factors, outer angles)
+    std::vector<double> brows;         // BLOCK mode row angles [2][n - 4][6] per block (host only)
     int64_t gates = 0;
     int64_t passes = 0;          // HBM sweeps over the state
 };

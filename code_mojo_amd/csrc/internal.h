@@ -27,8 +27,7 @@ struct bfly_plan {
     bfly::TileInstr* d_instrs = nullptr;
     double* d_udata = nullptr;
     float* d_bmat = nullptr;             // BLOCK mode block matrices
-    bfly::BlockPhase* d_bphase = nullptr;  // BLOCK mode phase records
-    double* d_btab = nullptr;              // BLOCK mode angle tables
+    double* d_btab = nullptr;              // BLOCK mode column factors and outer angles
     bool borrowed = false;   // device records live in a buffer the plan does not own
 };
 

@@ -30,10 +30,10 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
 hipError_t prepare_tile_kernels(int dtype, int k, int slots);
 // (b'') BLOCK mode pass (complex64): a 32 x 32 block on FP32 MFMA between
 // FP64-generated phase factors; `mat` is the block's 64 x 64 real form,
-// `ph` the plan's multi-bit phase records, `tab` its angle tables, `norm2`
-// the squared norm to divide out (B.renorm) left on the device by a reduction.
+// `tab` the plan's column factors and outer-bit angles, `norm2` the squared
+// norm to divide out (B.renorm) left on the device by a reduction.
 hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
-                             const BlockPhase* ph, const double* tab, const double* norm2, hipStream_t stream);
+                             const double* tab, const double* norm2, hipStream_t stream);
 // (a) one gate over the whole state (coalesced strided butterfly pairs).
 hipError_t launch_gate(int dtype, void* re, void* im, const GateLaunch& G, hipStream_t stream);
 // (c) bit reversal of all bits (mask 0) or of the bits in `mask`, in place.

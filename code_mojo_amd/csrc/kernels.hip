@@ -764,11 +764,16 @@ __device__ __forceinline__ void element_factors(const Z64 z0, const Z64 z1, cons
     for (int m = 0; m < 8; ++m) { fr[m] = static_cast<float>(f[m].r); fi[m] = static_cast<float>(f[m].i); }
 }
 
+// z_j = (column factor j) x (tile factor j), FP64.
+__device__ __forceinline__ void column_factors(const double2* cf, const double2* tf, Z64* z) {
+#pragma unroll
+    for (int j = 0; j <= kBlockQubits; ++j) z[j] = zmul(Z64{cf[j].x, cf[j].y}, Z64{tf[j].x, tf[j].y});
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re, float* __restrict__ im,
                                                          const TilePass P, const BlockPass B,
                                                          const float* __restrict__ mat,
-                                                         const BlockPhase* __restrict__ ph,
                                                          const double* __restrict__ tab,
                                                          const double* __restrict__ norm2) {
     constexpr uint32_t TS = 1u << K;
@@ -778,14 +783,13 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
     constexpr int NPF = static_cast<int>((NV + NT - 1) / NT);   // float4 per thread and part
     constexpr int NCOL = static_cast<int>(TS / kBlockDim);       // 32-amplitude columns per tile
     constexpr int NGROUP = NCOL / 16;                            // 16-column groups per tile
-    constexpr int NCB = K - kBlockQubits;                        // column bits
     constexpr int NZ = kBlockQubits + 1;
     static_assert(NT * VN >= 1024 || NPF == 1, "part stride");
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* s_re = reinterpret_cast<float*>(lds_raw);
     float* s_im = s_re + TS;
-    double2* s_z = reinterpret_cast<double2*>(s_im + TS);        // [2][NCOL][NZ]: pre, post factors
-    double* s_tp = reinterpret_cast<double*>(s_z + 2 * NCOL * NZ);  // [2][NZ]: the tile's angles
+    double2* s_cf = reinterpret_cast<double2*>(s_im + TS);       // [2][NCOL][NZ]: pre, post column factors
+    double2* s_tf = s_cf + 2 * NCOL * NZ;                        // [2 buffers][2][NZ]: tile factors
     const uint32_t tid = threadIdx.x;
     const int lane = static_cast<int>(tid & 63u);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
@@ -826,9 +830,27 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
     const uint32_t rl_in = swz<float>(dl ^ eloc(g));        // B-fragment rows e = g + 4 m
     const uint32_t rl_out = swz<float>(dl ^ eloc(4 * g));   // output rows e = 16 h + 4 g + v
     const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
-    const bool pre_any = B.pre_tab || B.pre_count > 0;
-    const bool post_any = B.post_tab || B.post_count > 0;
-    const int nrows = P.n - kBlockQubits + 1;
+    const bool pre_any = B.pre_tab != 0, post_any = B.post_tab != 0;
+    const double* outer_ang = tab + B.tab_off + 2 * NCOL * NZ * 2;   // [2][nouter][NZ]
+
+    // column factors stay in LDS for the whole launch
+    if (pre_any || post_any) {
+        const double2* cf = reinterpret_cast<const double2*>(tab + B.tab_off);
+        for (uint32_t i = tid; i < 2u * NCOL * NZ; i += NT) s_cf[i] = cf[i];
+    }
+    // the tile's factors exp(i sum of its outer-bit angles), pre scaled by `scale`
+    auto tile_factors = [&](uint64_t t, int buf) {
+        if (tid < 2 * NZ) {
+            const int w = static_cast<int>(tid) / NZ, j = static_cast<int>(tid) % NZ;
+            const double* a = outer_ang + static_cast<size_t>(w) * P.nouter * NZ;
+            double x = 0.0;
+            for (int i = 0; i < P.nouter; ++i)
+                if ((t >> i) & 1ull) x += a[i * NZ + j];
+            Z64 z = zexp(x);
+            if (w == 0 && j == 0) { z.r *= scale; z.i *= scale; }
+            s_tf[buf * 2 * NZ + tid] = make_double2(z.r, z.i);
+        }
+    };
 
     const uint64_t ntiles = 1ull << (P.n - P.k);
     float4 pa[NPF], pb[NPF];
@@ -843,8 +865,11 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
         }
     };
     uint64_t tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile_base<K>(tile, P));
-    for (; tile < ntiles; tile += gridDim.x) {
+    if (tile < ntiles) {
+        fetch(tile_base<K>(tile, P));
+        if (pre_any || post_any) tile_factors(tile, 0);
+    }
+    for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
         const uint64_t base = tile_base<K>(tile, P);
 #pragma unroll
         for (int c = 0; c < NPF; ++c) {
@@ -854,57 +879,15 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
                 s_im[a] = pb[c].x; s_im[a ^ 1u] = pb[c].y; s_im[a ^ 2u] = pb[c].z; s_im[a ^ 3u] = pb[c].w;
             }
         }
-        if (pre_any || post_any) {
-            // the tile's share of the angles: constant row plus its outer bits
-            if (tid < 2 * NZ) {
-                const int w = static_cast<int>(tid) / NZ, j = static_cast<int>(tid) % NZ;
-                const double* t = tab + B.tab_off + static_cast<size_t>(w) * nrows * NZ;
-                double a = t[j];
-                for (int i = 0; i < P.nouter; ++i)
-                    if ((tile >> i) & 1ull) a += t[(1 + NCB + i) * NZ + j];
-                s_tp[tid] = a;
-            }
-            __syncthreads();
-            // per column: the column bits' rows, multi-bit records, FP64 sincos
-            for (uint32_t w = tid; w < 2u * NCOL; w += NT) {
-                const uint32_t col = w % NCOL;
-                const int post = w >= static_cast<uint32_t>(NCOL) ? 1 : 0;
-                if (post ? !post_any : !pre_any) continue;
-                const double* t = tab + B.tab_off + static_cast<size_t>(post) * nrows * NZ;
-                double c[NZ];
-#pragma unroll
-                for (int j = 0; j < NZ; ++j) c[j] = s_tp[post * NZ + j];
-                if (post ? B.post_tab : B.pre_tab) {
-#pragma unroll
-                    for (int i = 0; i < NCB; ++i)
-                        if ((col >> i) & 1u) {
-#pragma unroll
-                            for (int j = 0; j < NZ; ++j) c[j] += t[(1 + i) * NZ + j];
-                        }
-                }
-                const int count = post ? B.post_count : B.pre_count;
-                if (count > 0) {
-                    const int first = post ? B.post_first : B.pre_first;
-                    const uint64_t colglobal = base + local_to_global<K>(col_local(col), P);
-                    for (int i = 0; i < count; ++i) {
-                        const BlockPhase q = ph[first + i];
-                        if ((colglobal & q.omask) == q.omask) c[q.slot + 1] += q.theta;
-                    }
-                }
-                double2* out = s_z + post * NCOL * NZ + col * NZ;
-#pragma unroll
-                for (int j = 0; j < NZ; ++j) {
-                    Z64 z = zexp(c[j]);
-                    if (j == 0 && !post) { z.r *= scale; z.i *= scale; }
-                    out[j] = make_double2(z.r, z.i);
-                }
-            }
-        }
         __syncthreads();
         {
             const uint64_t nt = tile + gridDim.x;
-            if (nt < ntiles) fetch(tile_base<K>(nt, P));
+            if (nt < ntiles) {
+                fetch(tile_base<K>(nt, P));
+                if (pre_any || post_any) tile_factors(nt, buf ^ 1);
+            }
         }
+        const double2* tf = s_tf + buf * 2 * NZ;
         for (int cg = wave; cg < NGROUP; cg += NT / 64) {
             const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
             const uint32_t ucg = swz<float>(col_local(static_cast<uint32_t>(cg * 16)));
@@ -912,11 +895,12 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
             if (pre_any) {
                 // pre-phase factors of the lane's 8 elements e = g + 4 m
                 float fr[8], fi[8];
-                const double2* z = s_z + col * NZ;
-                Z64 b0 = {z[0].x, z[0].y};
-                if (g & 1) b0 = zmul(b0, Z64{z[1].x, z[1].y});
-                if (g & 2) b0 = zmul(b0, Z64{z[2].x, z[2].y});
-                element_factors(b0, Z64{z[3].x, z[3].y}, Z64{z[4].x, z[4].y}, Z64{z[5].x, z[5].y}, fr, fi);
+                Z64 z[NZ];
+                column_factors(s_cf + col * NZ, tf, z);
+                Z64 b0 = z[0];
+                if (g & 1) b0 = zmul(b0, z[1]);
+                if (g & 2) b0 = zmul(b0, z[2]);
+                element_factors(b0, z[3], z[4], z[5], fr, fi);
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
                     const uint32_t j = rl_in ^ ucg ^ swz<float>(eloc(4 * m));
@@ -944,11 +928,12 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
             // lane holds rows 16 r + 4 g + v: re of e = 16 h + 4 g + v (r = h), im (r = 2 + h)
             if (post_any) {
                 float pr[8], pi2[8];
-                const double2* z = s_z + NCOL * NZ + col * NZ;
-                Z64 b0 = {z[0].x, z[0].y};
-                if (g & 1) b0 = zmul(b0, Z64{z[3].x, z[3].y});
-                if (g & 2) b0 = zmul(b0, Z64{z[4].x, z[4].y});
-                element_factors(b0, Z64{z[1].x, z[1].y}, Z64{z[2].x, z[2].y}, Z64{z[5].x, z[5].y}, pr, pi2);
+                Z64 z[NZ];
+                column_factors(s_cf + NCOL * NZ + col * NZ, tf + NZ, z);
+                Z64 b0 = z[0];
+                if (g & 1) b0 = zmul(b0, z[3]);
+                if (g & 2) b0 = zmul(b0, z[4]);
+                element_factors(b0, z[1], z[2], z[5], pr, pi2);
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -989,16 +974,16 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
 
 template <int K>
 hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
-                          const BlockPhase* ph, const double* tab, const double* norm2, hipStream_t st) {
-    // the tile (re, im), the per-column pre / post factors (6 complex FP64
-    // each) and the tile's 2 x 6 angles
-    const size_t lds = 2ull * (1ull << K) * sizeof(float) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 6 * 8;
+                          const double* tab, const double* norm2, hipStream_t st) {
+    // the tile (re, im), the pre / post column factors (6 complex FP64 per
+    // column) and two buffers of the tile's 2 x 6 factors
+    const size_t lds = 2ull * (1ull << K) * sizeof(float) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 2 * 6 * 16;
     static int grid_cap = 0;
     hipError_t e = persistent_grid(block_mfma_kernel<K>, 256, lds, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
-    hipLaunchKernelGGL(block_mfma_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, ph, tab, norm2);
+    hipLaunchKernelGGL(block_mfma_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, tab, norm2);
     return hipGetLastError();
 }
 
@@ -1603,15 +1588,15 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
 }
 
 hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
-                             const BlockPhase* ph, const double* tab, const double* norm2, hipStream_t stream) {
+                             const double* tab, const double* norm2, hipStream_t stream) {
     if (dtype != BFLY_C64) return hipErrorInvalidValue;
     float* r = static_cast<float*>(re);
     float* i = static_cast<float*>(im);
     switch (P.k) {
-        case 9: return launch_block_k<9>(r, i, P, B, mat, ph, tab, norm2, stream);
-        case 10: return launch_block_k<10>(r, i, P, B, mat, ph, tab, norm2, stream);
-        case 11: return launch_block_k<11>(r, i, P, B, mat, ph, tab, norm2, stream);
-        case 12: return launch_block_k<12>(r, i, P, B, mat, ph, tab, norm2, stream);
+        case 9: return launch_block_k<9>(r, i, P, B, mat, tab, norm2, stream);
+        case 10: return launch_block_k<10>(r, i, P, B, mat, tab, norm2, stream);
+        case 11: return launch_block_k<11>(r, i, P, B, mat, tab, norm2, stream);
+        case 12: return launch_block_k<12>(r, i, P, B, mat, tab, norm2, stream);
     }
     return hipErrorInvalidValue;
 }

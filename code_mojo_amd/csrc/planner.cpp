@@ -874,7 +874,7 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
                         if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
                     fold_into_block(U, c, slot_of[op.target], cm);
                 } else if ((c.kind == I_PHASE1 || c.kind == I_NEG1) && popc(qall & Q) <= 1 &&
-                           filler_room(qall & ~Q)) {
+                           popc(qall & ~Q) <= 1 && filler_room(qall & ~Q)) {
                     const uint64_t a = qall & Q;
                     BlockPhase ph{};
                     ph.omask = qall & ~Q;
@@ -951,8 +951,7 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
                 const double v = (r < 32) == (k2 < 32) ? u.real() : (r < 32 ? -u.imag() : u.imag());
                 plan.bmat.push_back(static_cast<float>(v));
             }
-        // records with at most one outside bit go to the angle tables (row of
-        // their bit), the others stay records
+        // phase records summed per table row (FP64), then the device tables
         {
             int row_of[64];
             int nrows = 1;
@@ -961,24 +960,35 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
             for (int i = 0; i < K; ++i)
                 if (!((qm >> i) & 1)) row_of[tp.qpos[i]] = nrows++;
             for (int i = 0; i < tp.nouter; ++i) row_of[tp.opos[i]] = nrows++;
+            const int ncb = K - b, ncol = 1 << ncb;
+            bp.rows_off = plan.brows.size();
+            plan.brows.resize(plan.brows.size() + 2 * static_cast<size_t>(nrows) * 6, 0.0);
             bp.tab_off = plan.btab.size();
-            plan.btab.resize(plan.btab.size() + 2 * static_cast<size_t>(nrows) * 6, 0.0);
+            plan.btab.resize(plan.btab.size() + 2 * static_cast<size_t>(ncol) * 6 * 2 + 2 * static_cast<size_t>(tp.nouter) * 6,
+                             0.0);
             for (int w = 0; w < 2; ++w) {
-                const std::vector<BlockPhase>& recs = w ? post : pre;
-                double* t = plan.btab.data() + bp.tab_off + static_cast<size_t>(w) * nrows * 6;
-                int first = static_cast<int32_t>(plan.bphase.size()), count = 0, tab = 0;
-                for (const BlockPhase& r : recs) {
-                    if (popc(r.omask) <= 1) {
-                        const int row = r.omask ? row_of[__builtin_ctzll(r.omask)] : 0;
-                        t[row * 6 + r.slot + 1] += r.theta;
-                        tab = 1;
-                    } else {
-                        plan.bphase.push_back(r);
-                        ++count;
-                    }
+                double* t = plan.brows.data() + bp.rows_off + static_cast<size_t>(w) * nrows * 6;
+                int tab = 0;
+                for (const BlockPhase& r : w ? post : pre) {
+                    const int row = r.omask ? row_of[__builtin_ctzll(r.omask)] : 0;
+                    t[row * 6 + r.slot + 1] += r.theta;
+                    tab = 1;
                 }
-                if (w) { bp.post_first = first; bp.post_count = count; bp.post_tab = tab; }
-                else { bp.pre_first = first; bp.pre_count = count; bp.pre_tab = tab; }
+                if (w) bp.post_tab = tab;
+                else bp.pre_tab = tab;
+                double* cf = plan.btab.data() + bp.tab_off + static_cast<size_t>(w) * ncol * 12;
+                for (int col = 0; col < ncol; ++col)
+                    for (int j = 0; j < 6; ++j) {
+                        double a = t[j];
+                        for (int i = 0; i < ncb; ++i)
+                            if ((col >> i) & 1) a += t[(1 + i) * 6 + j];
+                        cf[(col * 6 + j) * 2] = std::cos(a);
+                        cf[(col * 6 + j) * 2 + 1] = std::sin(a);
+                    }
+                double* oa = plan.btab.data() + bp.tab_off + 2 * static_cast<size_t>(ncol) * 12 +
+                             static_cast<size_t>(w) * tp.nouter * 6;
+                for (int i = 0; i < tp.nouter; ++i)
+                    for (int j = 0; j < 6; ++j) oa[i * 6 + j] = t[(1 + ncb + i) * 6 + j];
             }
         }
         plan.steps.push_back(st);
