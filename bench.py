@@ -252,7 +252,8 @@ def bench_precision(args) -> int:
         "roofline": {"bound": "hbm", "achieved": round(bytes_per / (per_launch * 1e-3) / 1e9, 2) if blk_n else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if blk_n else None,
-                     "traffic": None, "kernel": "block_mfma_kernel", "launches": blk_n,
+                     "traffic": read_traffic(f"block_mfma_kernel_{n}q_value_encoding_traffic.txt"),
+                     "kernel": "block_mfma_kernel", "launches": blk_n,
                      "mean_launch_ms": round(per_launch, 4),
                      "mfma_tflops": round(flops_per / (per_launch * 1e-3) / 1e12, 2) if blk_n else None,
                      "mfma_peak_tflops": 157.3},
@@ -260,6 +261,18 @@ def bench_precision(args) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary under profiles/ (None when there is none for this workload)."""
+    tfile = ROOT / "profiles" / name
+    if not tfile.exists():
+        return None
+    for line in tfile.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=", 1)[1])
+    return None
 
 
 def profile_families(lib) -> dict:
@@ -307,8 +320,9 @@ def main() -> int:
         a, b = GpuState(args.copy_probe), GpuState(args.copy_probe)
         b.copy_from(a)
         b.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": 2 * (1 << args.copy_probe) * 8 * 2,
-                          "copy_probe_bytes_written": 2 * (1 << args.copy_probe) * 8 * 2}), flush=True)
+        print(json.dumps({"copy_probe_bytes_read": 2 * (1 << args.copy_probe) * 8,
+                          "copy_probe_bytes_written": 2 * (1 << args.copy_probe) * 8,
+                          "dispatches": 2}), flush=True)
         return 0
 
     if args.gpus != 1:
@@ -375,12 +389,7 @@ def main() -> int:
     per_launch_ms = dom_ms / launches
     achieved = state_bytes / (per_launch_ms / 1e3) / 1e9
 
-    traffic = None
-    tfile = ROOT / "profiles" / f"{dom}_{n}q_layered_{args.dtype}_traffic.txt"
-    if tfile.exists():
-        for line in tfile.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch"):
-                traffic = float(line.split("=", 1)[1])
+    traffic = read_traffic(f"{dom}_{n}q_layered_{args.dtype}_traffic.txt")
 
     out = {
         "metric": "state GB/s vs HBM peak at 30q",
