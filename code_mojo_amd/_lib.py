@@ -110,6 +110,10 @@ _SIGS = {
     "bfly_inner": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "bfly_diff_norm2": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double)]),
     "bfly_scale": (ct.c_int, [ct.c_void_p, ct.c_double]),
+    "bfly_measure": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), ct.c_int32, ct.c_int64,
+                                ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)]),
+    "bfly_sample": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.c_int32, ct.c_int64, ct.c_uint64,
+                               ct.POINTER(ct.c_uint64)]),
     "bfly_profile_reset": (ct.c_int, []),
     "bfly_profile_get": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_int64),
                                     ct.POINTER(ct.c_double)]),

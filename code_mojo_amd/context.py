@@ -72,6 +72,7 @@ class ExecContext:
     gpu_fuse_k: int = 0          # block width for gpu_mode=block (0 = 5)
     gpu_max_gates_per_pass: int = 0
     gpu_renorm_every: int = 0
+    measure_seed: int = -1       # apply_measure's configured seed (state.mojo:509); -1 = clock
 
     def copy(self) -> "ExecContext":
         return replace(self)
@@ -81,7 +82,8 @@ class ExecContext:
         """Build from key=value pairs (utils/context.mojo:118-292 pattern)."""
         ctx = ExecContext()
         ints = ["threads", "quantum_simd_parallel_chunks", "grid_col_bits_min", "grid_col_bits_slack",
-                "gpu_tile_bits", "gpu_low_bits", "gpu_fuse_k", "gpu_max_gates_per_pass", "gpu_renorm_every"]
+                "gpu_tile_bits", "gpu_low_bits", "gpu_fuse_k", "gpu_max_gates_per_pass", "gpu_renorm_every",
+                "measure_seed"]
         bools = ["parallel_flat", "simd_use_specialized_h", "simd_use_specialized_p", "simd_use_specialized_cp",
                  "simd_use_specialized_x", "simd_use_specialized_cx", "simd_use_specialized_ry",
                  "simd_use_specialized_cry", "grid_use_parallel", "validate_circuit"]

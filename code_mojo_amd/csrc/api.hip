@@ -9,6 +9,7 @@
 
 #include <cstdio>
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -473,6 +474,15 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                 break;
             }
             case S_NORM: put("norm\n"); break;
+            case S_MEASURE: {
+                put("measure seeded=%d seed=%lld targets=", st.b, static_cast<long long>(st.controls));
+                for (int k = 0; k < st.a; ++k) {
+                    const double* ent = hp.udata.data() + 2 * (st.uoff + static_cast<uint64_t>(k));
+                    put(k ? ",%d:%d" : "%d:%d", static_cast<int>(ent[0]), static_cast<int>(ent[1]));
+                }
+                put("\n");
+                break;
+            }
         }
     }
     if (needed) *needed = out.size() + 1;
@@ -534,6 +544,19 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
                 prof.begin(F_REDUCE);
                 e = launch_step(st, s, p, s->stream);
                 break;
+            case S_MEASURE: {
+                // probabilities on the device, the choice on the host, collapse
+                std::vector<int32_t> tq(static_cast<size_t>(st.a)), tv(static_cast<size_t>(st.a));
+                for (int k = 0; k < st.a; ++k) {
+                    const double* ent = p->hp.udata.data() + 2 * (st.uoff + static_cast<uint64_t>(k));
+                    tq[k] = static_cast<int32_t>(ent[0]);
+                    tv[k] = static_cast<int32_t>(ent[1]);
+                }
+                rc = measure_state(s, tq.data(), tv.data(), st.a, st.b ? static_cast<int64_t>(st.controls) : -1,
+                                   nullptr, nullptr);
+                if (rc) return rc;
+                continue;
+            }
         }
         prof.end();
         if (e != hipSuccess) {
@@ -624,6 +647,7 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
         case S_NORM:
             return launch_reduce(s->dtype, R_NORM2, s->re, s->im, nullptr, nullptr, 1ull << s->n, 0, 0, s->scratch,
                                  stream);
+        case S_MEASURE: break;  // host round trip: bfly_plan_execute only
     }
     return hipErrorInvalidValue;
 }
@@ -672,6 +696,8 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
         if (!plans[i]) return fail(BFLY_EINVAL, "null plan");
         if (plans[i]->hp.n != states[i]->n || plans[i]->hp.dtype != states[i]->dtype)
             return fail(BFLY_EINVAL, "plan / state mismatch at item " + std::to_string(i));
+        if (plans[i]->hp.measures)
+            return fail(BFLY_EUNSUPPORTED, "measurement runs through bfly_plan_execute, not in batches");
         if (states[i]->device != states[0]->device) return fail(BFLY_EINVAL, "states on different devices");
         for (int j = 0; j < i; ++j)
             if (states[j] == states[i]) return fail(BFLY_EINVAL, "a state appears twice in one batch");
@@ -918,6 +944,199 @@ int bfly_state_fill_random(bfly_state* s, uint64_t seed) {
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
     return fill_random_on(s, seed, s->stream);
 }
+
+}  // extern "C"
+
+namespace {
+
+constexpr int64_t kMinstdMod = 2147483647;
+constexpr int64_t kMinstdMul = 48271;
+
+// Outcome probabilities of a measurement, laid out as the reference's
+// outcome_probs (state.mojo:463-500): outcome bit k = k-th free target.  The
+// device sums over the distinct free qubits that are not also forced; the
+// host spreads them over the listed free targets (repeats and forced
+// overlaps give zero-probability outcomes, as the reference's walk does).
+struct MeasureSetup {
+    std::vector<int32_t> free_q;
+    uint64_t forced_mask = 0, forced_val = 0, tmask = 0;
+    std::vector<double> P;
+};
+
+int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, int m, MeasureSetup& ms) {
+    bool conflict = false;
+    for (int k = 0; k < m; ++k) {
+        const int t = targets[k];
+        if (t < 0 || t >= s->n) return fail(BFLY_EINVAL, "Target out of bounds: " + std::to_string(t));
+        const int v = values ? values[k] : -1;
+        if (v < -1 || v > 1) return fail(BFLY_EINVAL, "measurement value must be -1 (free), 0 or 1");
+        ms.tmask |= 1ull << t;
+        if (v < 0) {
+            ms.free_q.push_back(t);
+        } else {
+            if (((ms.forced_mask >> t) & 1) && static_cast<int>((ms.forced_val >> t) & 1) != v) conflict = true;
+            ms.forced_mask |= 1ull << t;
+            ms.forced_val |= static_cast<uint64_t>(v) << t;
+        }
+    }
+    const int f = static_cast<int>(ms.free_q.size());
+    if (f > 20) return fail(BFLY_EUNSUPPORTED, "more than 20 free measured targets");
+    std::vector<int32_t> uq;
+    uint64_t umask = 0;
+    for (int32_t q : ms.free_q)
+        if (!((ms.forced_mask >> q) & 1) && !((umask >> q) & 1)) { uq.push_back(q); umask |= 1ull << q; }
+    if (static_cast<int>(uq.size()) > kOutcomeMaxBits)
+        return fail(BFLY_EUNSUPPORTED, "more than 12 distinct free measured qubits");
+    ms.P.assign(size_t(1) << f, 0.0);
+    if (conflict) return BFLY_OK;  // no amplitude matches the forced values
+    OutcomeSpec sp{};
+    sp.f = static_cast<int32_t>(uq.size());
+    for (size_t j = 0; j < uq.size(); ++j) sp.fpos[j] = uq[j];
+    sp.fval = ms.forced_val;
+    const uint64_t all = (1ull << s->n) - 1;
+    sp.rmask = all & ~(ms.forced_mask | umask);
+    const uint64_t rest_count = 1ull << __builtin_popcountll(sp.rmask);
+    const int nb = outcome_blocks(sp, rest_count);
+    sp.rest_per_block = rest_count / static_cast<uint64_t>(nb);
+    const size_t nout = size_t(1) << sp.f;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    double* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, (static_cast<size_t>(nb) * nout + nout) * sizeof(double)), "hipMalloc(measurement)");
+    std::vector<double> Pu(nout);
+    hipError_t e = launch_outcome_probs(s->dtype, s->re, s->im, sp, nb, buf, buf + static_cast<size_t>(nb) * nout,
+                                        s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(Pu.data(), buf + static_cast<size_t>(nb) * nout, nout * sizeof(double),
+                           hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
+    for (uint64_t o = 0; o < (1ull << f); ++o) {
+        uint64_t assigned = 0, bits = 0;
+        bool ok = true;
+        for (int k = 0; k < f && ok; ++k) {
+            const int q = ms.free_q[k];
+            const uint64_t b = (o >> k) & 1;
+            if ((ms.forced_mask >> q) & 1) ok = ((ms.forced_val >> q) & 1) == b;
+            else if ((assigned >> q) & 1) ok = ((bits >> q) & 1) == b;
+            assigned |= 1ull << q;
+            bits |= b << q;
+        }
+        if (!ok) continue;
+        uint64_t u = 0;
+        for (size_t j = 0; j < uq.size(); ++j) u |= ((bits >> uq[j]) & 1) << j;
+        ms.P[o] = Pu[u];
+    }
+    return BFLY_OK;
+}
+
+int64_t clock_seed() {
+    const int64_t t = static_cast<int64_t>(std::chrono::steady_clock::now().time_since_epoch().count());
+    return t < 0 ? -t : t;
+}
+
+// The reference's choice (state.mojo:505-533) in the state's precision T.
+template <typename T>
+int measure_choose(const std::vector<double>& P, int64_t seed, uint64_t* chosen_out, T* chosen_prob, T* inv) {
+    T total = T(0);
+    for (double p : P) total += static_cast<T>(p);
+    if (!(total > T(0))) return fail(BFLY_EINVAL, "Measurement probability is zero");
+    int64_t rng = seed % kMinstdMod;
+    rng = (rng * kMinstdMul) % kMinstdMod;
+    const T r = static_cast<T>(rng) / static_cast<T>(kMinstdMod) * total;
+    uint64_t chosen = 0;
+    T acc = T(0);
+    for (size_t o = 0; o < P.size(); ++o) {
+        acc += static_cast<T>(P[o]);
+        if (r <= acc) { chosen = o; break; }
+    }
+    const T cp = static_cast<T>(P[chosen]);
+    if (!(cp > T(0))) return fail(BFLY_EINVAL, "Measurement outcome probability is zero");
+    *chosen_out = chosen;
+    *chosen_prob = cp;
+    *inv = T(1) / static_cast<T>(std::sqrt(static_cast<double>(cp)));
+    return BFLY_OK;
+}
+
+}  // namespace
+
+namespace bfly {
+
+int measure_state(bfly_state* s, const int32_t* targets, const int32_t* values, int32_t m, int64_t seed,
+                  uint64_t* outcome, double* prob) {
+    if (m == 0) return BFLY_OK;  // the reference returns before any check
+    if (m < 0 || !targets) return fail(BFLY_EINVAL, "null argument");
+    MeasureSetup ms;
+    int rc = measure_setup(s, targets, values, m, ms);
+    if (rc) return rc;
+    if (seed < 0) seed = clock_seed();
+    uint64_t chosen = 0;
+    double cp = 0.0, inv = 0.0;
+    if (s->dtype == BFLY_C128) {
+        rc = measure_choose<double>(ms.P, seed, &chosen, &cp, &inv);
+    } else {
+        float cpf = 0.f, invf = 0.f;
+        rc = measure_choose<float>(ms.P, seed, &chosen, &cpf, &invf);
+        cp = cpf;
+        inv = invf;
+    }
+    if (rc) return rc;
+    uint64_t tval = ms.forced_val;
+    for (size_t k = 0; k < ms.free_q.size(); ++k)
+        if ((chosen >> k) & 1) tval |= 1ull << ms.free_q[k];
+    HIP_TRY(launch_collapse(s->dtype, s->re, s->im, 1ull << s->n, ms.tmask, tval, inv, s->stream), "collapse launch");
+    HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+    if (outcome) *outcome = chosen;
+    if (prob) *prob = cp;
+    return BFLY_OK;
+}
+
+}  // namespace bfly
+
+extern "C" {
+
+int bfly_measure(bfly_state* s, const int32_t* targets, const int32_t* values, int32_t m, int64_t seed,
+                 uint64_t* outcome, double* prob) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    return measure_state(s, targets, values, m, seed, outcome, prob);
+}
+
+int bfly_sample(bfly_state* s, const int32_t* targets, int32_t m, int64_t seed, uint64_t shots, uint64_t* samples) {
+    int rc = check_state(s);
+    if (rc) return rc;
+    if (m < 1 || !targets || (shots && !samples)) return fail(BFLY_EINVAL, "null argument");
+    if (seed < 0 || seed % kMinstdMod == 0) return fail(BFLY_EINVAL, "sample seed must be >= 0 and not a multiple of 2147483647");
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    MeasureSetup ms;
+    if ((rc = measure_setup(s, targets, nullptr, m, ms))) return rc;
+    // running sums in the state's precision, searched per shot
+    std::vector<double> cum(ms.P.size());
+    double total = 0.0;
+    if (s->dtype == BFLY_C128) {
+        for (size_t o = 0; o < ms.P.size(); ++o) cum[o] = total = total + ms.P[o];
+    } else {
+        float t = 0.f;
+        for (size_t o = 0; o < ms.P.size(); ++o) cum[o] = t = t + static_cast<float>(ms.P[o]);
+        total = t;
+    }
+    if (!(total > 0.0)) return fail(BFLY_EINVAL, "Measurement probability is zero");
+    int64_t rng = seed % kMinstdMod;
+    for (uint64_t j = 0; j < shots; ++j) {
+        rng = (rng * kMinstdMul) % kMinstdMod;
+        double r;
+        if (s->dtype == BFLY_C128) r = static_cast<double>(rng) / static_cast<double>(kMinstdMod) * total;
+        else r = static_cast<float>(rng) / static_cast<float>(kMinstdMod) * static_cast<float>(total);
+        const size_t o = static_cast<size_t>(std::lower_bound(cum.begin(), cum.end(), r) - cum.begin());
+        samples[j] = o < cum.size() ? o : 0;
+    }
+    return BFLY_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int bfly_scale(bfly_state* s, double scale) {
     int rc = check_state(s);

@@ -121,6 +121,7 @@ enum StepKind : int32_t {
     S_TILE = 0, S_GATE = 1, S_BITREV = 2, S_SWAP = 3, S_UNITARY = 4, S_FLIP = 5, S_DIFFUSE = 6,
     S_BLOCK = 7,   // dense 2^5-amplitude block on FP32 MFMA (complex64 BLOCK mode)
     S_NORM = 8,    // compensated squared norm into the state's scratch (BLOCK mode check)
+    S_MEASURE = 9, // measurement: a = targets, uoff = their (qubit, value) list, b = 1 when `controls` is the seed
 };
 
 // BLOCK mode: a block is a 32 x 32 complex unitary U on 5 qubits (the
@@ -175,6 +176,7 @@ struct HostPlan {
     std::vector<double> udata;   // complex, interleaved
     std::vector<float> bmat;     // BLOCK mode: 64 x 64 real forms of the blocks' U, row-major
     std::vector<double> btab;          // BLOCK mode device tables (column factors, outer angles)
+    int64_t measures = 0;              // S_MEASURE steps (they synchronise with the host)
     std::vector<double> brows;         // BLOCK mode row angles [2][n - 4][6] per block (host only)
     int64_t gates = 0;
     int64_t passes = 0;          // HBM sweeps over the state

@@ -498,7 +498,14 @@ void planner_loop(bfly_engine* e) {
         if (!r->cancel) {
             rc = bfly_plan_create(&plan, r->n, r->dtype, r->ops.data(), r->ops.size(),
                                   r->udata.empty() ? nullptr : r->udata.data(), r->udata.size() / 2, &r->exec);
-            if (rc != BFLY_OK) msg = bfly_last_error();
+            if (rc != BFLY_OK) {
+                msg = bfly_last_error();
+            } else if (plan->hp.measures) {
+                rc = BFLY_EUNSUPPORTED;
+                msg = "measurement runs through bfly_plan_execute, not in the engine";
+                bfly_plan_destroy(plan);
+                plan = nullptr;
+            }
         }
         std::vector<bfly_op>().swap(r->ops);
         std::vector<double>().swap(r->udata);

@@ -45,6 +45,9 @@ int upload_plan(bfly_plan* p, int device);
 size_t plan_device_bytes(const bfly_plan* p);
 // Seeded N(0,1) state normalised to 1, enqueued on `stream`.
 int fill_random_on(bfly_state* s, uint64_t seed, hipStream_t stream);
+// apply_measure on a device state (synchronous; m = 0 does nothing).
+int measure_state(bfly_state* s, const int32_t* targets, const int32_t* values, int32_t m, int64_t seed,
+                  uint64_t* outcome, double* prob);
 int upload_plan_into(bfly_plan* p, int device, void* buf, hipStream_t stream);
 // Enqueue one plan step on `stream`.
 hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t stream);

@@ -44,6 +44,25 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
                           const double* d_u, hipStream_t stream);
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
+// Measurement (apply_measure, state.mojo:447-553): outcome probabilities of
+// up to kOutcomeMaxBits free qubits among the amplitudes with the forced
+// values, FP64 compensated and deterministic; `part` holds nblocks * 2^f
+// doubles (outcome_blocks), `out` 2^f.
+constexpr int kOutcomeMaxBits = 12;
+struct OutcomeSpec {
+    int32_t f;                 // free qubits (outcome bit k = qubit fpos[k])
+    int32_t pad;
+    int32_t fpos[kOutcomeMaxBits];
+    uint64_t fval;             // values of the forced qubits (their bits set where forced to 1)
+    uint64_t rmask;            // every other qubit
+    uint64_t rest_per_block;   // rest values per block
+};
+int outcome_blocks(const OutcomeSpec& sp, uint64_t rest_count);
+hipError_t launch_outcome_probs(int dtype, const void* re, const void* im, const OutcomeSpec& sp, int nblocks,
+                                double* part, double* out, hipStream_t stream);
+// Amplitudes with (i & tmask) == tval times inv (in the state's precision), the others 0.
+hipError_t launch_collapse(int dtype, void* re, void* im, uint64_t size, uint64_t tmask, uint64_t tval, double inv,
+                           hipStream_t stream);
 // Element-wise precision conversion from a src_dtype state to the other dtype.
 hipError_t launch_convert(int src_dtype, const void* sre, const void* sim, void* dre, void* dim, uint64_t size,
                           hipStream_t stream);

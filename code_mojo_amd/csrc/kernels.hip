@@ -12,7 +12,7 @@
 //                         transformations_scalar.mojo:586-594)
 //   (b) tile_pass_kernel  a 2^K-amplitude tile staged in LDS with coalesced
 //                         16-byte loads; the pass's gates run as register
-//                         sweeps over 8-amplitude groups; one HBM read + write
+//                         sweeps over 8- or 16-amplitude groups; one HBM read + write
 //                         per pass (generalises execute_grid_fused's row-local
 //                         groups, executors.mojo:1062-1093)
 //   (c) bitrev kernels    in place: 32x32 LDS tiles pair index (h,m,l) with
@@ -21,7 +21,14 @@
 //   (d) reduce kernels    FP64 TwoSum-compensated sums with a fixed reduction
 //                         tree (deterministic), for norms / probabilities /
 //                         inner products (grover_classical.mojo:8-31,109-131)
+//   block_mfma_kernel     complex64 dense 5-qubit blocks on FP32 MFMA with FP64
+//                         phase factors (BLOCK mode)
+//   flip / diffuse        phase flips and block-mean reflections
+//                         (grover_classical.mojo:262-326)
+//   unitary_kernel        dense 2^m blocks (transformations_unitary.mojo:22-139)
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.h"
 
@@ -330,9 +337,10 @@ __device__ __forceinline__ void scale_all(T* ar, T* ai, T zr, T zi) {
 }
 
 // Sweeps of one tile pass on the tile staged in LDS (s_re, s_im): each of
-// the 2^(K-4) groups of 16 amplitudes spanned by the sweep's 4 slot bits is
-// loaded into registers, every gate of the sweep runs in registers, and the
-// group is written back.  Ends with a barrier after every sweep.
+// the 2^(K-S) groups of 2^S amplitudes spanned by the sweep's S slot bits
+// (S = 3 or 4) is loaded into registers, every gate of the sweep runs in
+// registers, and the group is written back.  Ends with a barrier after every
+// sweep.
 // Calls CALL with the compile-time slot index SL equal to `slot` (< S).
 #define BFLY_SLOT_SWITCH(S, slot, CALL)                                    \
     switch (slot) {                                                        \
@@ -470,7 +478,7 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t t, const TilePass& P) {
 // (b) One tile pass.  A tile is 2^K amplitudes: local bit i is global bit
 // P.qpos[i] (the low local bits are the low qubits, so VN consecutive local
 // indices are VN consecutive amplitudes), the other bits come from the tile
-// index.  NT = 2^(K-4) threads, one 16-amplitude group each per sweep.
+// index.  NT = 2^(K-S) threads, one 2^S-amplitude group each per sweep.
 // Blocks are persistent: block b handles tiles b, b + gridDim.x, ...; the
 // next tile's HBM loads are issued into registers before the current tile's
 // sweeps and land in LDS only after its stage-out, so HBM traffic overlaps
@@ -1563,6 +1571,106 @@ hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- measurement
+// Outcome probabilities P(o) = sum of |a_i|^2 over i = dep(o, free bits) |
+// rest | fval, the rest enumerating the bits of `rmask` (neither free nor
+// forced).  Block b takes the rest values [b * rpb, (b + 1) * rpb).  With
+// 2^f >= NT a thread owns outcomes tid + NT k (k < NO); with 2^f < NT the
+// NT / 2^f threads of one outcome interleave over the rest values and are
+// merged through LDS in thread order.  Per-thread TwoSum, fixed assignment and
+// order: deterministic.  Rest indices advance by masked addition:
+// dep(r + d) = ((dep(r) | ~rmask) + dep(d)) & rmask.
+constexpr int kOutThreads = 256;
+
+__host__ __device__ inline uint64_t deposit64(uint64_t x, uint64_t mask) {
+    uint64_t out = 0;
+    while (mask) {
+        const uint64_t low = mask & (~mask + 1);
+        if (x & 1) out |= low;
+        x >>= 1;
+        mask ^= low;
+    }
+    return out;
+}
+
+template <typename T, int NO>
+__global__ __launch_bounds__(kOutThreads) void outcome_partials_kernel(const T* __restrict__ re,
+                                                                       const T* __restrict__ im, OutcomeSpec sp,
+                                                                       double* __restrict__ part) {
+    __shared__ double s_part[kOutThreads];
+    const uint32_t nout = 1u << sp.f;
+    const uint32_t tid = threadIdx.x;
+    const bool wide = nout >= static_cast<uint32_t>(kOutThreads);
+    const uint32_t o0 = wide ? tid : tid % nout;
+    const uint64_t rsub = wide ? 0 : tid / nout;
+    const uint64_t rstep = wide ? 1 : kOutThreads / nout;
+    uint64_t dep_o[NO];
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+        const uint32_t o = o0 + static_cast<uint32_t>(k * kOutThreads);
+        uint64_t d = 0;
+        for (int q = 0; q < sp.f; ++q) d |= static_cast<uint64_t>((o >> q) & 1u) << sp.fpos[q];
+        dep_o[k] = d;
+    }
+    double s[NO], c[NO];
+#pragma unroll
+    for (int k = 0; k < NO; ++k) { s[k] = 0.0; c[k] = 0.0; }
+    const uint64_t r_begin = static_cast<uint64_t>(blockIdx.x) * sp.rest_per_block + rsub;
+    const uint64_t r_end = static_cast<uint64_t>(blockIdx.x + 1) * sp.rest_per_block;
+    uint64_t x = deposit64(r_begin, sp.rmask);
+    const uint64_t dstep = deposit64(rstep, sp.rmask);
+    for (uint64_t r = r_begin; r < r_end; r += rstep) {
+        const uint64_t base = x | sp.fval;
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+            const uint64_t i = base | dep_o[k];
+            const double a = static_cast<double>(re[i]), b = static_cast<double>(im[i]);
+            two_sum_add(s[k], c[k], a * a + b * b);
+        }
+        x = ((x | ~sp.rmask) + dstep) & sp.rmask;
+    }
+    if (wide) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k)
+            part[static_cast<uint64_t>(blockIdx.x) * nout + o0 + static_cast<uint32_t>(k * kOutThreads)] = s[k] + c[k];
+        return;
+    }
+    s_part[tid] = s[0] + c[0];
+    __syncthreads();
+    if (tid < nout) {
+        double ss = 0.0, cc = 0.0;
+        for (uint32_t j = 0; j < rstep; ++j) two_sum_add(ss, cc, s_part[tid + j * nout]);
+        part[static_cast<uint64_t>(blockIdx.x) * nout + tid] = ss + cc;
+    }
+}
+
+// out[o] = sum over blocks of part[b][o] (TwoSum, block order).
+__global__ void outcome_merge_kernel(const double* __restrict__ part, int nblocks, uint32_t nout,
+                                     double* __restrict__ out) {
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout) return;
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nblocks; ++b) two_sum_add(s, c, part[static_cast<uint64_t>(b) * nout + o]);
+    out[o] = s + c;
+}
+
+// Collapse: amplitudes with (i & tmask) == tval times inv, the others 0
+// (apply_measure's last loop, state.mojo:535-553).
+template <typename T>
+__global__ void collapse_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size, uint64_t tmask, uint64_t tval,
+                                T inv) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+        if ((i & tmask) == tval) {
+            re[i] = re[i] * inv;
+            im[i] = im[i] * inv;
+        } else {
+            re[i] = T(0);
+            im[i] = T(0);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- wrappers
 hipError_t launch_tile_pass(int dtype, void* re, void* im, const TilePass& P, const TileSweep* d_sweeps,
                             const TileInstr* d_instrs, hipStream_t stream) {
@@ -1696,6 +1804,53 @@ hipError_t launch_normalize(int dtype, void* re, void* im, uint64_t size, const 
     else
         hipLaunchKernelGGL(normalize_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream,
                            static_cast<double*>(re), static_cast<double*>(im), size, d_norm2);
+    return hipGetLastError();
+}
+
+int outcome_blocks(const OutcomeSpec& sp, uint64_t rest_count) {
+    // at most 2^20 partial doubles; rest values split evenly (powers of two)
+    uint64_t nb = 1024;
+    while (nb > 1 && (nb << sp.f) > (1ull << 20)) nb >>= 1;
+    while (nb > 1 && nb > rest_count) nb >>= 1;
+    return static_cast<int>(nb);
+}
+
+template <typename T>
+hipError_t launch_outcome_t(const T* re, const T* im, const OutcomeSpec& sp, int nblocks, double* part, double* out,
+                            hipStream_t st) {
+    const uint32_t nout = 1u << sp.f;
+    const int no = nout > static_cast<uint32_t>(kOutThreads) ? static_cast<int>(nout / kOutThreads) : 1;
+    switch (no) {
+        case 1: hipLaunchKernelGGL((outcome_partials_kernel<T, 1>), dim3(nblocks), dim3(kOutThreads), 0, st, re, im, sp, part); break;
+        case 2: hipLaunchKernelGGL((outcome_partials_kernel<T, 2>), dim3(nblocks), dim3(kOutThreads), 0, st, re, im, sp, part); break;
+        case 4: hipLaunchKernelGGL((outcome_partials_kernel<T, 4>), dim3(nblocks), dim3(kOutThreads), 0, st, re, im, sp, part); break;
+        case 8: hipLaunchKernelGGL((outcome_partials_kernel<T, 8>), dim3(nblocks), dim3(kOutThreads), 0, st, re, im, sp, part); break;
+        case 16: hipLaunchKernelGGL((outcome_partials_kernel<T, 16>), dim3(nblocks), dim3(kOutThreads), 0, st, re, im, sp, part); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(outcome_merge_kernel, dim3((nout + 255) / 256), dim3(256), 0, st, part, nblocks, nout, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_outcome_probs(int dtype, const void* re, const void* im, const OutcomeSpec& sp, int nblocks,
+                                double* part, double* out, hipStream_t st) {
+    if (sp.f < 0 || sp.f > kOutcomeMaxBits) return hipErrorInvalidValue;
+    if (dtype == BFLY_C128)
+        return launch_outcome_t(static_cast<const double*>(re), static_cast<const double*>(im), sp, nblocks, part, out, st);
+    return launch_outcome_t(static_cast<const float*>(re), static_cast<const float*>(im), sp, nblocks, part, out, st);
+}
+
+hipError_t launch_collapse(int dtype, void* re, void* im, uint64_t size, uint64_t tmask, uint64_t tval, double inv,
+                           hipStream_t st) {
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>((size + 255) / 256, 4096));
+    if (dtype == BFLY_C128)
+        hipLaunchKernelGGL(collapse_kernel<double>, dim3(grid), dim3(256), 0, st, static_cast<double*>(re),
+                           static_cast<double*>(im), size, tmask, tval, inv);
+    else
+        hipLaunchKernelGGL(collapse_kernel<float>, dim3(grid), dim3(256), 0, st, static_cast<float*>(re),
+                           static_cast<float*>(im), size, tmask, tval, static_cast<float>(inv));
     return hipGetLastError();
 }
 

@@ -169,6 +169,15 @@ int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_le
                 if (op.aux == 0 || (op.aux & ~all)) { err = "Target out of bounds"; return BFLY_EINVAL; }
                 if (op.controls & ~op.aux) { err = "phase-flip pattern outside its mask"; return BFLY_EINVAL; }
                 break;
+            case BFLY_OP_MEASURE: {
+                if (op.arg_i < 0 || op.aux + static_cast<uint64_t>(op.arg_i) > udata_len) {
+                    err = "Measurement values size mismatch";
+                    return BFLY_EINVAL;
+                }
+                if (op.gate != 0 && op.gate != 1) { err = "bad measurement seed flag"; return BFLY_EINVAL; }
+                if (op.gate == 1 && static_cast<int64_t>(op.controls) < 0) { err = "negative measurement seed"; return BFLY_EINVAL; }
+                break;
+            }
             case BFLY_OP_DIFFUSE:
                 if (op.aux == 0 || (op.aux & ~all)) { err = "Target out of bounds"; return BFLY_EINVAL; }
                 if (op.arg_i != 0 && op.arg_i != 1) { err = "unknown diffuser variant"; return BFLY_EINVAL; }
@@ -754,6 +763,11 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
                 st.kind = S_FLIP; st.mask = op.aux; st.controls = op.controls; break;
             case BFLY_OP_DIFFUSE:
                 st.kind = S_DIFFUSE; st.mask = op.aux; st.a = op.arg_i; break;
+            case BFLY_OP_MEASURE:
+                if (op.arg_i == 0) return;
+                st.kind = S_MEASURE; st.a = op.arg_i; st.uoff = op.aux; st.b = op.gate; st.controls = op.controls;
+                plan.measures += 1;
+                break;
             default: return;
         }
         plan.steps.push_back(st);
@@ -1072,6 +1086,11 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
                 st.kind = S_FLIP; st.mask = op.aux; st.controls = op.controls; break;
             case BFLY_OP_DIFFUSE:
                 st.kind = S_DIFFUSE; st.mask = op.aux; st.a = op.arg_i; break;
+            case BFLY_OP_MEASURE:
+                if (op.arg_i == 0) return;
+                st.kind = S_MEASURE; st.a = op.arg_i; st.uoff = op.aux; st.b = op.gate; st.controls = op.controls;
+                plan.measures += 1;
+                break;
             default: return;
         }
         plan.steps.push_back(st);

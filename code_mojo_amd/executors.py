@@ -32,8 +32,12 @@ def exec_opts_from_ctx(ctx: ExecContext, flags: int = 0) -> _lib.ExecOpts:
 
 
 def compile_plan(circuit: Circuit, dtype, ctx: ExecContext, flags: int = 0) -> Plan:
-    ops, udata = lower_circuit(circuit)
+    ops, udata = lower_circuit(circuit, measure_seed=_measure_seed(ctx))
     return Plan(circuit.num_qubits, dtype, ops, udata, exec_opts_from_ctx(ctx, flags))
+
+
+def _measure_seed(ctx: ExecContext):
+    return ctx.measure_seed if ctx.measure_seed >= 0 else None
 
 
 def execute_gpu(state: Union[QuantumState, GpuState], circuit: Circuit, ctx: ExecContext) -> None:
@@ -48,7 +52,7 @@ def execute_gpu(state: Union[QuantumState, GpuState], circuit: Circuit, ctx: Exe
         return
     if circuit.num_qubits != state.n:
         raise ValueError(f"Circuit qubits mismatch: circuit has {circuit.num_qubits}, state has {state.n}")
-    ops, udata = lower_circuit(circuit)
+    ops, udata = lower_circuit(circuit, measure_seed=_measure_seed(ctx))
     lib = _lib.load()
     opts = exec_opts_from_ctx(ctx)
     from .state import dtype_code

@@ -20,13 +20,15 @@ execute_with_handlers (butterfly/core/executors.mojo:268-319):
                                         (apply_grover_inversion_0, algos/grover_classical.mojo:262-281)
   ClassicalTransform "GROVER_DIFFUSER" -> DIFFUSE (mask = targets, reference variant)
                                         (apply_grover_diffuser, algos/grover_classical.mojo:284-326)
-  Measurement / other classical      -> error: not on the GPU path
+  MeasurementTransformation          -> MEASURE ((qubit, value) pairs in udata, seed; apply_measure,
+                                        state.mojo:447-553)
+  other classical                    -> error: not on the GPU path
 
 Validation follows validate_circuit (butterfly/core/executors.mojo:90-185).
 """
 from __future__ import annotations
 
-from typing import List, Tuple
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -39,7 +41,7 @@ OP_DTYPE = np.dtype([
 ], align=True)
 assert OP_DTYPE.itemsize == 104
 
-OP_GATE, OP_BITREV, OP_SWAP, OP_UNITARY, OP_PHASE_FLIP, OP_DIFFUSE = 0, 1, 2, 3, 4, 5
+OP_GATE, OP_BITREV, OP_SWAP, OP_UNITARY, OP_PHASE_FLIP, OP_DIFFUSE, OP_MEASURE = 0, 1, 2, 3, 4, 5, 6
 
 
 class LoweringError(ValueError):
@@ -81,6 +83,18 @@ class OpList:
 
     def diffuse(self, mask: int, variant: int):
         self.rows.append((OP_DIFFUSE, 0, 0, int(variant), 0, int(mask), 0.0, [0.0] * 8))
+
+    def measure(self, targets: List[int], values: List[Optional[bool]], seed: Optional[int]):
+        """MEASURE record: (qubit, value) pairs in udata, value -1 = free."""
+        inter = np.empty(2 * len(targets), dtype=np.float64)
+        for k, (t, v) in enumerate(zip(targets, values)):
+            inter[2 * k] = int(t)
+            inter[2 * k + 1] = -1 if v is None else (1 if v else 0)
+        has_seed = seed is not None and seed >= 0
+        self.rows.append((OP_MEASURE, 1 if has_seed else 0, 0, len(targets), int(seed) if has_seed else 0,
+                          self.ulen, 0.0, [0.0] * 8))
+        self.udata.append(inter)
+        self.ulen += len(targets)
 
     def unitary(self, u: np.ndarray, target: int, m: int, controls: List[int]):
         mask = 0
@@ -172,7 +186,26 @@ def _target_mask(targets: List[int], n: int) -> int:
     return mask
 
 
-def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None) -> Tuple[np.ndarray, np.ndarray]:
+def _lower_measure(out: OpList, tr, n: int, measure_seed: Optional[int]) -> None:
+    """apply_measure's checks (state.mojo:447-462): values either empty (all
+    free) or one per target; targets within the register.  A transformation
+    without a seed takes `measure_seed` (the reference's configured
+    measure_seed), else the clock at execution time."""
+    targets = [int(t) for t in tr.targets]
+    if not targets:
+        return
+    values = list(tr.values) if tr.values else [None] * len(targets)
+    if len(values) != len(targets):
+        raise LoweringError("Measurement values size mismatch")
+    for t in targets:
+        if t < 0 or t >= n:
+            raise LoweringError(f"Target out of bounds: {t}")
+    seed = tr.seed if tr.seed is not None else measure_seed
+    out.measure(targets, values, seed)
+
+
+def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None,
+                  measure_seed: Optional[int] = None) -> Tuple[np.ndarray, np.ndarray]:
     """Return (ops, udata) for `circuit`; raises LoweringError on anything the
     GPU strategy does not run (validate_circuit semantics)."""
     n = circuit.num_qubits if num_qubits is None else num_qubits
@@ -250,7 +283,7 @@ def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None) -> Tuple[np
                 raise LoweringError(
                     f"classical transform '{tr.name}' cannot be lowered to the GPU strategy")
         elif isinstance(tr, C.MeasurementTransformation):
-            raise LoweringError("measurement is not supported by the GPU strategy")
+            _lower_measure(out, tr, n, measure_seed)
         else:
             raise LoweringError("Unknown transformation type")
     return out.finish()

@@ -12,7 +12,7 @@ HBM between executions, which is what the serving loop relies on.
 from __future__ import annotations
 
 import ctypes as ct
-from typing import Optional, Sequence
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -186,6 +186,33 @@ class GpuState:
 
     def scale(self, s: float) -> None:
         check(_lib.load().bfly_scale(self.handle, float(s)))
+
+    def measure(self, targets: Sequence[int], values: Optional[Sequence[Optional[bool]]] = None,
+                seed: Optional[int] = None) -> Tuple[int, float]:
+        """apply_measure (state.mojo:447-553) on the device state: collapse onto
+        an outcome of the free targets chosen with the reference's MINSTD draw
+        from `seed` (None: the clock).  Returns (outcome, its probability);
+        bit k of the outcome is the k-th free target."""
+        m = len(targets)
+        vals = list(values) if values else [None] * m
+        if len(vals) != m:
+            raise ValueError("Measurement values size mismatch")
+        t = (ct.c_int32 * max(m, 1))(*targets)
+        v = (ct.c_int32 * max(m, 1))(*[-1 if x is None else int(bool(x)) for x in vals])
+        out, prob = ct.c_uint64(), ct.c_double()
+        check(_lib.load().bfly_measure(self.handle, t, v, m, -1 if seed is None else int(seed),
+                                       ct.byref(out), ct.byref(prob)))
+        return int(out.value), float(prob.value)
+
+    def sample(self, targets: Sequence[int], shots: int, seed: int) -> np.ndarray:
+        """`shots` outcomes of `targets` without collapse (MINSTD sequence from
+        `seed`; shot 0 is the outcome measure() would choose)."""
+        m = len(targets)
+        t = (ct.c_int32 * max(m, 1))(*targets)
+        out = np.zeros(max(int(shots), 1), dtype=np.uint64)
+        check(_lib.load().bfly_sample(self.handle, t, m, int(seed), int(shots),
+                                      out.ctypes.data_as(ct.POINTER(ct.c_uint64))))
+        return out[:int(shots)]
 
     def fill_random(self, seed: int) -> None:
         """re, im ~ N(0,1) from `seed` (SplitMix64 + Box-Muller), normalised."""

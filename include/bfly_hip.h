@@ -66,6 +66,10 @@ extern "C" {
                                  arg_i = 1: re <- 2 mean_re - re, im <- -2 mean_im - im, exactly
                                             apply_grover_diffuser (algos/grover_classical.mojo:284-326);
                                  at most 12 bits unless aux covers every qubit */
+#define BFLY_OP_MEASURE  6    /* apply_measure (state.mojo:447-553): arg_i = number of targets, their list at
+                                 udata[2*aux ..] as (qubit, value) pairs, value -1 = free, 0 / 1 = forced;
+                                 gate = 1: seed in `controls` (int64 >= 0), gate = 0: seed from the clock;
+                                 the plan synchronises on the state's stream at this op (bfly_measure) */
 
 /* ---- gate kinds: the reference's GateKind values (gates.mojo:83-91) ---- */
 #define BFLY_GATE_H      0
@@ -270,6 +274,26 @@ int bfly_inner(bfly_state* a, bfly_state* b, double* re, double* im);          /
 int bfly_diff_norm2(bfly_state* a, bfly_state* b, double* out);                /* sum |a-b|^2 */
 /* Multiply every amplitude by `scale` (renormalisation). */
 int bfly_scale(bfly_state* s, double scale);
+
+/* ---- measurement and sampling: replaces apply_measure (state.mojo:447-553) --
+   targets[k] with values[k] = -1 (free) or 0 / 1 (forced).  Outcome
+   probabilities of the free targets (bit k of an outcome = the k-th free
+   target) among the amplitudes matching the forced values are FP64
+   compensated sums on the device (at most 12 distinct free qubits); the
+   outcome is chosen on the host exactly as the reference does: one MINSTD
+   step rng = (seed mod 2147483647) * 48271 mod 2147483647,
+   r = rng / 2147483647 * total in the state's precision, the first outcome
+   whose running sum reaches r.  The state is collapsed onto it and divided by
+   sqrt(its probability).  seed < 0 takes the clock, as the reference does
+   without a configured measure_seed.  `outcome` / `prob` (optional) receive
+   the chosen outcome and its (unnormalised) probability.  Synchronous. */
+int bfly_measure(bfly_state* s, const int32_t* targets, const int32_t* values, int32_t m, int64_t seed,
+                 uint64_t* outcome, double* prob);
+/* `shots` outcomes of `targets` (all free) without collapse: shot j uses the
+   (j+1)-th state of the same MINSTD sequence (shot 0 is the outcome
+   bfly_measure would choose with `seed`).  seed >= 0, not a multiple of
+   2147483647.  Synchronous; the state is unchanged. */
+int bfly_sample(bfly_state* s, const int32_t* targets, int32_t m, int64_t seed, uint64_t shots, uint64_t* samples);
 
 /* ---- profiling (BFLY_FLAG_PROFILE) -------------------------------------- */
 /* Per kernel family: launches and summed device milliseconds since reset.

@@ -18,6 +18,8 @@
  *   dense unitaries     butterfly/core/transformations_unitary.mojo:22-139
  *   bit reversal        butterfly/core/state.mojo:131-204 (full), 252-301 (partial)
  *   SWAP via 3 CX       butterfly/core/executors.mojo:231-241
+ *   phase flip, diffuser butterfly/algos/grover_classical.mojo:262-326
+ *   measurement         butterfly/core/state.mojo:447-553
  * Build with -O2 -ffp-contract=off so that no FMA contraction changes the
  * rounding of the reference's separate multiply/add expressions.
  *
@@ -347,6 +349,80 @@ static void SFX(apply_diffuse)(int n, real* re, real* im, uint64_t mask, int var
     }
 }
 
+/* MEASURE: apply_measure (state.mojo:447-553) step by step: outcome
+   probabilities of the free targets among the amplitudes matching the forced
+   values, summed naively in index order in the state's precision; one MINSTD
+   step from the seed; the first outcome whose running sum reaches r; the
+   matching amplitudes times 1 / sqrt(p), the others 0.  `tv` pairs are the
+   op's udata entries: (qubit, value), value -1 = free. */
+static int SFX(apply_measure)(int n, real* re, real* im, const double* tv, int m, int64_t seed, char* err,
+                              int errlen) {
+    if (m == 0) return BFLY_OK;
+    int targets[64], values[64], free_targets[64], free_pos[64];
+    int nfree = 0;
+    if (m > 64) { set_err(err, errlen, "too many measurement targets"); return BFLY_EINVAL; }
+    for (int j = 0; j < m; ++j) {
+        targets[j] = (int)tv[2 * j];
+        values[j] = (int)tv[2 * j + 1];
+        if (targets[j] < 0 || targets[j] >= n) { set_err(err, errlen, "Target out of bounds"); return BFLY_EINVAL; }
+    }
+    for (int j = 0; j < m; ++j) {
+        if (values[j] >= 0) {
+            free_pos[j] = -1;
+        } else {
+            free_pos[j] = nfree;
+            free_targets[nfree++] = targets[j];
+        }
+    }
+    if (nfree > 24) { set_err(err, errlen, "too many free measurement targets"); return BFLY_EUNSUPPORTED; }
+    uint64_t size = 1ull << n, outcome_count = 1ull << nfree;
+    real* probs = (real*)calloc(outcome_count, sizeof(real));
+    if (!probs) { set_err(err, errlen, "allocation failed"); return BFLY_ENOMEM; }
+    real total = 0;
+    for (uint64_t i = 0; i < size; ++i) {
+        int is_match = 1;
+        for (int j = 0; j < m; ++j) {
+            if (values[j] >= 0 && (int)((i >> targets[j]) & 1) != values[j]) { is_match = 0; break; }
+        }
+        if (!is_match) continue;
+        uint64_t outcome = 0;
+        for (int k = 0; k < nfree; ++k)
+            if ((i >> free_targets[k]) & 1) outcome |= 1ull << k;
+        real p = re[i] * re[i] + im[i] * im[i];
+        probs[outcome] = probs[outcome] + p;
+        total += p;
+    }
+    if (total <= 0) { free(probs); set_err(err, errlen, "Measurement probability is zero"); return BFLY_EINVAL; }
+    int64_t rng = seed % 2147483647;
+    rng = (rng * 48271) % 2147483647;
+    real r = (real)rng / (real)2147483647 * total;
+    uint64_t chosen = 0;
+    real acc = 0;
+    for (uint64_t o = 0; o < outcome_count; ++o) {
+        acc += probs[o];
+        if (r <= acc) { chosen = o; break; }
+    }
+    real chosen_prob = probs[chosen];
+    free(probs);
+    if (chosen_prob <= 0) { set_err(err, errlen, "Measurement outcome probability is zero"); return BFLY_EINVAL; }
+    real inv = (real)1.0 / (real)sqrt((double)chosen_prob);
+    for (uint64_t i = 0; i < size; ++i) {
+        int is_match = 1;
+        for (int j = 0; j < m; ++j) {
+            int desired = values[j] >= 0 ? values[j] : (int)((chosen >> free_pos[j]) & 1);
+            if ((int)((i >> targets[j]) & 1) != desired) { is_match = 0; break; }
+        }
+        if (is_match) {
+            re[i] = re[i] * inv;
+            im[i] = im[i] * inv;
+        } else {
+            re[i] = 0;
+            im[i] = 0;
+        }
+    }
+    return BFLY_OK;
+}
+
 /* validate_circuit (executors.mojo:129-185) for the lowered op kinds. */
 static int validate_ops(int n, const bfly_op* ops, uint64_t nops, uint64_t udata_len, char* err, int errlen) {
     char buf[160];
@@ -391,6 +467,12 @@ static int validate_ops(int n, const bfly_op* ops, uint64_t nops, uint64_t udata
                     set_err(err, errlen, "unknown diffuser variant"); return BFLY_EINVAL;
                 }
                 break;
+            case BFLY_OP_MEASURE:
+                if (op->arg_i < 0 || op->aux + (uint64_t)op->arg_i > udata_len) {
+                    set_err(err, errlen, "Measurement values size mismatch"); return BFLY_EINVAL;
+                }
+                if (op->gate != 1) { set_err(err, errlen, "the oracle needs a measurement seed"); return BFLY_EINVAL; }
+                break;
             default:
                 set_err(err, errlen, "Unknown transformation type");
                 return BFLY_EUNSUPPORTED;
@@ -423,6 +505,10 @@ int SFX(oracle_execute)(int n, real* re, real* im, const bfly_op* ops, uint64_t 
                 break;
             case BFLY_OP_PHASE_FLIP: SFX(apply_flip)(n, re, im, op->aux, op->controls); break;
             case BFLY_OP_DIFFUSE: SFX(apply_diffuse)(n, re, im, op->aux, op->arg_i); break;
+            case BFLY_OP_MEASURE:
+                rc = SFX(apply_measure)(n, re, im, udata + 2 * op->aux, op->arg_i, (int64_t)op->controls, err, errlen);
+                if (rc) return rc;
+                break;
         }
     }
     return BFLY_OK;

@@ -78,7 +78,44 @@ def parse(text: str):
         elif kind == "unitary":
             steps.append({"kind": "unitary", "target": int(kv["target"]), "m": int(kv["m"]),
                           "controls": int(kv["controls"]), "uoff": int(kv["uoff"])})
+        elif kind == "measure":
+            pairs = [tuple(int(x) for x in tv.split(":")) for tv in kv["targets"].split(",")]
+            steps.append({"kind": "measure", "seeded": int(kv["seeded"]), "seed": int(kv["seed"]),
+                          "targets": [p[0] for p in pairs], "values": [p[1] for p in pairs]})
     return int(head["n"]), steps
+
+
+def apply_measure(psi: np.ndarray, n: int, targets, values, seed: int) -> np.ndarray:
+    """apply_measure (butterfly/core/state.mojo:447-553) in FP64 numpy: outcome
+    probabilities of the free targets among the amplitudes matching the forced
+    values (-1 = free), one MINSTD step from `seed`, the first outcome whose
+    running sum reaches r, collapse and division by sqrt(p)."""
+    idx = np.arange(1 << n, dtype=np.int64)
+    match = np.ones(1 << n, dtype=bool)
+    free = []
+    for t, v in zip(targets, values):
+        if v >= 0:
+            match &= ((idx >> t) & 1) == v
+        else:
+            free.append(t)
+    outcome = np.zeros(1 << n, dtype=np.int64)
+    for k, t in enumerate(free):
+        outcome |= ((idx >> t) & 1) << k
+    p = np.abs(psi) ** 2
+    probs = np.zeros(1 << len(free))
+    np.add.at(probs, outcome[match], p[match])
+    total = probs.sum()
+    if total <= 0:
+        raise ValueError("Measurement probability is zero")
+    rng = (seed % 2147483647) * 48271 % 2147483647
+    r = rng / 2147483647 * total
+    cum = np.cumsum(probs)
+    hit = np.nonzero(r <= cum)[0]
+    chosen = int(hit[0]) if hit.size else 0
+    if probs[chosen] <= 0:
+        raise ValueError("Measurement outcome probability is zero")
+    keep = match & (outcome == chosen)
+    return np.where(keep, psi / np.sqrt(probs[chosen]), 0)
 
 
 def _apply_pair(kind, m, u, v):
@@ -272,4 +309,8 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
             bases = idx[((idx & block) == 0) & ((idx & cm) == cm)]
             cols = bases[:, None] + (np.arange(dim)[None, :] << t)
             psi[cols] = psi[cols] @ U.T
+        elif st["kind"] == "measure":
+            if not st["seeded"]:
+                raise ValueError("replay needs a measurement seed")
+            psi = apply_measure(psi, n, st["targets"], st["values"], st["seed"])
     return psi

@@ -1,0 +1,143 @@
+"""GPU: measurement and sampling through the C ABI against the oracle.
+complex128: collapsed state within 1e-12 of the oracle's; complex64 within
+2e-6.  The outcome probabilities are FP64 compensated sums on the device and
+the reference's naive sums in the oracle, so a draw landing within ~1e-15 of
+an outcome boundary could pick a neighbour; the seeds here do not."""
+import numpy as np
+import pytest
+
+from code_mojo_amd import _lib, lower_circuit, measure, serve
+from code_mojo_amd import workloads as W
+from code_mojo_amd.lowering import OpList
+from code_mojo_amd.state import Batch, GpuState, Plan
+from oracle import oracle as O
+
+from .plan_sim import apply_measure
+
+pytestmark = pytest.mark.gpu
+M = 2147483647
+
+
+def oracle_measure(n, st, targets, values, seed, dtype=np.float64):
+    out = OpList(n)
+    out.measure(targets, [None if v < 0 else bool(v) for v in values], seed)
+    ops, ud = out.finish()
+    re, im = st.re.astype(dtype).copy(), st.im.astype(dtype).copy()
+    O.execute(n, re, im, ops, ud)
+    return re, im
+
+
+def gpu_state(n, st, dtype):
+    g = GpuState(n, dtype)
+    g.upload_arrays(st.re.astype(dtype), st.im.astype(dtype))
+    return g
+
+
+CASES = [
+    (10, [3], [-1], 1234567),
+    (12, [0, 5, 11], [-1, -1, -1], 98765431),
+    (14, [2, 9, 4, 13], [-1, 1, -1, 0], 55555),
+    (16, [15, 0], [-1, -1], 2 ** 35 + 17),
+    (17, list(range(6)), [-1] * 6, 424242424),
+    (18, [1, 3, 5, 7, 9, 11, 13, 15, 17, 0, 2, 4], [-1] * 12, 31337),   # 12 free: 4096 outcomes
+    (13, [4, 4, 7], [-1, -1, -1], 777777),                             # a repeated target
+    (12, [6, 2, 6], [1, -1, -1], 13579135),                            # free and forced on one qubit
+]
+
+
+@pytest.mark.parametrize("n,targets,values,seed", CASES)
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-6)])
+def test_measure_matches_oracle(gpu_lib, n, targets, values, seed, dtype, tol):
+    st = W.normal_state(n, n + seed % 97)
+    g = gpu_state(n, st, dtype)
+    outcome, prob = g.measure(targets, [None if v < 0 else bool(v) for v in values], seed)
+    got = g.download()
+    re_o, im_o = oracle_measure(n, st, targets, values, seed, dtype)
+    assert max(np.abs(got.re - re_o).max(), np.abs(got.im - im_o).max()) <= tol, (outcome, prob)
+    assert abs(g.norm2() - 1.0) < (1e-12 if dtype == np.float64 else 1e-5)
+    g.free()
+
+
+def test_reference_errors_on_gpu(gpu_lib):
+    n = 6
+    g = GpuState(n)                     # |0...0>
+    with pytest.raises(_lib.BflyError, match="Measurement probability is zero"):
+        g.measure([2], [True], 5)
+    with pytest.raises(_lib.BflyError, match="Target out of bounds"):
+        g.measure([6], None, 5)
+    one = GpuState(1)
+    one.upload_arrays(np.array([0.0, 1.0]), np.zeros(2))
+    with pytest.raises(_lib.BflyError, match="outcome probability is zero"):
+        one.measure([0], None, 0)       # seed 0 draws r = 0: outcome 0 with p = 0
+    with pytest.raises(_lib.BflyError, match="distinct free"):
+        GpuState(14).measure(list(range(13)), None, 3)
+    g.free()
+    one.free()
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_FUSED, _lib.MODE_UNFUSED])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+def test_mid_circuit_measurement_in_plans(gpu_lib, mode, dtype, tol):
+    n = 16
+    qc = W.random_layered(n, 4, 5)
+    measure(qc, [2, 7, 12], [None, None, False], seed=1234567)
+    for q in range(0, n, 2):
+        qc.h(q)
+    measure(qc, [0, 1, 2, 3, 4], seed=7654321)
+    ops, ud = lower_circuit(qc)
+    st = W.normal_state(n, 4)
+    re_o, im_o = st.re.copy(), st.im.copy()
+    O.execute(n, re_o, im_o, ops, ud)
+    g = gpu_state(n, st, dtype)
+    p = Plan(n, dtype, ops, ud, _lib.opts(mode=mode))
+    p.execute(g)
+    got = g.download()
+    assert max(np.abs(got.re - re_o).max(), np.abs(got.im - im_o).max()) <= tol
+    p.free()
+    g.free()
+
+
+def test_sample_shot0_and_histogram(gpu_lib):
+    n = 14
+    st = W.normal_state(n, 21)
+    g = gpu_state(n, st, np.float64)
+    targets = [13, 0, 6]
+    seed = 192837465
+    shots = g.sample(targets, 40000, seed)
+    assert np.array_equal(shots, g.sample(targets, 40000, seed))          # deterministic
+    # shot 0 is the outcome apply_measure picks with the same seed
+    psi = st.to_complex()
+    collapsed = apply_measure(psi, n, targets, [-1, -1, -1], seed)
+    idx = np.arange(1 << n)
+    o = ((idx >> 13) & 1) | (((idx >> 0) & 1) << 1) | (((idx >> 6) & 1) << 2)
+    assert set(np.unique(o[np.abs(collapsed) > 0])) == {int(shots[0])}
+    # histogram against the marginal (binomial 5-sigma bars)
+    p = np.array([np.sum(np.abs(psi[o == k]) ** 2) for k in range(8)])
+    counts = np.bincount(shots.astype(np.int64), minlength=8)
+    sigma = np.sqrt(40000 * p * (1 - p))
+    assert np.all(np.abs(counts - 40000 * p) <= 5 * sigma + 1)
+    out_before = g.download()
+    assert np.array_equal(out_before.re, st.re)                            # sampling does not collapse
+    with pytest.raises(_lib.BflyError):
+        g.sample(targets, 10, M)                                          # r stuck at 0
+    g.free()
+
+
+def test_batches_and_engine_reject_measurement(gpu_lib):
+    n = 10
+    qc = W.random_layered(n, 2, 3)
+    measure(qc, [1], seed=99)
+    ops, ud = lower_circuit(qc)
+    g = GpuState(n)
+    p = Plan(n, np.float64, ops, ud, _lib.opts())
+    with pytest.raises(_lib.BflyError, match="not in batches"):
+        Batch([g], [p])
+    job = serve.Job(name="m", n=n, dtype=np.float64, ops=ops, udata=ud)
+    with serve.Engine() as eng:
+        rid = eng.submit(job)
+        eng.wait(rid, 60)
+        assert eng.status(rid)[0] == _lib.REQ_FAILED
+        assert "measurement" in eng.error(rid)
+        eng.release(rid)
+    p.free()
+    g.free()

@@ -107,8 +107,8 @@ def test_lowering_errors():
     with pytest.raises(LoweringError, match="Target out of bounds"):
         lower_circuit(qc)
     qc = QuantumCircuit(3)
-    measure(qc, [0])
-    with pytest.raises(LoweringError, match="measurement"):
+    measure(qc, [0], [None, True])
+    with pytest.raises(LoweringError, match="Measurement values size mismatch"):
         lower_circuit(qc)
     qc = QuantumCircuit(3)
     qc.add_classical("mystery", [0], lambda s, t: None)
