@@ -17,6 +17,7 @@ from .executors import compile_plan, execute, execute_gpu, executor_registry, ge
 from .gates import (Custom_Gate, GateInfo, GateKind, H, P, P_Gate, RX, RX_Gate, RY, RY_Gate, RZ,
                     RZ_Gate, U1, U2, U3, X, Y, Z, H_Gate, X_Gate, Y_Gate, Z_Gate)
 from .lowering import LoweringError, OP_DTYPE, lower_circuit
+from .qasm3 import QasmError, parse_qasm3_file, parse_qasm3_string
 from .state import GpuState, Plan, QuantumState
 
 __all__ = [name for name in dir() if not name.startswith("_")]

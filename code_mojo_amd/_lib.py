@@ -110,6 +110,8 @@ _SIGS = {
     "bfly_inner": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "bfly_diff_norm2": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double)]),
     "bfly_scale": (ct.c_int, [ct.c_void_p, ct.c_double]),
+    "bfly_qasm3_lower": (ct.c_int, [ct.c_char_p, ct.POINTER(ct.c_int32), ct.c_void_p, ct.c_uint64,
+                                    ct.POINTER(ct.c_uint64), ct.c_void_p, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
     "bfly_measure": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), ct.c_int32, ct.c_int64,
                                 ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)]),
     "bfly_sample": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.c_int32, ct.c_int64, ct.c_uint64,
@@ -163,3 +165,17 @@ def ops_ptr(ops: np.ndarray) -> ct.c_void_p:
 def udata_ptr(udata: np.ndarray) -> ct.c_void_p:
     assert udata.dtype == np.float64 and udata.flags["C_CONTIGUOUS"]
     return ct.c_void_p(udata.ctypes.data) if udata.size else ct.c_void_p(0)
+
+
+def qasm3_lower(text: str):
+    """(num_qubits, ops, udata) from OpenQASM 3 text, parsed and lowered by the
+    library's C++ front end (bfly_qasm3_lower)."""
+    lib = load()
+    raw = text.encode("utf-8")
+    n, nops, nud = ct.c_int32(), ct.c_uint64(), ct.c_uint64()
+    check(lib.bfly_qasm3_lower(raw, ct.byref(n), None, 0, ct.byref(nops), None, 0, ct.byref(nud)))
+    ops = np.zeros(nops.value, dtype=OP_DTYPE)
+    udata = np.zeros(2 * nud.value, dtype=np.float64)
+    check(lib.bfly_qasm3_lower(raw, ct.byref(n), ops.ctypes.data if ops.size else None, ops.size, ct.byref(nops),
+                               udata.ctypes.data if udata.size else None, nud.value, ct.byref(nud)))
+    return n.value, ops, udata

@@ -42,6 +42,13 @@ class Job:
     meta: Dict = field(default_factory=dict)
 
 
+def qasm_job(name: str, text: str, dtype=np.float64) -> Job:
+    """A job from OpenQASM 3 text, parsed and lowered by the library's C++
+    front end (no Python circuit objects on the serving path)."""
+    n, ops, udata = _lib.qasm3_lower(text)
+    return Job(name=name, n=n, dtype=dtype, ops=ops, udata=udata)
+
+
 class Engine:
     """Python handle of a bfly_engine."""
 

@@ -295,6 +295,16 @@ int bfly_measure(bfly_state* s, const int32_t* targets, const int32_t* values, i
    2147483647.  Synchronous; the state is unchanged. */
 int bfly_sample(bfly_state* s, const int32_t* targets, int32_t m, int64_t seed, uint64_t shots, uint64_t* samples);
 
+/* ---- circuit ingestion: replaces parse_qasm3_string (utils/qasm3_import.mojo:52-123)
+   and the lowering of its circuit.  The reference's statements (qubit[n], h, x,
+   p(a), cp(a); angles pi, c*pi, pi/d, c/pi, numbers) keep its results and
+   errors; also accepted: named registers, y z s sdg t tdg rx ry rz u3 U u2 u1
+   phase cx cy cz crx cry crz ccx cphase cu1 swap mcp, ctrl(k) @, measure,
+   barrier / bit declarations, several statements per line, arithmetic angle
+   expressions.  Call with ops = NULL (and udata = NULL) to get the counts. */
+int bfly_qasm3_lower(const char* text, int32_t* num_qubits, bfly_op* ops, uint64_t ops_cap, uint64_t* nops,
+                     double* udata, uint64_t udata_cap, uint64_t* udata_len);
+
 /* ---- profiling (BFLY_FLAG_PROFILE) -------------------------------------- */
 /* Per kernel family: launches and summed device milliseconds since reset.
    `name` receives a static string.  Returns BFLY_EINVAL past the last family. */
