@@ -229,6 +229,13 @@ def bench_precision(args) -> int:
     g64.free()
     up.free()
     log(f"[bench] block {t_blk * 1e3:.2f} ms, fused c128 {t_fused * 1e3:.2f} ms, L2 {l2:.3g}, fidelity {fid:.12f}")
+    # complex128 BLOCK mode (FP64 MFMA) on the same circuit
+    p_blk128 = Plan(n, np.float64, ops, ud, _lib.opts(mode=_lib.MODE_BLOCK, flags=_lib.FLAG_PROFILE))
+    lib.bfly_profile_reset()
+    t_blk128 = timed(p_blk128, g128, max(args.steps, 1), reset)
+    prof128 = profile_families(lib)
+    fid_exact_blk128 = g128.prob_mask((1 << n) - 1, rev)
+    log(f"[bench] block c128 {t_blk128 * 1e3:.2f} ms, fidelity vs exact {fid_exact_blk128:.15f}")
     p_plain = Plan(n, np.float64, ops, ud, _lib.opts(mode=_lib.MODE_UNFUSED))
     t_plain = timed(p_plain, g128, 1, reset)
     log(f"[bench] plain c128 {t_plain * 1e3:.1f} ms")
@@ -249,6 +256,10 @@ def bench_precision(args) -> int:
         "plain_c128_ms": round(t_plain * 1e3, 3), "speedup_vs_fused_c128": round(t_fused / t_blk, 3),
         "l2_vs_c128": l2, "fidelity_vs_c128": fid, "fidelity_vs_exact_c64": fid_exact_64,
         "fidelity_vs_exact_c128": fid_exact_128,
+        "block_c128_ms": round(t_blk128 * 1e3, 3), "block_c128_fidelity_vs_exact": fid_exact_blk128,
+        "block_c128_speedup_vs_fused_c128": round(t_fused / t_blk128, 3),
+        "block_c128_kernel_mean_launch_ms": round(prof128.get("block_mfma_kernel", (0.0, 1))[0] /
+                                                  max(prof128.get("block_mfma_kernel", (0.0, 1))[1], 1), 4),
         "roofline": {"bound": "hbm", "achieved": round(bytes_per / (per_launch * 1e-3) / 1e9, 2) if blk_n else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if blk_n else None,

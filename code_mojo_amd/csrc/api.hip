@@ -445,7 +445,16 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                 for (int i = 0; i < tp.k; ++i) put(i ? ",%d" : "%d", tp.qpos[i]);
                 out += " mat=";
                 for (int i = 0; i < 64 * 64; ++i) {
-                    std::snprintf(line, sizeof line, i ? ",%a" : "%a", static_cast<double>(hp.bmat[bp.mat_off + i]));
+                    double v;
+                    if (hp.dtype == BFLY_C64) {
+                        v = static_cast<double>(hp.bmat[bp.mat_off + i]);
+                    } else {   // rows 32..63 of the real form follow from the stored half
+                        const int r = i / 64, c = i % 64;
+                        const int rr = r & 31, cc = r < 32 ? c : (c + 32) & 63;
+                        std::memcpy(&v, &hp.bmat[bp.mat_off + 2 * (rr * 64 + cc)], sizeof v);
+                        if (r >= 32 && c < 32) v = -v;
+                    }
+                    std::snprintf(line, sizeof line, i ? ",%a" : "%a", v);
                     out += line;
                 }
                 out += "\n";

@@ -174,7 +174,8 @@ struct HostPlan {
     std::vector<TileSweep> sweeps;
     std::vector<TileInstr> instrs;
     std::vector<double> udata;   // complex, interleaved
-    std::vector<float> bmat;     // BLOCK mode: 64 x 64 real forms of the blocks' U, row-major
+    std::vector<float> bmat;     // BLOCK mode, 4096 words per block, row-major: complex64 the 64 x 64
+                                 // FP32 real form of U, complex128 the 32 x 64 FP64 half [Ur, -Ui]
     std::vector<double> btab;          // BLOCK mode device tables (column factors, outer angles)
     int64_t measures = 0;              // S_MEASURE steps (they synchronise with the host)
     std::vector<double> brows;         // BLOCK mode row angles [2][n - 4][6] per block (host only)

@@ -980,6 +980,243 @@ __global__ __launch_bounds__(256) void block_mfma_kernel(float* __restrict__ re,
     }
 }
 
+// complex128 BLOCK mode on v_mfma_f64_16x16x4f64.  Only the upper half of the
+// real form, A = [Ur, -Ui] (32 x 64), stays in registers: the real outputs are
+// A [re; im] and the imaginary outputs A [im; -re].  The f64 instruction's
+// result layout differs from the f32 one: lane l holds rows (l / 16) + 4 v
+// of its 16-row block (v = 0..3), column l % 16.  Tiles of 2^K amplitudes
+// (K = 9 .. 11), 16-byte vectors of two amplitudes, phase factors in FP64.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+template <int K>
+__global__ __launch_bounds__(256) void block_mfma64_kernel(double* __restrict__ re, double* __restrict__ im,
+                                                           const TilePass P, const BlockPass B,
+                                                           const double* __restrict__ mat,
+                                                           const double* __restrict__ tab,
+                                                           const double* __restrict__ norm2) {
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NT = 256;
+    constexpr int VN = 2;
+    constexpr uint32_t NV = TS / VN;
+    constexpr int NPF = static_cast<int>((NV + NT - 1) / NT);
+    constexpr int NCOL = static_cast<int>(TS / kBlockDim);
+    constexpr int NGROUP = NCOL / 16;
+    constexpr int NZ = kBlockQubits + 1;
+    static_assert(NT * VN >= 512 || NPF == 1, "part stride");
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* s_re = reinterpret_cast<double*>(lds_raw);
+    double* s_im = s_re + TS;
+    double2* s_cf = reinterpret_cast<double2*>(s_im + TS);
+    double2* s_tf = s_cf + 2 * NCOL * NZ;
+    const uint32_t tid = threadIdx.x;
+    const int lane = static_cast<int>(tid & 63u);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+    const int g = lane >> 4;
+
+    // A fragments: row block r (rows 16 r + lane % 16), k step s2 (k = 4 s2 + g)
+    double afrag[2][16];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int s2 = 0; s2 < 16; ++s2) afrag[r][s2] = mat[(16 * r + (lane & 15)) * 64 + 4 * s2 + g];
+
+    uint32_t qmask = 0;
+#pragma unroll
+    for (int j = 0; j < kBlockQubits; ++j) qmask |= 1u << B.qloc[j];
+    const uint32_t wmask = (TS - 1) & ~qmask;
+    auto eloc = [&](int e) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < kBlockQubits; ++j) x |= static_cast<uint32_t>((e >> j) & 1) << B.qloc[j];
+        return x;
+    };
+    auto col_local = [&](uint32_t col) {
+        uint32_t colloc = 0, x = col, m = wmask;
+        while (m) {
+            const uint32_t low = m & (~m + 1);
+            if (x & 1) colloc |= low;
+            x >>= 1;
+            m ^= low;
+        }
+        return colloc;
+    };
+    const uint32_t lane_j = VN * tid;
+    const uint64_t off_lane = local_to_global<K>(lane_j & (TS - 1), P);
+    const uint32_t lds_lane = swz<double>(lane_j & (TS - 1));
+    const uint32_t dl = col_local(static_cast<uint32_t>(lane & 15));
+    const uint32_t rl_in = swz<double>(dl ^ eloc(g));        // B rows e = g + 4 m
+    const uint32_t rl_out = swz<double>(dl ^ eloc(g));       // output rows e = g + 4 v + 16 h
+    const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
+    const bool pre_any = B.pre_tab != 0, post_any = B.post_tab != 0;
+    const double* outer_ang = tab + B.tab_off + 2 * NCOL * NZ * 2;
+
+    if (pre_any || post_any) {
+        const double2* cf = reinterpret_cast<const double2*>(tab + B.tab_off);
+        for (uint32_t i = tid; i < 2u * NCOL * NZ; i += NT) s_cf[i] = cf[i];
+    }
+    auto tile_factors = [&](uint64_t t, int buf) {
+        if (tid < 2 * NZ) {
+            const int w = static_cast<int>(tid) / NZ, j = static_cast<int>(tid) % NZ;
+            const double* a = outer_ang + static_cast<size_t>(w) * P.nouter * NZ;
+            double x = 0.0;
+            for (int i = 0; i < P.nouter; ++i)
+                if ((t >> i) & 1ull) x += a[i * NZ + j];
+            Z64 z = zexp(x);
+            if (w == 0 && j == 0) { z.r *= scale; z.i *= scale; }
+            s_tf[buf * 2 * NZ + tid] = make_double2(z.r, z.i);
+        }
+    };
+
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    double2 pa[NPF], pb[NPF];
+    auto fetch = [&](uint64_t base) {
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint64_t gi = base + off_lane + local_to_global<K>(static_cast<uint32_t>(c * NT * VN), P);
+                pa[c] = *reinterpret_cast<const double2*>(re + gi);
+                pb[c] = *reinterpret_cast<const double2*>(im + gi);
+            }
+        }
+    };
+    uint64_t tile = blockIdx.x;
+    if (tile < ntiles) {
+        fetch(tile_base<K>(tile, P));
+        if (pre_any || post_any) tile_factors(tile, 0);
+    }
+    for (int buf = 0; tile < ntiles; tile += gridDim.x, buf ^= 1) {
+        const uint64_t base = tile_base<K>(tile, P);
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint32_t a = lds_lane + static_cast<uint32_t>(c * NT * VN);
+                s_re[a] = pa[c].x; s_re[a ^ 1u] = pa[c].y;
+                s_im[a] = pb[c].x; s_im[a ^ 1u] = pb[c].y;
+            }
+        }
+        __syncthreads();
+        {
+            const uint64_t nt = tile + gridDim.x;
+            if (nt < ntiles) {
+                fetch(tile_base<K>(nt, P));
+                if (pre_any || post_any) tile_factors(nt, buf ^ 1);
+            }
+        }
+        const double2* tf = s_tf + buf * 2 * NZ;
+        for (int cg = wave; cg < NGROUP; cg += NT / 64) {
+            const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
+            const uint32_t ucg = swz<double>(col_local(static_cast<uint32_t>(cg * 16)));
+            double br[8], bi[8];   // the lane's elements e = g + 4 m, pre-phased
+            if (pre_any) {
+                Z64 z[NZ];
+                column_factors(s_cf + col * NZ, tf, z);
+                Z64 b0 = z[0];
+                if (g & 1) b0 = zmul(b0, z[1]);
+                if (g & 2) b0 = zmul(b0, z[2]);
+                Z64 f[8];
+                f[0] = b0;
+                f[1] = zmul(f[0], z[3]);
+                f[2] = zmul(f[0], z[4]);
+                f[3] = zmul(f[1], z[4]);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) f[4 + m] = zmul(f[m], z[5]);
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const uint32_t j = rl_in ^ ucg ^ swz<double>(eloc(4 * m));
+                    const double xr = s_re[j], xi = s_im[j];
+                    br[m] = xr * f[m].r - xi * f[m].i;
+                    bi[m] = xr * f[m].i + xi * f[m].r;
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const uint32_t j = rl_in ^ ucg ^ swz<double>(eloc(4 * m));
+                    br[m] = s_re[j] * scale;
+                    bi[m] = s_im[j] * scale;
+                }
+            }
+            // [re'] = A [re; im], [im'] = A [im; -re]
+            d4_t acc_r[2], acc_i[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                acc_r[r] = d4_t{0.0, 0.0, 0.0, 0.0};
+                acc_i[r] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s2 = 0; s2 < 8; ++s2) {
+                    acc_r[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][s2], br[s2], acc_r[r], 0, 0, 0);
+                    acc_i[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][s2], bi[s2], acc_i[r], 0, 0, 0);
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < 8; ++s2) {
+                    acc_r[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][8 + s2], bi[s2], acc_r[r], 0, 0, 0);
+                    acc_i[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][8 + s2], -br[s2], acc_i[r], 0, 0, 0);
+                }
+            }
+            // lane holds e = 16 h + g + 4 v (row block h, accumulator v)
+            if (post_any) {
+                Z64 z[NZ];
+                column_factors(s_cf + NCOL * NZ + col * NZ, tf + NZ, z);
+                Z64 b0 = z[0];
+                if (g & 1) b0 = zmul(b0, z[1]);
+                if (g & 2) b0 = zmul(b0, z[2]);
+                Z64 f[8];   // index v + 4 h: v bits are e bits 2, 3; h is e bit 4
+                f[0] = b0;
+                f[1] = zmul(f[0], z[3]);
+                f[2] = zmul(f[0], z[4]);
+                f[3] = zmul(f[1], z[4]);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) f[4 + v] = zmul(f[v], z[5]);
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const double yr = acc_r[h][v], yi = acc_i[h][v];
+                        const Z64 fq = f[v + 4 * h];
+                        const uint32_t j = rl_out ^ ucg ^ swz<double>(eloc(16 * h + 4 * v));
+                        s_re[j] = yr * fq.r - yi * fq.i;
+                        s_im[j] = yr * fq.i + yi * fq.r;
+                    }
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const uint32_t j = rl_out ^ ucg ^ swz<double>(eloc(16 * h + 4 * v));
+                        s_re[j] = acc_r[h][v];
+                        s_im[j] = acc_i[h][v];
+                    }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NPF; ++c) {
+            if (NPF > 1 || tid < NV) {
+                const uint32_t a = lds_lane + static_cast<uint32_t>(c * NT * VN);
+                double2 x, y;
+                x.x = s_re[a]; x.y = s_re[a ^ 1u];
+                y.x = s_im[a]; y.y = s_im[a ^ 1u];
+                const uint64_t gi = base + off_lane + local_to_global<K>(static_cast<uint32_t>(c * NT * VN), P);
+                *reinterpret_cast<double2*>(re + gi) = x;
+                *reinterpret_cast<double2*>(im + gi) = y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int K>
+hipError_t launch_block64_k(double* re, double* im, const TilePass& P, const BlockPass& B, const double* mat,
+                            const double* tab, const double* norm2, hipStream_t st) {
+    const size_t lds = 2ull * (1ull << K) * sizeof(double) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 2 * 6 * 16;
+    static int grid_cap = 0;
+    hipError_t e = persistent_grid(block_mfma64_kernel<K>, 256, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
+    hipLaunchKernelGGL(block_mfma64_kernel<K>, dim3(grid), dim3(256), lds, st, re, im, P, B, mat, tab, norm2);
+    return hipGetLastError();
+}
+
 template <int K>
 hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
                           const double* tab, const double* norm2, hipStream_t st) {
@@ -1697,7 +1934,17 @@ hipError_t launch_tile_batch(int dtype, int k, int slots, const BatchItem* d_ite
 
 hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, const BlockPass& B, const float* mat,
                              const double* tab, const double* norm2, hipStream_t stream) {
-    if (dtype != BFLY_C64) return hipErrorInvalidValue;
+    if (dtype == BFLY_C128) {
+        double* r = static_cast<double*>(re);
+        double* i = static_cast<double*>(im);
+        const double* m = reinterpret_cast<const double*>(mat);
+        switch (P.k) {
+            case 9: return launch_block64_k<9>(r, i, P, B, m, tab, norm2, stream);
+            case 10: return launch_block64_k<10>(r, i, P, B, m, tab, norm2, stream);
+            case 11: return launch_block64_k<11>(r, i, P, B, m, tab, norm2, stream);
+        }
+        return hipErrorInvalidValue;
+    }
     float* r = static_cast<float*>(re);
     float* i = static_cast<float*>(im);
     switch (P.k) {

@@ -733,12 +733,16 @@ double phase_angle(const bfly_op& op, const Classified& c) {
     return std::atan2(c.m[3], c.m[2]);
 }
 
-int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, const bfly_exec_opts& opts,
-                     std::string& err) {
+int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, uint64_t nops,
+                     const bfly_exec_opts& opts, std::string& err) {
     const int b = kBlockQubits;
-    const int K = n < 12 ? n : 12;
+    // complex128 tiles of 2^11 amplitudes (32 KiB of LDS), complex64 2^12
+    const int kmax = dtype == BFLY_C128 ? 11 : 12;
+    const int K = n < kmax ? n : kmax;
     const uint64_t all = (1ull << n) - 1;
-    const int renorm = opts.renorm_every == 0 ? 8 : opts.renorm_every;
+    // the norm check and renormalisation every 8 blocks is the complex64
+    // default; complex128 keeps the reference's unnormalised arithmetic
+    const int renorm = opts.renorm_every != 0 ? opts.renorm_every : (dtype == BFLY_C64 ? 8 : -1);
     std::vector<uint32_t> remaining(nops);
     for (uint64_t i = 0; i < nops; ++i) remaining[i] = static_cast<uint32_t>(i);
     std::vector<Classified> cls(nops);
@@ -959,12 +963,27 @@ int build_block_plan(HostPlan& plan, int n, const bfly_op* ops, uint64_t nops, c
         bp.renorm = renorm_next ? 1 : 0;
         renorm_next = false;
         bp.mat_off = plan.bmat.size();
-        for (int r = 0; r < 2 * kBlockDim; ++r)
-            for (int k2 = 0; k2 < 2 * kBlockDim; ++k2) {
-                const Cplx u = U[(r & 31) * kBlockDim + (k2 & 31)];
-                const double v = (r < 32) == (k2 < 32) ? u.real() : (r < 32 ? -u.imag() : u.imag());
-                plan.bmat.push_back(static_cast<float>(v));
-            }
+        if (dtype == BFLY_C64) {
+            // the 64 x 64 real form [[Ur, -Ui], [Ui, Ur]] in FP32
+            for (int r = 0; r < 2 * kBlockDim; ++r)
+                for (int k2 = 0; k2 < 2 * kBlockDim; ++k2) {
+                    const Cplx u = U[(r & 31) * kBlockDim + (k2 & 31)];
+                    const double v = (r < 32) == (k2 < 32) ? u.real() : (r < 32 ? -u.imag() : u.imag());
+                    plan.bmat.push_back(static_cast<float>(v));
+                }
+        } else {
+            // complex128: the 32 x 64 upper half [Ur, -Ui] in FP64, stored as
+            // the same 16 KiB (two 32-bit words per double)
+            for (int r = 0; r < kBlockDim; ++r)
+                for (int k2 = 0; k2 < 2 * kBlockDim; ++k2) {
+                    const Cplx u = U[r * kBlockDim + (k2 & 31)];
+                    const double v = k2 < 32 ? u.real() : -u.imag();
+                    float w[2];
+                    std::memcpy(w, &v, sizeof v);
+                    plan.bmat.push_back(w[0]);
+                    plan.bmat.push_back(w[1]);
+                }
+        }
         // phase records summed per table row (FP64), then the device tables
         {
             int row_of[64];
@@ -1097,13 +1116,7 @@ int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uin
         if (st.kind != S_FLIP) plan.passes += 1;  // a phase flip touches 2^(n - |mask|) amplitudes
     };
 
-    if (opts.mode == BFLY_MODE_BLOCK && n >= 9) {
-        if (dtype != BFLY_C64) {
-            err = "BLOCK mode runs complex64 states (FP32 MFMA); use FUSED for complex128";
-            return BFLY_EUNSUPPORTED;
-        }
-        return build_block_plan(plan, n, ops, nops, opts, err);
-    }
+    if (opts.mode == BFLY_MODE_BLOCK && n >= 9) return build_block_plan(plan, n, dtype, ops, nops, opts, err);
     if (!fused) {
         for (uint64_t i = 0; i < nops; ++i) {
             const bfly_op& op = ops[i];

@@ -1,7 +1,7 @@
-"""BLOCK mode schedules (complex64 on FP32 MFMA) on the CPU: the compiled
-blocks (32 x 32 unitaries in FP32, FP64 phase records), replayed by the plan
-interpreter, against the oracle; dense-block count and phase routing for the
-value-encoding circuit; refusal for complex128."""
+"""BLOCK mode schedules on the CPU: the compiled blocks (32 x 32 unitaries
+in FP32 for complex64, FP64 for complex128; FP64 phase tables), replayed by
+the plan interpreter, against the oracle; dense-block count and phase routing
+for the value-encoding circuit; small states stay on fused tiles."""
 import math
 
 import numpy as np
@@ -67,10 +67,33 @@ def test_renorm_steps_every_eight_blocks():
     assert text.count("renorm=1") == lines.count("norm")
 
 
-def test_block_mode_complex128_refused_and_small_states_fused():
-    ops, ud = lower_circuit(W.qft_circuit(12))
-    with pytest.raises(_lib.BflyError, match="complex64"):
-        Plan(12, np.float64, ops, ud, _lib.opts(mode=BLOCK))
+@pytest.mark.parametrize("n,seed", [(9, 5), (11, 6), (13, 7)])
+def test_complex128_block_schedule(n, seed):
+    """complex128 blocks keep U in FP64 (tiles of at most 2^11 amplitudes,
+    no renormalisation unless asked): the replay meets the 1e-12 bar."""
+    rng = np.random.default_rng(seed)
+    ops, ud = random_ops(rng, n, 70, with_blocks=False)
+    psi = W.normal_state(n, seed).to_complex()
+    text = Plan(n, np.float64, ops, ud, _lib.opts(mode=BLOCK)).dump()
+    assert text.count("\nblock ") > 0 and "\nnorm" not in text and "renorm=1" not in text
+    assert all(int(ln.split("k=")[1].split()[0]) <= 11 for ln in text.split("\n") if ln.startswith("block "))
+    assert np.abs(run_plan(text, psi, ud) - oracle(n, ops, ud, psi)).max() < 1e-12
+
+
+def test_complex128_value_encoding_blocks():
+    n = 14
+    qc, v = W.value_encoding_precision(n, 3)
+    ops, ud = lower_circuit(qc)
+    text = Plan(n, np.float64, ops, ud, _lib.opts(mode=BLOCK)).dump()
+    z = np.zeros(1 << n, complex)
+    z[0] = 1
+    got = run_plan(text, z, ud)
+    rev = int(format(v, f"0{n}b")[::-1], 2)
+    assert abs(got[rev]) > 1 - 1e-12 and np.abs(got - oracle(n, ops, ud, z)).max() < 1e-12
+
+
+def test_small_states_run_fused_tiles():
     ops, ud = lower_circuit(W.qft_circuit(7))
-    text = Plan(7, np.float32, ops, ud, _lib.opts(mode=BLOCK)).dump()
-    assert "block " not in text and "tile " in text
+    for dtype in (np.float32, np.float64):
+        text = Plan(7, dtype, ops, ud, _lib.opts(mode=BLOCK)).dump()
+        assert "block " not in text and "tile " in text

@@ -67,3 +67,30 @@ def test_block_renorm_and_layered(gpu_lib):
     err = max(np.abs(re - re_o).max(), np.abs(im - im_o).max())
     assert err <= 2e-5, err
     assert abs(float(np.sum(re.astype(np.float64) ** 2 + im.astype(np.float64) ** 2)) - 1) < 1e-6
+
+
+@pytest.mark.parametrize("n,seed", [(9, 11), (11, 12), (14, 13), (17, 14)])
+def test_block_complex128_random_ops(gpu_lib, n, seed):
+    """complex128 blocks on FP64 MFMA: FP64 products and sums, only the
+    summation order differs from the oracle's (<= 1e-12)."""
+    rng = np.random.default_rng(seed)
+    ops, ud = random_ops(rng, n, 90, with_blocks=False)
+    st = W.normal_state(n, seed)
+    re_o, im_o = run_oracle(n, ops, ud, st.re, st.im)
+    re, im, _ = run_gpu(n, ops, ud, st.re.copy(), st.im.copy(), mode=BLOCK, dtype=np.float64)
+    assert max(np.abs(re - re_o).max(), np.abs(im - im_o).max()) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [20, 24])
+def test_block_complex128_value_encoding_and_renorm(gpu_lib, n):
+    qc, v = W.value_encoding_precision(n, 5 + n)
+    ops, ud = lower_circuit(qc)
+    rev = int(format(v, f"0{n}b")[::-1], 2)
+    for renorm in (0, 4):   # default (none) and a norm check every 4 blocks
+        g = GpuState(n, np.float64)
+        p = Plan(n, np.float64, ops, ud, _lib.opts(mode=BLOCK, renorm_every=renorm))
+        p.execute(g)
+        assert g.prob_mask((1 << n) - 1, rev) > 1 - 1e-12
+        assert abs(g.norm2() - 1) < 1e-12
+        p.free()
+        g.free()
