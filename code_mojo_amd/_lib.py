@@ -18,7 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbfly_hip.so"
 
 BFLY_OK, BFLY_EINVAL, BFLY_ENOMEM, BFLY_EHIP, BFLY_EUNSUPPORTED, BFLY_ENUMERIC = 0, -1, -2, -3, -4, -5
 C128, C64 = 0, 1
-MODE_FUSED, MODE_UNFUSED, MODE_BLOCK = 0, 1, 2
+MODE_FUSED, MODE_UNFUSED, MODE_BLOCK, MODE_AUTO = 0, 1, 2, 3
 FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS = 1, 2, 4
 BATCH_GRAPH = 1
 

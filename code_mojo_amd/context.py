@@ -43,7 +43,7 @@ def execution_strategy_name(value: int) -> str:
     return ExecutionStrategy._names[value] if is_valid_execution_strategy(value) else "UNKNOWN"
 
 
-GPU_MODES = {"fused": 0, "unfused": 1, "block": 2}
+GPU_MODES = {"fused": 0, "unfused": 1, "block": 2, "auto": 3}
 
 
 @dataclass
@@ -66,7 +66,7 @@ class ExecContext:
     validate_circuit: bool = True
     execution_strategy: int = ExecutionStrategy.GPU
     # GPU strategy knobs
-    gpu_mode: str = "fused"      # fused | unfused | block
+    gpu_mode: str = "fused"      # fused | unfused | block | auto
     gpu_tile_bits: int = 0       # 0 = tuned default
     gpu_low_bits: int = 0
     gpu_fuse_k: int = 0          # block width for gpu_mode=block (0 = 5)

@@ -341,7 +341,7 @@ int bfly_plan_create(bfly_plan** out, int32_t num_qubits, int32_t dtype, const b
     if (nops && !ops) return fail(BFLY_EINVAL, "null op array");
     bfly_exec_opts o{};
     if (opts) o = *opts;
-    if (o.mode != BFLY_MODE_FUSED && o.mode != BFLY_MODE_UNFUSED && o.mode != BFLY_MODE_BLOCK)
+    if (o.mode < BFLY_MODE_FUSED || o.mode > BFLY_MODE_AUTO)
         return fail(BFLY_EINVAL, "unknown execution mode");
     auto* p = new bfly_plan();
     std::string err;

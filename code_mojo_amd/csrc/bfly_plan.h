@@ -164,6 +164,7 @@ struct Step {
     int32_t a = 0, b = 0;  // S_SWAP qubits; S_UNITARY: a = target, b = m; S_DIFFUSE: a = variant
     uint64_t controls = 0; // S_UNITARY control mask; S_FLIP: pattern
     uint64_t uoff = 0;     // S_UNITARY: complex offset into the plan's unitary data
+    int32_t gates = 0;     // S_TILE: gates the pass applies (cost model of BFLY_MODE_AUTO)
 };
 
 struct HostPlan {

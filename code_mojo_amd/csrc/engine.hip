@@ -638,7 +638,7 @@ int bfly_engine_submit(bfly_engine* e, const bfly_request* rq, uint64_t* id) {
     }
     if (rq->dtype != BFLY_C128 && rq->dtype != BFLY_C64) return fail(BFLY_EINVAL, "unknown dtype");
     if (rq->nops && !rq->ops) return fail(BFLY_EINVAL, "null op array");
-    if (rq->exec.mode != BFLY_MODE_FUSED && rq->exec.mode != BFLY_MODE_UNFUSED && rq->exec.mode != BFLY_MODE_BLOCK)
+    if (rq->exec.mode < BFLY_MODE_FUSED || rq->exec.mode > BFLY_MODE_AUTO)
         return fail(BFLY_EINVAL, "unknown execution mode");
     // invalid circuits are refused here; compilation runs on a planner thread
     std::string err;

@@ -692,6 +692,7 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
     }
     close();
     tp.nsweeps = static_cast<int32_t>(plan.sweeps.size()) - tp.first_sweep;
+    st.gates = static_cast<int32_t>(gates.size());
     plan.steps.push_back(st);
     plan.passes += 1;
 }
@@ -733,8 +734,33 @@ double phase_angle(const bfly_op& op, const Classified& c) {
     return std::atan2(c.m[3], c.m[2]);
 }
 
+// Predicted execution time of a schedule in units of one full HBM pass over
+// the state (2 * 2^n * b bytes at 8 TB/s), from measured MI355X pass times:
+// a tile pass costs 0.12 per gate it carries, at least 1.6 (30 qubits: 0.5 ms
+// per U3-class gate, 6.9 ms floor, complex128); a dense block 2.2 (complex128,
+// 9.5 ms) or 2.4 (complex64, 5.2 ms); a bit reversal 2.9; other steps about
+// one pass (a flip touches only its pattern).
+double step_cost(const Step& st, int32_t dtype) {
+    switch (st.kind) {
+        case S_TILE: return std::max(1.6, 0.12 * st.gates);
+        case S_BLOCK: return dtype == BFLY_C128 ? 2.2 : 2.4;
+        case S_BITREV: return 2.9;
+        case S_NORM: return 0.6;
+        case S_FLIP: return 0.1;
+        default: return 1.2;
+    }
+}
+
+double schedule_cost(const HostPlan& plan) {
+    double c = 0.0;
+    for (const Step& st : plan.steps) c += step_cost(st, plan.dtype);
+    return c;
+}
+
+// Returns BFLY_EUNSUPPORTED as soon as the schedule's predicted cost exceeds
+// `budget` (AUTO mode compares it with the tile-pass schedule).
 int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, uint64_t nops,
-                     const bfly_exec_opts& opts, std::string& err) {
+                     const bfly_exec_opts& opts, std::string& err, double budget = 1e300) {
     const int b = kBlockQubits;
     // complex128 tiles of 2^11 amplitudes (32 KiB of LDS), complex64 2^12
     const int kmax = dtype == BFLY_C128 ? 11 : 12;
@@ -1027,6 +1053,10 @@ int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, u
         plan.steps.push_back(st);
         plan.passes += 1;
         ++blocks;
+        if (budget < 1e300 && schedule_cost(plan) > budget) {
+            err = "block schedule over budget";
+            return BFLY_EUNSUPPORTED;
+        }
         if (renorm > 0 && blocks % renorm == 0) {
             Step nst{};
             nst.kind = S_NORM;
@@ -1049,6 +1079,38 @@ int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, u
 
 int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
                const double* udata, uint64_t udata_len, const bfly_exec_opts& opts, std::string& err) {
+    if (opts.mode == BFLY_MODE_AUTO) {
+        bfly_exec_opts fo = opts, bo = opts;
+        fo.mode = BFLY_MODE_FUSED;
+        bo.mode = BFLY_MODE_BLOCK;
+        int rc = build_plan(plan, n, dtype, ops, nops, udata, udata_len, fo, err);
+        if (rc || n < 9) return rc;
+        // Grover sequences already became flips / diffusers, and circuits whose
+        // gates are mostly non-diagonal fill blocks with few gates each: in
+        // both cases the tile passes win and the block attempt is skipped
+        bool grover = false;
+        for (const Step& st : plan.steps) grover |= st.kind == S_FLIP || st.kind == S_DIFFUSE;
+        uint64_t ngates = 0, nondiag = 0;
+        for (uint64_t i = 0; i < nops; ++i)
+            if (ops[i].kind == BFLY_OP_GATE) {
+                ++ngates;
+                nondiag += classify(ops[i]).diag ? 0 : 1;
+            }
+        if (!grover && 2 * nondiag <= ngates) {
+            HostPlan alt;
+            alt.n = n;
+            alt.dtype = dtype;
+            alt.opts = bo;
+            if (udata && udata_len) alt.udata.assign(udata, udata + 2 * udata_len);
+            std::string err2;
+            const double fused_cost = schedule_cost(plan);
+            if (build_block_plan(alt, n, dtype, ops, nops, bo, err2, fused_cost) == BFLY_OK &&
+                schedule_cost(alt) < fused_cost)
+                plan = std::move(alt);
+        }
+        plan.opts.mode = BFLY_MODE_AUTO;
+        return BFLY_OK;
+    }
     plan = HostPlan{};
     plan.n = n;
     plan.dtype = dtype;

@@ -42,6 +42,13 @@ class Job:
     meta: Dict = field(default_factory=dict)
 
 
+def default_exec_opts() -> _lib.ExecOpts:
+    """Served circuits compile in AUTO mode: the planner keeps the faster of
+    the tile-pass and the dense-block schedule (QFT-type ladders run as
+    MFMA blocks, layered circuits as tile passes)."""
+    return _lib.opts(mode=_lib.MODE_AUTO)
+
+
 def qasm_job(name: str, text: str, dtype=np.float64) -> Job:
     """A job from OpenQASM 3 text, parsed and lowered by the library's C++
     front end (no Python circuit objects on the serving path)."""
@@ -75,7 +82,7 @@ class Engine:
         if job.init_kind == _lib.INIT_HOST:
             rq.init_re = job.init.re.ctypes.data
             rq.init_im = job.init.im.ctypes.data
-        rq.exec = exec_opts or _lib.opts()
+        rq.exec = exec_opts or default_exec_opts()
         rid = ct.c_uint64()
         check(_lib.load().bfly_engine_submit(self._h, ct.byref(rq), ct.byref(rid)))
         return rid.value
@@ -301,7 +308,7 @@ def run_sequential(jobs: List[Job], arrivals: Optional[np.ndarray] = None, reuse
             g.fill_random(job.init_seed)
         elif job.init_kind == _lib.INIT_HOST:
             g.upload(job.init)
-        p = Plan(job.n, job.dtype, job.ops, job.udata, _lib.opts())
+        p = Plan(job.n, job.dtype, job.ops, job.udata, default_exec_opts())
         p.execute(g)
         g.synchronize()
         t_done = time.perf_counter()

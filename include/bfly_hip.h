@@ -97,7 +97,9 @@ typedef struct bfly_op {
 /* ---- execution modes --------------------------------------------------- */
 #define BFLY_MODE_FUSED   0   /* LDS tile passes: many gates per HBM sweep (default) */
 #define BFLY_MODE_UNFUSED 1   /* one strided butterfly launch per gate */
-#define BFLY_MODE_BLOCK   2   /* gates fused into dense 2^k blocks (MFMA for c64) */
+#define BFLY_MODE_BLOCK   2   /* gates fused into dense 2^5 blocks on MFMA (FP32 for c64, FP64 for c128) */
+#define BFLY_MODE_AUTO    3   /* build the FUSED and the BLOCK schedule, keep the one a cost model
+                                 (measured pass times on MI355X) predicts faster */
 
 typedef struct bfly_exec_opts {
     int32_t mode;            /* BFLY_MODE_* */

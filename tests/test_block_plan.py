@@ -97,3 +97,18 @@ def test_small_states_run_fused_tiles():
     for dtype in (np.float32, np.float64):
         text = Plan(7, dtype, ops, ud, _lib.opts(mode=BLOCK)).dump()
         assert "block " not in text and "tile " in text
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_auto_mode_picks_the_cheaper_schedule(dtype):
+    """AUTO compiles both schedules and keeps the one its cost model predicts
+    faster: dense blocks for QFT-type ladders, tile passes for layered U3
+    circuits; either way the replay equals the oracle."""
+    n = 14
+    tol = 1e-12 if dtype == np.float64 else TOL
+    for qc, want in ((W.qft_circuit(n), "block"), (W.random_layered(n, 12, 4), "tile")):
+        ops, ud = lower_circuit(qc)
+        text = Plan(n, dtype, ops, ud, _lib.opts(mode=_lib.MODE_AUTO)).dump()
+        assert f"\n{want} " in text
+        psi = W.normal_state(n, 1).to_complex()
+        assert np.abs(run_plan(text, psi, ud) - oracle(n, ops, ud, psi)).max() < tol

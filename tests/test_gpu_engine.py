@@ -17,7 +17,7 @@ def solo(job):
         g.fill_random(job.init_seed)
     elif job.init_kind == _lib.INIT_HOST:
         g.upload(job.init)
-    p = Plan(job.n, job.dtype, job.ops, job.udata, _lib.opts())
+    p = Plan(job.n, job.dtype, job.ops, job.udata, serve.default_exec_opts())
     p.execute(g)
     out = g.download()
     p.free()
