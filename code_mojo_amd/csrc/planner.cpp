@@ -726,6 +726,35 @@ void fold_into_block(std::vector<Cplx>& U, const Classified& c, int t, uint32_t 
     }
 }
 
+// U <- G U for a dense 2^m unitary (udata entries, row-major, index bit i =
+// qubit target + i, apply_unitary's convention) acting on block slots
+// `slots[0..m)` for the rows whose control slots `cm` are all set.
+void fold_unitary_into_block(std::vector<Cplx>& U, const double* g, int m, const int* slots, uint32_t cm) {
+    const int dim = 1 << m;
+    uint32_t smask = 0;
+    for (int i = 0; i < m; ++i) smask |= 1u << slots[i];
+    std::vector<Cplx> v(static_cast<size_t>(dim)), w(static_cast<size_t>(dim));
+    std::vector<int> rows(static_cast<size_t>(dim));
+    for (int base = 0; base < kBlockDim; ++base) {
+        if ((static_cast<uint32_t>(base) & smask) || (static_cast<uint32_t>(base) & cm) != cm) continue;
+        for (int e = 0; e < dim; ++e) {
+            int r = base;
+            for (int i = 0; i < m; ++i)
+                if ((e >> i) & 1) r |= 1 << slots[i];
+            rows[e] = r;
+        }
+        for (int col = 0; col < kBlockDim; ++col) {
+            for (int e = 0; e < dim; ++e) v[e] = U[rows[e] * kBlockDim + col];
+            for (int r = 0; r < dim; ++r) {
+                Cplx acc(0.0, 0.0);
+                for (int c = 0; c < dim; ++c) acc += Cplx(g[2 * (r * dim + c)], g[2 * (r * dim + c) + 1]) * v[c];
+                w[r] = acc;
+            }
+            for (int e = 0; e < dim; ++e) U[rows[e] * kBlockDim + col] = w[e];
+        }
+    }
+}
+
 // Phase angle of a controlled-phase gate diag(1, e^{i theta}): the exact
 // angle the circuit gave for P gates (FP64), pi for Z.
 double phase_angle(const bfly_op& op, const Classified& c) {
@@ -818,9 +847,16 @@ int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, u
     size_t head = 0;
     constexpr size_t kScanWindow = 8192;
     std::vector<uint32_t> rest;
+    // dense unitaries whose qubits (range and controls) fit one block are
+    // folded into it like gates
+    auto block_unitary = [&](const bfly_op& op) {
+        if (op.kind != BFLY_OP_UNITARY || op.arg_i < 1 || op.arg_i > b) return false;
+        const uint64_t q = (((1ull << op.arg_i) - 1) << op.target) | op.controls;
+        return popc(q) <= b;
+    };
     while (head < remaining.size()) {
         const bfly_op& front = ops[remaining[head]];
-        if (front.kind != BFLY_OP_GATE) {
+        if (front.kind != BFLY_OP_GATE && !block_unitary(front)) {
             emit_other(front);
             ++head;
             continue;
@@ -843,11 +879,46 @@ int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, u
             const bfly_op& op = ops[oi];
             if (deferred_any && ++scanned_after_defer > kScanWindow) break;
             if ((d_nd & all) == all) break;
-            if (op.kind != BFLY_OP_GATE) {
+            if (op.kind != BFLY_OP_GATE && !block_unitary(op)) {
                 const uint64_t q = op_qubits(op, n);
                 d_nd |= q; d_all |= q;
                 rest.push_back(oi);
                 deferred_any = true;
+                continue;
+            }
+            if (op.kind == BFLY_OP_UNITARY) {
+                const uint64_t range = ((1ull << op.arg_i) - 1) << op.target;
+                const uint64_t qall = range | op.controls;
+                const uint64_t need = qall & ~Q;
+                bool uok = !(range & d_all) && !(qall & d_nd) && !(range & frozen) && nq + popc(need) <= b;
+                if (uok) {
+                    for (const BlockPhase& ph : pre)
+                        if ((ph.omask & need) && (ph.slot >= 0 || popc(ph.omask & need) > 1)) uok = false;
+                    for (const BlockPhase& ph : post)
+                        if (ph.omask & need) uok = false;
+                }
+                if (!uok) {
+                    d_nd |= range; d_all |= qall;
+                    rest.push_back(oi);
+                    deferred_any = true;
+                    continue;
+                }
+                for (int q = 0; q < n; ++q)
+                    if ((need >> q) & 1) { slot_of[q] = nq; qubit_of[nq++] = q; }
+                Q |= need;
+                for (BlockPhase& ph : pre)
+                    if (ph.omask & need) {
+                        ph.slot = slot_of[__builtin_ctzll(ph.omask & need)];
+                        ph.omask &= ~need;
+                    }
+                int slots[kBlockQubits];
+                for (int i = 0; i < op.arg_i; ++i) slots[i] = slot_of[op.target + i];
+                uint32_t cm = 0;
+                for (int q = 0; q < n; ++q)
+                    if ((op.controls >> q) & 1) cm |= 1u << slot_of[q];
+                fold_unitary_into_block(U, plan.udata.data() + 2 * op.aux, op.arg_i, slots, cm);
+                nd_done |= range;
+                ++placed;
                 continue;
             }
             const Classified& c = cls[oi];

@@ -112,3 +112,17 @@ def test_auto_mode_picks_the_cheaper_schedule(dtype):
         assert f"\n{want} " in text
         psi = W.normal_state(n, 1).to_complex()
         assert np.abs(run_plan(text, psi, ud) - oracle(n, ops, ud, psi)).max() < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, TOL)])
+@pytest.mark.parametrize("n,seed", [(9, 21), (12, 22)])
+def test_dense_unitaries_fold_into_blocks(dtype, tol, n, seed):
+    """Dense m-qubit unitaries (m <= 3 here, with controls) whose qubits fit a
+    block are folded into its U instead of running as separate passes."""
+    rng = np.random.default_rng(seed)
+    ops, ud = random_ops(rng, n, 80, with_blocks=True)
+    assert (ops["kind"] == 3).sum() > 3
+    psi = W.normal_state(n, seed).to_complex()
+    text = Plan(n, dtype, ops, ud, _lib.opts(mode=BLOCK, renorm_every=-1)).dump()
+    assert "\nunitary " not in text
+    assert np.abs(run_plan(text, psi, ud) - oracle(n, ops, ud, psi)).max() < tol

@@ -94,3 +94,16 @@ def test_block_complex128_value_encoding_and_renorm(gpu_lib, n):
         assert abs(g.norm2() - 1) < 1e-12
         p.free()
         g.free()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+@pytest.mark.parametrize("n,seed", [(10, 31), (15, 32)])
+def test_block_dense_unitaries_on_mfma(gpu_lib, dtype, tol, n, seed):
+    """Dense controlled unitaries folded into MFMA blocks vs the oracle."""
+    rng = np.random.default_rng(seed)
+    ops, ud = random_ops(rng, n, 90, with_blocks=True)
+    st = W.normal_state(n, seed)
+    re_o, im_o = run_oracle(n, ops, ud, st.re, st.im)
+    re, im, _ = run_gpu(n, ops, ud, st.re.astype(dtype), st.im.astype(dtype), mode=BLOCK, dtype=dtype,
+                        renorm_every=-1)
+    assert max(np.abs(re - re_o).max(), np.abs(im - im_o).max()) <= tol
