@@ -1217,6 +1217,276 @@ hipError_t launch_block64_k(double* re, double* im, const TilePass& P, const Blo
     return hipGetLastError();
 }
 
+// Warp-specialised complex64 BLOCK kernel for tiles of 2^12 amplitudes: one
+// workgroup per CU, 8 waves.  Waves 0-3 multiply (two 16-column groups each,
+// one wave per SIMD; measured faster than two multiplying waves per SIMD),
+// keeping only the upper half A = [Ur, -Ui] of the real form in registers
+// (re' = A [re; im], im' = A [im; -re]); waves 4-5 load tiles straight into
+// LDS with 16-byte global_load_lds (no VGPRs held for data in flight, two
+// tiles ahead); waves 6-7 write finished tiles back and prepare the tile
+// factors of the tile two ahead.  Four LDS tile
+// buffers rotate: in iteration it, buffer it % 4 is multiplied, (it + 1) % 4
+// is landing, (it + 2) % 4 is being filled and (it - 1) % 4 is written back.
+// The LDS layout is float4-granular (a DMA lane writes 16 contiguous bytes):
+// float4 slot P holds local float4 J = P ^ ((P >> 3) & 7), i.e. element j
+// sits at swz4(j) = j ^ (((j >> 5) & 7) << 2).  Barriers are raw s_barrier
+// with compiler memory clobbers; each role waits for its own counters first
+// (compute and store: LDS; load: all but the newest tile's 16 DMA loads).
+constexpr int kWsBuffers = 4;
+constexpr int kWsOuterMax = kMaxQubits - 12;
+constexpr size_t ws_lds_bytes() {
+    return static_cast<size_t>(kWsBuffers) * 2 * 4096 * sizeof(float) + 2ull * 128 * 6 * 16 +
+           static_cast<size_t>(kWsBuffers) * 2 * 6 * 16 + 2ull * kWsOuterMax * 6 * sizeof(double);
+}
+
+__device__ __forceinline__ uint32_t swz4(uint32_t j) { return j ^ (((j >> 5) & 7u) << 2); }
+
+__device__ __forceinline__ void ws_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+__global__ __launch_bounds__(512) void block_ws_kernel(float* __restrict__ re, float* __restrict__ im,
+                                                       const TilePass P, const BlockPass B,
+                                                       const float* __restrict__ mat,
+                                                       const double* __restrict__ tab,
+                                                       const double* __restrict__ norm2) {
+    constexpr int K = 12;
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NB = kWsBuffers;
+    constexpr int NLD = 128;                          // loader / storer lanes
+    constexpr int NCHUNK = static_cast<int>(TS / 4 / NLD);   // DMA instructions per array and tile
+    constexpr int NCOL = static_cast<int>(TS / kBlockDim);
+    constexpr int NGROUP = NCOL / 16;
+    constexpr int NZ = kBlockQubits + 1;
+    constexpr int NCW = 4;                            // compute waves
+    constexpr int NT = 512;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* s_buf = reinterpret_cast<float*>(lds_raw);                         // [NB][re | im][TS]
+    double2* s_cf = reinterpret_cast<double2*>(s_buf + NB * 2 * TS);          // [2][NCOL][NZ]
+    double2* s_tf = s_cf + 2 * NCOL * NZ;                                     // [NB][2][NZ]
+    double* s_oa = reinterpret_cast<double*>(s_tf + NB * 2 * NZ);            // [2][nouter][NZ]
+    const uint32_t tid = threadIdx.x;
+    const int lane = static_cast<int>(tid & 63u);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+    const bool pre_any = B.pre_tab != 0, post_any = B.post_tab != 0;
+    const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const uint32_t niter = static_cast<uint64_t>(blockIdx.x) < ntiles
+                               ? static_cast<uint32_t>((ntiles - 1 - blockIdx.x) / gridDim.x + 1) : 0u;
+    auto tile_of = [&](uint32_t i) { return static_cast<uint64_t>(blockIdx.x) + static_cast<uint64_t>(i) * gridDim.x; };
+
+    if (pre_any || post_any) {
+        const double2* cf = reinterpret_cast<const double2*>(tab + B.tab_off);
+        for (uint32_t i = tid; i < 2u * NCOL * NZ; i += NT) s_cf[i] = cf[i];
+        const double* oa = tab + B.tab_off + 2 * NCOL * NZ * 2;
+        for (uint32_t i = tid; i < 2u * P.nouter * NZ; i += NT) s_oa[i] = oa[i];
+    }
+    // tile factors exp(i sum of the tile's outer-bit angles), pre scaled (from LDS)
+    auto tile_factors = [&](uint32_t i) {
+        const uint32_t r = tid & 127u;
+        if (r < 2 * NZ) {
+            const int w = static_cast<int>(r) / NZ, j = static_cast<int>(r) % NZ;
+            const uint64_t t = tile_of(i);
+            const double* a = s_oa + static_cast<size_t>(w) * P.nouter * NZ;
+            double x = 0.0;
+            for (int q = 0; q < P.nouter; ++q)
+                if ((t >> q) & 1ull) x += a[q * NZ + j];
+            Z64 z = zexp(x);
+            if (w == 0 && j == 0) { z.r *= scale; z.i *= scale; }
+            s_tf[(i % NB) * 2 * NZ + r] = make_double2(z.r, z.i);
+        }
+    };
+    // loader / storer lanes: float4 slot c * 128 + ll holds local float4
+    // c * 128 + jll (the XOR only touches bits 0-2 of ll)
+    const uint32_t ll = tid & 127u;
+    const uint32_t jll = ll ^ ((ll >> 3) & 7u);
+    const uint64_t goff = local_to_global<K>(4u * jll, P);
+    auto issue_loads = [&](uint32_t i) {
+        const uint64_t base = tile_base<K>(tile_of(i), P);
+        float* br = s_buf + (i % NB) * 2 * TS;
+#pragma unroll
+        for (int c = 0; c < NCHUNK; ++c) {
+            const uint64_t g = base + goff + local_to_global<K>(512u * static_cast<uint32_t>(c), P);
+            const uint32_t slot0 = static_cast<uint32_t>(c * NLD + (wave - NCW) * 64);
+            __builtin_amdgcn_global_load_lds(re + g, br + slot0 * 4, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(im + g, br + TS + slot0 * 4, 16, 0, 0);
+        }
+    };
+    constexpr int kVmcnt0 = 0x0F70, kVmcnt16 = 0x4F70, kLgkm0 = 0xC07F;
+
+    // prologue: tiles 0 and 1 in flight, their factors ready, tile 0 landed
+    if (wave == NCW || wave == NCW + 1) {
+        if (niter > 0) issue_loads(0);
+        if (niter > 1) { issue_loads(1); __builtin_amdgcn_s_waitcnt(kVmcnt16); }
+        else __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    }
+    __syncthreads();
+    if (wave >= NCW + 2 && (pre_any || post_any)) {
+        if (niter > 0) tile_factors(0);
+        if (niter > 1) tile_factors(1);
+    }
+    __syncthreads();
+
+    // compute-wave constants
+    float afrag[2][16];   // rows 0-31 of the real form: [Ur, -Ui]
+    uint32_t wmask = 0, rl_in = 0, rl_out = 0;
+    if (wave < NCW) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int s2 = 0; s2 < 16; ++s2) afrag[r][s2] = mat[(16 * r + (lane & 15)) * 64 + 4 * s2 + (lane >> 4)];
+    }
+    uint32_t qmask = 0;
+#pragma unroll
+    for (int j = 0; j < kBlockQubits; ++j) qmask |= 1u << B.qloc[j];
+    wmask = (TS - 1) & ~qmask;
+    auto eloc = [&](int e) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < kBlockQubits; ++j) x |= static_cast<uint32_t>((e >> j) & 1) << B.qloc[j];
+        return x;
+    };
+    auto col_local = [&](uint32_t col) {
+        uint32_t colloc = 0, x = col, m = wmask;
+        while (m) {
+            const uint32_t low = m & (~m + 1);
+            if (x & 1) colloc |= low;
+            x >>= 1;
+            m ^= low;
+        }
+        return colloc;
+    };
+    const int g = lane >> 4;
+    {
+        const uint32_t dl = col_local(static_cast<uint32_t>(lane & 15));
+        rl_in = swz4(dl ^ eloc(g));
+        rl_out = swz4(dl ^ eloc(4 * g));
+    }
+
+    for (uint32_t it = 0; it <= niter; ++it) {
+        if (wave < NCW) {
+            if (it < niter) {
+                float* s_re = s_buf + (it % NB) * 2 * TS;
+                float* s_im = s_re + TS;
+                const double2* tf = s_tf + (it % NB) * 2 * NZ;
+                for (int cg = wave; cg < NGROUP; cg += NCW) {
+                    const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
+                    const uint32_t ucg = swz4(col_local(static_cast<uint32_t>(cg * 16)));
+                    float bfrag[16];
+                    if (pre_any) {
+                        float fr[8], fi[8];
+                        Z64 z[NZ];
+                        column_factors(s_cf + col * NZ, tf, z);
+                        Z64 b0 = z[0];
+                        if (g & 1) b0 = zmul(b0, z[1]);
+                        if (g & 2) b0 = zmul(b0, z[2]);
+                        element_factors(b0, z[3], z[4], z[5], fr, fi);
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            const uint32_t j = rl_in ^ ucg ^ swz4(eloc(4 * m));
+                            const float xr = s_re[j], xi = s_im[j];
+                            bfrag[m] = xr * fr[m] - xi * fi[m];
+                            bfrag[m + 8] = xr * fi[m] + xi * fr[m];
+                        }
+                    } else {
+                        const float sc = static_cast<float>(scale);
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            const uint32_t j = rl_in ^ ucg ^ swz4(eloc(4 * m));
+                            bfrag[m] = s_re[j] * sc;
+                            bfrag[m + 8] = s_im[j] * sc;
+                        }
+                    }
+                    // acc[h]: re of rows 16 h + ..; acc[2 + h]: im (B' = [im; -re])
+                    f4_t acc[4];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        acc[h] = f4_t{0.f, 0.f, 0.f, 0.f};
+                        acc[2 + h] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int s2 = 0; s2 < 16; ++s2)
+                            acc[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[h][s2], bfrag[s2], acc[h], 0, 0, 0);
+#pragma unroll
+                        for (int s2 = 0; s2 < 8; ++s2) {
+                            acc[2 + h] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[h][s2], bfrag[8 + s2], acc[2 + h], 0, 0, 0);
+                            acc[2 + h] = __builtin_amdgcn_mfma_f32_16x16x4f32(afrag[h][8 + s2], -bfrag[s2], acc[2 + h], 0, 0, 0);
+                        }
+                    }
+                    if (post_any) {
+                        float pr[8], pi2[8];
+                        Z64 z[NZ];
+                        column_factors(s_cf + NCOL * NZ + col * NZ, tf + NZ, z);
+                        Z64 b0 = z[0];
+                        if (g & 1) b0 = zmul(b0, z[3]);
+                        if (g & 2) b0 = zmul(b0, z[4]);
+                        element_factors(b0, z[1], z[2], z[5], pr, pi2);
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const float yr = acc[h][v], yi = acc[2 + h][v];
+                                const int q = 4 * h + v;
+                                const uint32_t j = rl_out ^ ucg ^ swz4(eloc(16 * h + v));
+                                s_re[j] = yr * pr[q] - yi * pi2[q];
+                                s_im[j] = yr * pi2[q] + yi * pr[q];
+                            }
+                    } else {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const uint32_t j = rl_out ^ ucg ^ swz4(eloc(16 * h + v));
+                                s_re[j] = acc[h][v];
+                                s_im[j] = acc[2 + h][v];
+                            }
+                    }
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(kLgkm0);
+        } else if (wave < NCW + 2) {
+            if (it + 2 < niter) {
+                issue_loads(it + 2);
+                __builtin_amdgcn_s_waitcnt(kVmcnt16);
+            } else {
+                __builtin_amdgcn_s_waitcnt(kVmcnt0);
+            }
+        } else {
+            // factors first: an LDS write after the stores would wait for them
+            if ((pre_any || post_any) && it + 2 < niter) tile_factors(it + 2);
+            if (it >= 1) {
+                const uint32_t i = it - 1;
+                const uint64_t base = tile_base<K>(tile_of(i), P);
+                const float* br = s_buf + (i % NB) * 2 * TS;
+#pragma unroll
+                for (int c = 0; c < NCHUNK; ++c) {
+                    const uint32_t slot = static_cast<uint32_t>(c * NLD) + ll;
+                    const float4 x = *reinterpret_cast<const float4*>(br + 4 * slot);
+                    const float4 y = *reinterpret_cast<const float4*>(br + TS + 4 * slot);
+                    const uint64_t gi = base + goff + local_to_global<K>(512u * static_cast<uint32_t>(c), P);
+                    *reinterpret_cast<float4*>(re + gi) = x;
+                    *reinterpret_cast<float4*>(im + gi) = y;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(kLgkm0);
+        }
+        ws_barrier();
+    }
+}
+
+hipError_t launch_block_ws(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
+                           const double* tab, const double* norm2, hipStream_t st) {
+    const size_t lds = ws_lds_bytes();
+    static int grid_cap = 0;
+    hipError_t e = persistent_grid(block_ws_kernel, 512, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
+    hipLaunchKernelGGL(block_ws_kernel, dim3(grid), dim3(512), lds, st, re, im, P, B, mat, tab, norm2);
+    return hipGetLastError();
+}
+
 template <int K>
 hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
                           const double* tab, const double* norm2, hipStream_t st) {
@@ -1951,7 +2221,9 @@ hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, c
         case 9: return launch_block_k<9>(r, i, P, B, mat, tab, norm2, stream);
         case 10: return launch_block_k<10>(r, i, P, B, mat, tab, norm2, stream);
         case 11: return launch_block_k<11>(r, i, P, B, mat, tab, norm2, stream);
-        case 12: return launch_block_k<12>(r, i, P, B, mat, tab, norm2, stream);
+        case 12:
+            if (P.nouter <= kWsOuterMax) return launch_block_ws(r, i, P, B, mat, tab, norm2, stream);
+            return launch_block_k<12>(r, i, P, B, mat, tab, norm2, stream);
     }
     return hipErrorInvalidValue;
 }
