@@ -1487,6 +1487,259 @@ hipError_t launch_block_ws(float* re, float* im, const TilePass& P, const BlockP
     return hipGetLastError();
 }
 
+// complex128 counterpart of block_ws_kernel for tiles of 2^11 amplitudes: the
+// same roles and buffer rotation; a 16-byte DMA lane carries two doubles, so
+// 16-byte slot P holds local pair J = P ^ ((P >> 3) & 7) and element j sits at
+// swz2(j) = j ^ (((j >> 4) & 7) << 1).  The compute waves (one 16-column
+// group each) run the FP64 MFMA code of block_mfma64_kernel.
+constexpr int kWs64OuterMax = kMaxQubits - 11;
+constexpr size_t ws64_lds_bytes() {
+    return static_cast<size_t>(kWsBuffers) * 2 * 2048 * sizeof(double) + 2ull * 64 * 6 * 16 +
+           static_cast<size_t>(kWsBuffers) * 2 * 6 * 16 + 2ull * kWs64OuterMax * 6 * sizeof(double);
+}
+
+__device__ __forceinline__ uint32_t swz2(uint32_t j) { return j ^ (((j >> 4) & 7u) << 1); }
+
+__global__ __launch_bounds__(512) void block_ws64_kernel(double* __restrict__ re, double* __restrict__ im,
+                                                         const TilePass P, const BlockPass B,
+                                                         const double* __restrict__ mat,
+                                                         const double* __restrict__ tab,
+                                                         const double* __restrict__ norm2) {
+    constexpr int K = 11;
+    constexpr uint32_t TS = 1u << K;
+    constexpr int NB = kWsBuffers;
+    constexpr int NLD = 128;
+    constexpr int NCHUNK = static_cast<int>(TS / 2 / NLD);
+    constexpr int NCOL = static_cast<int>(TS / kBlockDim);
+    constexpr int NGROUP = NCOL / 16;
+    constexpr int NZ = kBlockQubits + 1;
+    constexpr int NCW = 4;
+    constexpr int NT = 512;
+    static_assert(NGROUP == NCW, "one group per compute wave");
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* s_buf = reinterpret_cast<double*>(lds_raw);                       // [NB][re | im][TS]
+    double2* s_cf = reinterpret_cast<double2*>(s_buf + NB * 2 * TS);          // [2][NCOL][NZ]
+    double2* s_tf = s_cf + 2 * NCOL * NZ;                                     // [NB][2][NZ]
+    double* s_oa = reinterpret_cast<double*>(s_tf + NB * 2 * NZ);            // [2][nouter][NZ]
+    const uint32_t tid = threadIdx.x;
+    const int lane = static_cast<int>(tid & 63u);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+    const bool pre_any = B.pre_tab != 0, post_any = B.post_tab != 0;
+    const double scale = B.renorm ? 1.0 / sqrt(norm2[0]) : 1.0;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const uint32_t niter = static_cast<uint64_t>(blockIdx.x) < ntiles
+                               ? static_cast<uint32_t>((ntiles - 1 - blockIdx.x) / gridDim.x + 1) : 0u;
+    auto tile_of = [&](uint32_t i) { return static_cast<uint64_t>(blockIdx.x) + static_cast<uint64_t>(i) * gridDim.x; };
+
+    if (pre_any || post_any) {
+        const double2* cf = reinterpret_cast<const double2*>(tab + B.tab_off);
+        for (uint32_t i = tid; i < 2u * NCOL * NZ; i += NT) s_cf[i] = cf[i];
+        const double* oa = tab + B.tab_off + 2 * NCOL * NZ * 2;
+        for (uint32_t i = tid; i < 2u * P.nouter * NZ; i += NT) s_oa[i] = oa[i];
+    }
+    auto tile_factors = [&](uint32_t i) {
+        const uint32_t r = tid & 127u;
+        if (r < 2 * NZ) {
+            const int w = static_cast<int>(r) / NZ, j = static_cast<int>(r) % NZ;
+            const uint64_t t = tile_of(i);
+            const double* a = s_oa + static_cast<size_t>(w) * P.nouter * NZ;
+            double x = 0.0;
+            for (int q = 0; q < P.nouter; ++q)
+                if ((t >> q) & 1ull) x += a[q * NZ + j];
+            Z64 z = zexp(x);
+            if (w == 0 && j == 0) { z.r *= scale; z.i *= scale; }
+            s_tf[(i % NB) * 2 * NZ + r] = make_double2(z.r, z.i);
+        }
+    };
+    const uint32_t ll = tid & 127u;
+    const uint32_t jll = ll ^ ((ll >> 3) & 7u);
+    const uint64_t goff = local_to_global<K>(2u * jll, P);
+    auto issue_loads = [&](uint32_t i) {
+        const uint64_t base = tile_base<K>(tile_of(i), P);
+        double* br = s_buf + (i % NB) * 2 * TS;
+#pragma unroll
+        for (int c = 0; c < NCHUNK; ++c) {
+            const uint64_t g = base + goff + local_to_global<K>(256u * static_cast<uint32_t>(c), P);
+            const uint32_t slot0 = static_cast<uint32_t>(c * NLD + (wave - NCW) * 64);
+            __builtin_amdgcn_global_load_lds(re + g, br + slot0 * 2, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(im + g, br + TS + slot0 * 2, 16, 0, 0);
+        }
+    };
+    constexpr int kVmcnt0 = 0x0F70, kVmcnt16 = 0x4F70, kLgkm0 = 0xC07F;
+
+    if (wave == NCW || wave == NCW + 1) {
+        if (niter > 0) issue_loads(0);
+        if (niter > 1) { issue_loads(1); __builtin_amdgcn_s_waitcnt(kVmcnt16); }
+        else __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    }
+    __syncthreads();
+    if (wave >= NCW + 2 && (pre_any || post_any)) {
+        if (niter > 0) tile_factors(0);
+        if (niter > 1) tile_factors(1);
+    }
+    __syncthreads();
+
+    double afrag[2][16];
+    if (wave < NCW) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int s2 = 0; s2 < 16; ++s2) afrag[r][s2] = mat[(16 * r + (lane & 15)) * 64 + 4 * s2 + (lane >> 4)];
+    }
+    uint32_t qmask = 0;
+#pragma unroll
+    for (int j = 0; j < kBlockQubits; ++j) qmask |= 1u << B.qloc[j];
+    const uint32_t wmask = (TS - 1) & ~qmask;
+    auto eloc = [&](int e) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < kBlockQubits; ++j) x |= static_cast<uint32_t>((e >> j) & 1) << B.qloc[j];
+        return x;
+    };
+    auto col_local = [&](uint32_t col) {
+        uint32_t colloc = 0, x = col, m = wmask;
+        while (m) {
+            const uint32_t low = m & (~m + 1);
+            if (x & 1) colloc |= low;
+            x >>= 1;
+            m ^= low;
+        }
+        return colloc;
+    };
+    const int g = lane >> 4;
+    const uint32_t dl = col_local(static_cast<uint32_t>(lane & 15));
+    const uint32_t rl = swz2(dl ^ eloc(g));   // rows e = g + 4 m (in) and e = g + 4 v + 16 h (out)
+
+    for (uint32_t it = 0; it <= niter; ++it) {
+        if (wave < NCW) {
+            if (it < niter) {
+                double* s_re = s_buf + (it % NB) * 2 * TS;
+                double* s_im = s_re + TS;
+                const double2* tf = s_tf + (it % NB) * 2 * NZ;
+                const int cg = wave;
+                const uint32_t col = static_cast<uint32_t>(cg * 16 + (lane & 15));
+                const uint32_t ucg = swz2(col_local(static_cast<uint32_t>(cg * 16)));
+                double br[8], bi[8];
+                if (pre_any) {
+                    Z64 z[NZ];
+                    column_factors(s_cf + col * NZ, tf, z);
+                    Z64 b0 = z[0];
+                    if (g & 1) b0 = zmul(b0, z[1]);
+                    if (g & 2) b0 = zmul(b0, z[2]);
+                    Z64 f[8];
+                    f[0] = b0;
+                    f[1] = zmul(f[0], z[3]);
+                    f[2] = zmul(f[0], z[4]);
+                    f[3] = zmul(f[1], z[4]);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) f[4 + m] = zmul(f[m], z[5]);
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) {
+                        const uint32_t j = rl ^ ucg ^ swz2(eloc(4 * m));
+                        const double xr = s_re[j], xi = s_im[j];
+                        br[m] = xr * f[m].r - xi * f[m].i;
+                        bi[m] = xr * f[m].i + xi * f[m].r;
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) {
+                        const uint32_t j = rl ^ ucg ^ swz2(eloc(4 * m));
+                        br[m] = s_re[j] * scale;
+                        bi[m] = s_im[j] * scale;
+                    }
+                }
+                d4_t acc_r[2], acc_i[2];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    acc_r[r] = d4_t{0.0, 0.0, 0.0, 0.0};
+                    acc_i[r] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int s2 = 0; s2 < 8; ++s2) {
+                        acc_r[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][s2], br[s2], acc_r[r], 0, 0, 0);
+                        acc_i[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][s2], bi[s2], acc_i[r], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int s2 = 0; s2 < 8; ++s2) {
+                        acc_r[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][8 + s2], bi[s2], acc_r[r], 0, 0, 0);
+                        acc_i[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(afrag[r][8 + s2], -br[s2], acc_i[r], 0, 0, 0);
+                    }
+                }
+                if (post_any) {
+                    Z64 z[NZ];
+                    column_factors(s_cf + NCOL * NZ + col * NZ, tf + NZ, z);
+                    Z64 b0 = z[0];
+                    if (g & 1) b0 = zmul(b0, z[1]);
+                    if (g & 2) b0 = zmul(b0, z[2]);
+                    Z64 f[8];
+                    f[0] = b0;
+                    f[1] = zmul(f[0], z[3]);
+                    f[2] = zmul(f[0], z[4]);
+                    f[3] = zmul(f[1], z[4]);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) f[4 + v] = zmul(f[v], z[5]);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const double yr = acc_r[h][v], yi = acc_i[h][v];
+                            const Z64 fq = f[v + 4 * h];
+                            const uint32_t j = rl ^ ucg ^ swz2(eloc(16 * h + 4 * v));
+                            s_re[j] = yr * fq.r - yi * fq.i;
+                            s_im[j] = yr * fq.i + yi * fq.r;
+                        }
+                } else {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const uint32_t j = rl ^ ucg ^ swz2(eloc(16 * h + 4 * v));
+                            s_re[j] = acc_r[h][v];
+                            s_im[j] = acc_i[h][v];
+                        }
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(kLgkm0);
+        } else if (wave < NCW + 2) {
+            if (it + 2 < niter) {
+                issue_loads(it + 2);
+                __builtin_amdgcn_s_waitcnt(kVmcnt16);
+            } else {
+                __builtin_amdgcn_s_waitcnt(kVmcnt0);
+            }
+        } else {
+            if ((pre_any || post_any) && it + 2 < niter) tile_factors(it + 2);
+            if (it >= 1) {
+                const uint32_t i = it - 1;
+                const uint64_t base = tile_base<K>(tile_of(i), P);
+                const double* br = s_buf + (i % NB) * 2 * TS;
+#pragma unroll
+                for (int c = 0; c < NCHUNK; ++c) {
+                    const uint32_t slot = static_cast<uint32_t>(c * NLD) + ll;
+                    const double2 x = *reinterpret_cast<const double2*>(br + 2 * slot);
+                    const double2 y = *reinterpret_cast<const double2*>(br + TS + 2 * slot);
+                    const uint64_t gi = base + goff + local_to_global<K>(256u * static_cast<uint32_t>(c), P);
+                    *reinterpret_cast<double2*>(re + gi) = x;
+                    *reinterpret_cast<double2*>(im + gi) = y;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(kLgkm0);
+        }
+        ws_barrier();
+    }
+}
+
+hipError_t launch_block_ws64(double* re, double* im, const TilePass& P, const BlockPass& B, const double* mat,
+                             const double* tab, const double* norm2, hipStream_t st) {
+    const size_t lds = ws64_lds_bytes();
+    static int grid_cap = 0;
+    hipError_t e = persistent_grid(block_ws64_kernel, 512, lds, &grid_cap);
+    if (e != hipSuccess) return e;
+    const uint64_t ntiles = 1ull << (P.n - P.k);
+    const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
+    hipLaunchKernelGGL(block_ws64_kernel, dim3(grid), dim3(512), lds, st, re, im, P, B, mat, tab, norm2);
+    return hipGetLastError();
+}
+
 template <int K>
 hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
                           const double* tab, const double* norm2, hipStream_t st) {
@@ -2211,7 +2464,9 @@ hipError_t launch_block_pass(int dtype, void* re, void* im, const TilePass& P, c
         switch (P.k) {
             case 9: return launch_block64_k<9>(r, i, P, B, m, tab, norm2, stream);
             case 10: return launch_block64_k<10>(r, i, P, B, m, tab, norm2, stream);
-            case 11: return launch_block64_k<11>(r, i, P, B, m, tab, norm2, stream);
+            case 11:
+                if (P.nouter <= kWs64OuterMax) return launch_block_ws64(r, i, P, B, m, tab, norm2, stream);
+                return launch_block64_k<11>(r, i, P, B, m, tab, norm2, stream);
         }
         return hipErrorInvalidValue;
     }
