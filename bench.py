@@ -263,8 +263,9 @@ def bench_precision(args) -> int:
         "roofline": {"bound": "hbm", "achieved": round(bytes_per / (per_launch * 1e-3) / 1e9, 2) if blk_n else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bytes_per / (per_launch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if blk_n else None,
-                     "traffic": read_traffic(f"block_mfma_kernel_{n}q_value_encoding_traffic.txt"),
-                     "kernel": "block_mfma_kernel", "launches": blk_n,
+                     "traffic": read_traffic(f"block_ws_kernel_{n}q_value_encoding_traffic.txt"),
+                     "kernel": "block_ws_kernel (complex64, 2^12 tiles)" if n >= 12 else "block_mfma_kernel",
+                     "launches": blk_n,
                      "mean_launch_ms": round(per_launch, 4),
                      "mfma_tflops": round(flops_per / (per_launch * 1e-3) / 1e12, 2) if blk_n else None,
                      "mfma_peak_tflops": 157.3},
