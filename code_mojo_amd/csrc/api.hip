@@ -435,7 +435,9 @@ int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed
                     static_cast<unsigned long long>(st.controls));
                 break;
             case S_DIFFUSE:
-                put("diffuse mask=%llu variant=%d\n", static_cast<unsigned long long>(st.mask), st.a);
+                put("diffuse mask=%llu variant=%d chain=%d fmask=%llu fvalue=%llu\n",
+                    static_cast<unsigned long long>(st.mask), st.a, st.chain,
+                    static_cast<unsigned long long>(st.fmask), static_cast<unsigned long long>(st.fvalue));
                 break;
             case S_BLOCK: {
                 const TilePass& tp = st.tile;
@@ -543,7 +545,8 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
                 break;
             case S_DIFFUSE:
                 prof.begin(F_DIFFUSE);
-                e = launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, s->stream);
+                e = launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, st.chain, st.fmask,
+                                   st.fvalue, s->stream);
                 break;
             case S_BLOCK:
                 prof.begin(F_BLOCK);
@@ -649,7 +652,9 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
             return launch_unitary(s->dtype, s->re, s->im, s->n, st.a, st.b, st.controls, p->d_udata + 2 * st.uoff,
                                   stream);
         case S_FLIP: return launch_flip(s->dtype, s->re, s->im, s->n, st.mask, st.controls, stream);
-        case S_DIFFUSE: return launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, stream);
+        case S_DIFFUSE:
+            return launch_diffuse(s->dtype, s->re, s->im, s->n, st.mask, st.a, s->scratch, st.chain, st.fmask,
+                                  st.fvalue, stream);
         case S_BLOCK:
             return launch_block_pass(s->dtype, s->re, s->im, st.tile, st.block, p->d_bmat + st.block.mat_off,
                                      p->d_btab, s->scratch, stream);
@@ -764,7 +769,8 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
                 const Step& st = plans[i]->hp.steps[r];
                 bfly_state* s = states[i];
                 OpItem it = L.kind == S_FLIP ? flip_item(s->re, s->im, s->n, st.mask, st.controls)
-                                             : diffuse_item(s->re, s->im, s->n, st.a, s->scratch);
+                                             : diffuse_item(s->re, s->im, s->n, st.a, s->scratch, st.chain, st.fmask,
+                                                            st.fvalue);
                 it.begin = units;
                 units += it.count;
                 b->ops.push_back(it);

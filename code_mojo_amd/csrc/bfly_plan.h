@@ -155,6 +155,17 @@ struct BlockPass {
     int32_t pre_tab, post_tab;   // 1: that side has a nonzero angle
 };
 
+// Grover chains D [F] D: the apply pass of a whole-state diffuser can also
+// apply the phase flip that follows it and leave the chunk sums the next
+// diffuser needs (the same per-thread order as its own partial pass:
+// identical bits), so the next diffuser skips its partial pass.  Partials
+// alternate between two halves of the state's scratch.
+constexpr int32_t kChainNext = 1;    // apply pass writes the next partials
+constexpr int32_t kChainReady = 2;   // partials already written by the previous diffuser
+constexpr int32_t kChainOdd = 4;     // this diffuser's partials are in the second half
+constexpr int32_t kChainFlip = 8;    // apply pass applies the flip (fmask, fvalue)
+constexpr uint64_t kPartialSpan = 4ull << 14;   // doubles per partial buffer (2^14 chunks)
+
 struct Step {
     StepKind kind;
     TilePass tile;       // S_TILE, S_BLOCK (tile bits = the block's slots + column bits)
@@ -165,6 +176,8 @@ struct Step {
     uint64_t controls = 0; // S_UNITARY control mask; S_FLIP: pattern
     uint64_t uoff = 0;     // S_UNITARY: complex offset into the plan's unitary data
     int32_t gates = 0;     // S_TILE: gates the pass applies (cost model of BFLY_MODE_AUTO)
+    int32_t chain = 0;     // S_DIFFUSE over every qubit: kChain* flags (Grover chains)
+    uint64_t fmask = 0, fvalue = 0;  // S_DIFFUSE with kChainFlip: the absorbed phase flip
 };
 
 struct HostPlan {

@@ -372,7 +372,8 @@ int issue_round(bfly_engine* e) {
                 const Step& st = r->plan->hp.steps[r->next_step];
                 bfly_state* sst = r->state;
                 OpItem it = kind == S_FLIP ? flip_item(sst->re, sst->im, sst->n, st.mask, st.controls)
-                                           : diffuse_item(sst->re, sst->im, sst->n, st.a, sst->scratch);
+                                           : diffuse_item(sst->re, sst->im, sst->n, st.a, sst->scratch, st.chain,
+                                                          st.fmask, st.fvalue);
                 it.begin = units;
                 units += it.count;
                 ho[oused + i] = it;

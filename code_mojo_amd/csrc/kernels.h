@@ -83,9 +83,15 @@ struct OpItem {
     double* partial;      // diffuse: 4 doubles per chunk (the state's scratch)
     int32_t chunk_bits;   // diffuse: log2 of the amplitudes per chunk
     int32_t variant;      // diffuse: 0 gate action, 1 the reference's classical update
+    double* partial_next; // diffuse chains: where the next diffuser's partials go
+    uint64_t fmask;       // diffuse chains: the phase flip applied after the update
+    uint64_t fvalue;
+    int32_t flags;        // kChain* below
+    int32_t pad;
 };
 OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value);
-OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch);
+OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch, int32_t chain = 0,
+                    uint64_t fmask = 0, uint64_t fvalue = 0);
 // PHASE_FLIP: negate every amplitude with (i & mask) == value.  d_items ==
 // nullptr runs `one` alone.
 hipError_t launch_flip_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
@@ -98,6 +104,7 @@ hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint
 // DIFFUSE over the bits of `mask` (every qubit, or at most kDiffuseMaxBits);
 // uses `scratch` (reduce_scratch_doubles()) when the mask covers every qubit.
 hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, int variant, double* scratch,
+                          int32_t chain, uint64_t fmask, uint64_t fvalue,
                           hipStream_t stream);
 
 // (d) reductions.  Partials live in `scratch` (at least reduce_scratch_doubles()

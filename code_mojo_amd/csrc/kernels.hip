@@ -2119,6 +2119,7 @@ __global__ __launch_bounds__(kDiffChunkThreads) void diffuse_partial_kernel(cons
     __shared__ double sh[2][2][kDiffChunkThreads];
     for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
         const OpItem& it = find_item(one, items, nitems, u);
+        if (it.flags & kChainReady) continue;   // written by the previous diffuser's apply pass
         const uint64_t c = u - it.begin;
         const uint64_t len = 1ull << it.chunk_bits;
         const T* re = static_cast<const T*>(it.re) + (c << it.chunk_bits);
@@ -2194,15 +2195,51 @@ __global__ __launch_bounds__(kDiffChunkThreads) void diffuse_apply_kernel(const 
         const uint64_t len = 1ull << it.chunk_bits;
         T* re = static_cast<T*>(it.re) + (c << it.chunk_bits);
         T* im = static_cast<T*>(it.im) + (c << it.chunk_bits);
+        const bool next = (it.flags & kChainNext) != 0;
+        const bool flip = (it.flags & kChainFlip) != 0;
+        const uint64_t g0 = c << it.chunk_bits;
+        double nr = 0, ncr = 0, ni = 0, nci = 0;
         for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
             const double xr = re[i], xi = im[i];
+            T yr, yi;
             if (it.variant == 0) {
-                re[i] = static_cast<T>(xr - ar);
-                im[i] = static_cast<T>(xi - ai);
+                yr = static_cast<T>(xr - ar);
+                yi = static_cast<T>(xi - ai);
             } else {
-                re[i] = static_cast<T>(ar - xr);
-                im[i] = static_cast<T>(-ai - xi);
+                yr = static_cast<T>(ar - xr);
+                yi = static_cast<T>(-ai - xi);
             }
+            if (flip && ((g0 + i) & it.fmask) == it.fvalue) { yr = -yr; yi = -yi; }
+            re[i] = yr;
+            im[i] = yi;
+            if (next) {   // the next diffuser's chunk sums, in its partial pass's order
+                two_sum_add(nr, ncr, static_cast<double>(yr));
+                two_sum_add(ni, nci, static_cast<double>(yi));
+            }
+        }
+        if (next) {
+            sh[0][0][threadIdx.x] = nr; sh[0][1][threadIdx.x] = ncr;
+            sh[1][0][threadIdx.x] = ni; sh[1][1][threadIdx.x] = nci;
+            __syncthreads();
+            for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
+                if (threadIdx.x < static_cast<unsigned>(w)) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
+                        acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
+                        sh[q][0][threadIdx.x] = ss;
+                        sh[q][1][threadIdx.x] = cc;
+                    }
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                it.partial_next[4 * c] = sh[0][0][0];
+                it.partial_next[4 * c + 1] = sh[0][1][0];
+                it.partial_next[4 * c + 2] = sh[1][0][0];
+                it.partial_next[4 * c + 3] = sh[1][1][0];
+            }
+            __syncthreads();
         }
     }
 }
@@ -2680,14 +2717,22 @@ OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value) {
 
 // Chunks of at least 2^14 amplitudes and at most 2^14 chunks: the partials
 // (4 doubles per chunk) fit the state's reduction scratch.
-OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch) {
+OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch, int32_t chain, uint64_t fmask,
+                    uint64_t fvalue) {
+    static_assert(2 * kPartialSpan <= 2ull * (1 << kMargMaxBits) * kRedBlocks + (1 << kMargMaxBits),
+                  "two partial buffers fit the state's reduction scratch");
     OpItem it{};
     it.re = re;
     it.im = im;
     it.chunk_bits = n <= 14 ? n : (n - 14 > 14 ? n - 14 : 14);
     it.count = 1ull << (n - it.chunk_bits);
-    it.partial = scratch;
+    const bool odd = (chain & kChainOdd) != 0;
+    it.partial = scratch + (odd ? kPartialSpan : 0);
+    it.partial_next = scratch + (odd ? 0 : kPartialSpan);
     it.variant = variant;
+    it.flags = chain;
+    it.fmask = fmask;
+    it.fvalue = fvalue;
     return it;
 }
 
@@ -2724,12 +2769,12 @@ hipError_t launch_flip(int dtype, void* re, void* im, int n, uint64_t mask, uint
 }
 
 hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, int variant, double* scratch,
-                          hipStream_t stream) {
+                          int32_t chain, uint64_t fmask, uint64_t fvalue, hipStream_t stream) {
     const uint64_t size = 1ull << n;
     const uint64_t all = size - 1;
     const int t = __builtin_popcountll(mask);
     if (mask == all) {
-        const OpItem it = diffuse_item(re, im, n, variant, scratch);
+        const OpItem it = diffuse_item(re, im, n, variant, scratch, chain, fmask, fvalue);
         return launch_diffuse_items(dtype, it, nullptr, 0, it.count, stream);
     }
     if (t > kDiffuseMaxBits) return hipErrorInvalidValue;

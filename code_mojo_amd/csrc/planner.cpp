@@ -1148,13 +1148,64 @@ int build_block_plan(HostPlan& plan, int n, int32_t dtype, const bfly_op* ops, u
 
 }  // namespace
 
+int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
+                     const double* udata, uint64_t udata_len, const bfly_exec_opts& opts, std::string& err);
+
+namespace {
+// Grover chains: a whole-state diffuser followed (optionally through one
+// phase flip) by another whole-state diffuser of the same variant absorbs the
+// flip into its apply pass and leaves the next diffuser's partial sums
+// (identical bits to running the steps one by one, one HBM read less).
+void chain_diffusers(HostPlan& plan) {
+    const uint64_t all = (1ull << plan.n) - 1;
+    auto whole = [&](const Step& st) { return st.kind == S_DIFFUSE && (st.mask == all || st.mask == 0); };
+    std::vector<Step> out;
+    out.reserve(plan.steps.size());
+    bool odd = false;
+    for (size_t i = 0; i < plan.steps.size(); ++i) {
+        Step st = plan.steps[i];
+        if (whole(st)) {
+            const bool ready = !out.empty() && out.back().kind == S_DIFFUSE && (out.back().chain & kChainNext);
+            if (!ready) odd = false;
+            st.chain = (ready ? kChainReady : 0) | (odd ? kChainOdd : 0);
+            size_t j = i + 1;
+            bool flip = false;
+            if (j < plan.steps.size() && plan.steps[j].kind == S_FLIP) { flip = true; ++j; }
+            if (j < plan.steps.size() && whole(plan.steps[j]) && plan.steps[j].a == st.a) {
+                st.chain |= kChainNext;
+                if (flip) {
+                    st.chain |= kChainFlip;
+                    st.fmask = plan.steps[i + 1].mask;
+                    st.fvalue = plan.steps[i + 1].controls;
+                }
+                out.push_back(st);
+                i = j - 1;          // the flip (if any) is absorbed; the next diffuser comes next
+                odd = !odd;
+                continue;
+            }
+            out.push_back(st);
+            continue;
+        }
+        out.push_back(st);
+    }
+    plan.steps.swap(out);
+}
+}  // namespace
+
 int build_plan(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
                const double* udata, uint64_t udata_len, const bfly_exec_opts& opts, std::string& err) {
+    const int rc0 = build_plan_steps(plan, n, dtype, ops, nops, udata, udata_len, opts, err);
+    if (rc0 == BFLY_OK) chain_diffusers(plan);
+    return rc0;
+}
+
+int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* ops, uint64_t nops,
+                     const double* udata, uint64_t udata_len, const bfly_exec_opts& opts, std::string& err) {
     if (opts.mode == BFLY_MODE_AUTO) {
         bfly_exec_opts fo = opts, bo = opts;
         fo.mode = BFLY_MODE_FUSED;
         bo.mode = BFLY_MODE_BLOCK;
-        int rc = build_plan(plan, n, dtype, ops, nops, udata, udata_len, fo, err);
+        int rc = build_plan_steps(plan, n, dtype, ops, nops, udata, udata_len, fo, err);
         if (rc || n < 9) return rc;
         // Grover sequences already became flips / diffusers, and circuits whose
         // gates are mostly non-diagonal fill blocks with few gates each: in

@@ -74,7 +74,9 @@ def parse(text: str):
         elif kind == "flip":
             steps.append({"kind": "flip", "mask": int(kv["mask"]), "value": int(kv["value"])})
         elif kind == "diffuse":
-            steps.append({"kind": "diffuse", "mask": int(kv["mask"]), "variant": int(kv["variant"])})
+            steps.append({"kind": "diffuse", "mask": int(kv["mask"]), "variant": int(kv["variant"]),
+                          "chain": int(kv.get("chain", 0)), "fmask": int(kv.get("fmask", 0)),
+                          "fvalue": int(kv.get("fvalue", 0))})
         elif kind == "unitary":
             steps.append({"kind": "unitary", "target": int(kv["target"]), "m": int(kv["m"]),
                           "controls": int(kv["controls"]), "uoff": int(kv["uoff"])})
@@ -296,6 +298,8 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
                 psi = psi - 2 * mean
             else:
                 psi = (2 * mean.real - psi.real) + 1j * (-2 * mean.imag - psi.imag)
+            if st["chain"] & 8:   # the phase flip absorbed into this diffuser's pass
+                psi = np.where((idx & st["fmask"]) == st["fvalue"], -psi, psi)
         elif st["kind"] == "swap":
             a, b = st["a"], st["b"]
             x = ((idx >> a) & 1) ^ ((idx >> b) & 1)
