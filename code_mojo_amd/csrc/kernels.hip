@@ -2108,139 +2108,212 @@ __global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double*
     }
 }
 
-// DIFFUSE over every qubit, pass 1: block b sums chunk b of its item (a
-// fixed 2^chunk_bits amplitudes) with TwoSum in a fixed order and leaves
-// (s_re, c_re, s_im, c_im) in the item's partials.
+// DIFFUSE over every qubit.  Chunk c of an item holds amplitudes
+// [c 2^chunk_bits, (c + 1) 2^chunk_bits); an item has at most 1024 chunks.
+// Summation order, shared by the partial pass and a chained apply pass:
+// thread t takes the 16-byte vectors t, t + 256, t + 512, ... of the chunk
+// and each vector's amplitudes in order (TwoSum per thread); lane 0 of each
+// wave merges its wave's pairs by shuffles at distances 32, 16, ..., 1, and
+// thread 0 merges the four waves in order.  Every workgroup of the apply
+// pass merges the item's chunk pairs in the same fixed order (thread t takes
+// chunks t, t + 256, ...; then the same workgroup merge), so all chunks use
+// the same mean and the bits do not depend on the other items of a batch.
 constexpr int kDiffChunkThreads = 256;
+constexpr int kDiffWaves = kDiffChunkThreads / 64;
+constexpr int kDiffVecMinBits = 6;   // smaller chunks go amplitude by amplitude
+
+template <typename T>
+struct DiffVec;
+template <>
+struct DiffVec<float> {
+    using V = float4;
+    static constexpr int E = 4;
+};
+template <>
+struct DiffVec<double> {
+    using V = double2;
+    static constexpr int E = 2;
+};
+
+// v = (s_re, c_re, s_im, c_im); the merged result is valid in thread 0.
+__device__ __forceinline__ void diff_block_merge(double (&v)[4], double (*sh)[4]) {
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) {
+        double o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = __shfl_down(v[q], w, 64);
+        acc_merge(v[0], v[1], o[0], o[1]);
+        acc_merge(v[2], v[3], o[2], o[3]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sh[threadIdx.x >> 6][q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 1; k < kDiffWaves; ++k) {
+            acc_merge(v[0], v[1], sh[k][0], sh[k][1]);
+            acc_merge(v[2], v[3], sh[k][2], sh[k][3]);
+        }
+    }
+    __syncthreads();
+}
+
+// Chunk c's merged pair -> partial[4c .. 4c + 3].
+__device__ __forceinline__ void diff_store_chunk(double (&v)[4], double* partial, uint64_t c, double (*sh)[4]) {
+    diff_block_merge(v, sh);
+    if (threadIdx.x == 0) {
+        reinterpret_cast<double2*>(partial)[2 * c] = make_double2(v[0], v[1]);
+        reinterpret_cast<double2*>(partial)[2 * c + 1] = make_double2(v[2], v[3]);
+    }
+}
+
+// Pass 1: chunk sums (items whose sums the previous diffuser's apply pass
+// already wrote are skipped).
 template <typename T>
 __global__ __launch_bounds__(kDiffChunkThreads) void diffuse_partial_kernel(const OpItem one,
                                                                            const OpItem* __restrict__ items,
                                                                            int nitems, uint64_t total) {
-    __shared__ double sh[2][2][kDiffChunkThreads];
+    using V = typename DiffVec<T>::V;
+    constexpr int E = DiffVec<T>::E;
+    __shared__ double sh[kDiffWaves][4];
     for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
         const OpItem& it = find_item(one, items, nitems, u);
-        if (it.flags & kChainReady) continue;   // written by the previous diffuser's apply pass
+        if (it.flags & kChainReady) continue;
         const uint64_t c = u - it.begin;
         const uint64_t len = 1ull << it.chunk_bits;
         const T* re = static_cast<const T*>(it.re) + (c << it.chunk_bits);
         const T* im = static_cast<const T*>(it.im) + (c << it.chunk_bits);
-        double sr = 0, cr = 0, si = 0, ci = 0;
-        for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
-            two_sum_add(sr, cr, static_cast<double>(re[i]));
-            two_sum_add(si, ci, static_cast<double>(im[i]));
-        }
-        sh[0][0][threadIdx.x] = sr; sh[0][1][threadIdx.x] = cr;
-        sh[1][0][threadIdx.x] = si; sh[1][1][threadIdx.x] = ci;
-        __syncthreads();
-        for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < static_cast<unsigned>(w)) {
+        double v[4] = {0, 0, 0, 0};
+        if (it.chunk_bits >= kDiffVecMinBits) {
+            const V* rv = reinterpret_cast<const V*>(re);
+            const V* iv = reinterpret_cast<const V*>(im);
+#pragma unroll 4
+            for (uint64_t k = threadIdx.x; k < len / E; k += kDiffChunkThreads) {
+                const V xr = rv[k], xi = iv[k];
+                const T* a = reinterpret_cast<const T*>(&xr);
+                const T* b = reinterpret_cast<const T*>(&xi);
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
-                    acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
-                    sh[q][0][threadIdx.x] = ss;
-                    sh[q][1][threadIdx.x] = cc;
+                for (int j = 0; j < E; ++j) {
+                    two_sum_add(v[0], v[1], static_cast<double>(a[j]));
+                    two_sum_add(v[2], v[3], static_cast<double>(b[j]));
                 }
             }
-            __syncthreads();
+        } else {
+            for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
+                two_sum_add(v[0], v[1], static_cast<double>(re[i]));
+                two_sum_add(v[2], v[3], static_cast<double>(im[i]));
+            }
         }
-        if (threadIdx.x == 0) {
-            it.partial[4 * c] = sh[0][0][0];
-            it.partial[4 * c + 1] = sh[0][1][0];
-            it.partial[4 * c + 2] = sh[1][0][0];
-            it.partial[4 * c + 3] = sh[1][1][0];
-        }
-        __syncthreads();
+        diff_store_chunk(v, it.partial, c, sh);
     }
 }
 
-// Pass 2: every block of an item merges the item's chunk partials in the
-// same order (so all chunks use the same mean), then updates its chunk:
-// variant 0: a <- a - 2 mean; variant 1 (the reference's classical
-// diffuser): re <- 2 mean_re - re, im <- -2 mean_im - im.
+// Pass 2: every chunk merges the item's pairs into 2 mean and updates its
+// amplitudes: variant 0: a <- a - 2 mean; variant 1 (the reference's
+// classical diffuser): re <- 2 mean_re - re, im <- -2 mean_im - im.  A
+// chained pass then negates the amplitudes with (i & fmask) == fvalue and
+// sums the results, in pass 1's order, into the next diffuser's partials.
+template <typename T>
+__device__ __forceinline__ void diff_update(T& x_re, T& x_im, double ar, double ai, int variant, bool flip,
+                                            uint64_t idx, uint64_t fmask, uint64_t fvalue, bool next,
+                                            double (&v)[4]) {
+    const double xr = x_re, xi = x_im;
+    T yr, yi;
+    if (variant == 0) {
+        yr = static_cast<T>(xr - ar);
+        yi = static_cast<T>(xi - ai);
+    } else {
+        yr = static_cast<T>(ar - xr);
+        yi = static_cast<T>(-ai - xi);
+    }
+    if (flip && (idx & fmask) == fvalue) {
+        yr = -yr;
+        yi = -yi;
+    }
+    x_re = yr;
+    x_im = yi;
+    if (next) {
+        two_sum_add(v[0], v[1], static_cast<double>(yr));
+        two_sum_add(v[2], v[3], static_cast<double>(yi));
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kDiffChunkThreads) void diffuse_apply_kernel(const OpItem one,
                                                                          const OpItem* __restrict__ items,
                                                                          int nitems, uint64_t total) {
-    __shared__ double sh[2][2][kDiffChunkThreads];
+    using V = typename DiffVec<T>::V;
+    constexpr int E = DiffVec<T>::E;
+    __shared__ double sh[kDiffWaves][4];
+    __shared__ double mean[2];
     for (uint64_t u = blockIdx.x; u < total; u += gridDim.x) {
         const OpItem& it = find_item(one, items, nitems, u);
         const uint64_t c = u - it.begin;
-        double sr = 0, cr = 0, si = 0, ci = 0;
-        for (uint64_t i = threadIdx.x; i < it.count; i += kDiffChunkThreads) {
-            acc_merge(sr, cr, it.partial[4 * i], it.partial[4 * i + 1]);
-            acc_merge(si, ci, it.partial[4 * i + 2], it.partial[4 * i + 3]);
-        }
-        sh[0][0][threadIdx.x] = sr; sh[0][1][threadIdx.x] = cr;
-        sh[1][0][threadIdx.x] = si; sh[1][1][threadIdx.x] = ci;
-        __syncthreads();
-        for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < static_cast<unsigned>(w)) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
-                    acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
-                    sh[q][0][threadIdx.x] = ss;
-                    sh[q][1][threadIdx.x] = cc;
-                }
+        {
+            const double2* pp = reinterpret_cast<const double2*>(it.partial);
+            double m[4] = {0, 0, 0, 0};
+            for (uint64_t i = threadIdx.x; i < it.count; i += kDiffChunkThreads) {
+                const double2 a = pp[2 * i], b = pp[2 * i + 1];
+                acc_merge(m[0], m[1], a.x, a.y);
+                acc_merge(m[2], m[3], b.x, b.y);
+            }
+            diff_block_merge(m, sh);
+            if (threadIdx.x == 0) {
+                const int nbits = it.chunk_bits + (63 - __builtin_clzll(it.count));
+                const double inv = ldexp(1.0, -nbits);   // 2^-n is exact: mean = sum * 2^-n
+                mean[0] = 2.0 * ((m[0] + m[1]) * inv);
+                mean[1] = 2.0 * ((m[2] + m[3]) * inv);
             }
             __syncthreads();
         }
-        // 2^-n is exact: mean = sum * 2^-n
-        const int n = it.chunk_bits + (63 - __builtin_clzll(it.count));
-        const double inv = ldexp(1.0, -n);
-        const double ar = 2.0 * ((sh[0][0][0] + sh[0][1][0]) * inv);
-        const double ai = 2.0 * ((sh[1][0][0] + sh[1][1][0]) * inv);
-        __syncthreads();
+        const double ar = mean[0], ai = mean[1];
         const uint64_t len = 1ull << it.chunk_bits;
-        T* re = static_cast<T*>(it.re) + (c << it.chunk_bits);
-        T* im = static_cast<T*>(it.im) + (c << it.chunk_bits);
+        const uint64_t g0 = c << it.chunk_bits;
+        T* re = static_cast<T*>(it.re) + g0;
+        T* im = static_cast<T*>(it.im) + g0;
         const bool next = (it.flags & kChainNext) != 0;
         const bool flip = (it.flags & kChainFlip) != 0;
-        const uint64_t g0 = c << it.chunk_bits;
-        double nr = 0, ncr = 0, ni = 0, nci = 0;
-        for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads) {
-            const double xr = re[i], xi = im[i];
-            T yr, yi;
-            if (it.variant == 0) {
-                yr = static_cast<T>(xr - ar);
-                yi = static_cast<T>(xi - ai);
-            } else {
-                yr = static_cast<T>(ar - xr);
-                yi = static_cast<T>(-ai - xi);
-            }
-            if (flip && ((g0 + i) & it.fmask) == it.fvalue) { yr = -yr; yi = -yi; }
-            re[i] = yr;
-            im[i] = yi;
-            if (next) {   // the next diffuser's chunk sums, in its partial pass's order
-                two_sum_add(nr, ncr, static_cast<double>(yr));
-                two_sum_add(ni, nci, static_cast<double>(yi));
-            }
-        }
-        if (next) {
-            sh[0][0][threadIdx.x] = nr; sh[0][1][threadIdx.x] = ncr;
-            sh[1][0][threadIdx.x] = ni; sh[1][1][threadIdx.x] = nci;
-            __syncthreads();
-            for (int w = kDiffChunkThreads / 2; w > 0; w >>= 1) {
-                if (threadIdx.x < static_cast<unsigned>(w)) {
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        double ss = sh[q][0][threadIdx.x], cc = sh[q][1][threadIdx.x];
-                        acc_merge(ss, cc, sh[q][0][threadIdx.x + w], sh[q][1][threadIdx.x + w]);
-                        sh[q][0][threadIdx.x] = ss;
-                        sh[q][1][threadIdx.x] = cc;
-                    }
+        const int variant = it.variant;
+        const uint64_t fmask = it.fmask, fvalue = it.fvalue;
+        double v[4] = {0, 0, 0, 0};
+        if (it.chunk_bits >= kDiffVecMinBits) {
+            V* rv = reinterpret_cast<V*>(re);
+            V* iv = reinterpret_cast<V*>(im);
+            const uint64_t nv = len / E;
+            // two vectors of re and im in flight per step, updated in order
+            for (uint64_t k = threadIdx.x; k < nv; k += 2 * kDiffChunkThreads) {
+                const uint64_t k2 = k + kDiffChunkThreads;
+                const bool two = k2 < nv;
+                V xr0 = rv[k], xi0 = iv[k], xr1{}, xi1{};
+                if (two) {
+                    xr1 = rv[k2];
+                    xi1 = iv[k2];
                 }
-                __syncthreads();
+                T* a = reinterpret_cast<T*>(&xr0);
+                T* b = reinterpret_cast<T*>(&xi0);
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    diff_update(a[j], b[j], ar, ai, variant, flip, g0 + k * E + j, fmask, fvalue, next, v);
+                rv[k] = xr0;
+                iv[k] = xi0;
+                if (two) {
+                    a = reinterpret_cast<T*>(&xr1);
+                    b = reinterpret_cast<T*>(&xi1);
+#pragma unroll
+                    for (int j = 0; j < E; ++j)
+                        diff_update(a[j], b[j], ar, ai, variant, flip, g0 + k2 * E + j, fmask, fvalue, next, v);
+                    rv[k2] = xr1;
+                    iv[k2] = xi1;
+                }
             }
-            if (threadIdx.x == 0) {
-                it.partial_next[4 * c] = sh[0][0][0];
-                it.partial_next[4 * c + 1] = sh[0][1][0];
-                it.partial_next[4 * c + 2] = sh[1][0][0];
-                it.partial_next[4 * c + 3] = sh[1][1][0];
-            }
-            __syncthreads();
+        } else {
+            for (uint64_t i = threadIdx.x; i < len; i += kDiffChunkThreads)
+                diff_update(re[i], im[i], ar, ai, variant, flip, g0 + i, fmask, fvalue, next, v);
         }
+        if (next) diff_store_chunk(v, it.partial_next, c, sh);
+        __syncthreads();   // mean[] and sh[] are rewritten by the next chunk
     }
 }
 
@@ -2715,16 +2788,19 @@ OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value) {
     return it;
 }
 
-// Chunks of at least 2^14 amplitudes and at most 2^14 chunks: the partials
-// (4 doubles per chunk) fit the state's reduction scratch.
+// Chunks of at least 2^12 amplitudes (16 per thread) and at most 2^10
+// chunks: every apply workgroup merges at most 32 KB of pairs, and two
+// partial buffers fit the state's reduction scratch.
 OpItem diffuse_item(void* re, void* im, int n, int variant, double* scratch, int32_t chain, uint64_t fmask,
                     uint64_t fvalue) {
     static_assert(2 * kPartialSpan <= 2ull * (1 << kMargMaxBits) * kRedBlocks + (1 << kMargMaxBits),
                   "two partial buffers fit the state's reduction scratch");
+    constexpr int kMinChunkBits = 12, kMaxCountBits = 10;
+    static_assert(4ull << kMaxCountBits <= kPartialSpan, "the pairs fit one partial buffer");
     OpItem it{};
     it.re = re;
     it.im = im;
-    it.chunk_bits = n <= 14 ? n : (n - 14 > 14 ? n - 14 : 14);
+    it.chunk_bits = n <= kMinChunkBits ? n : (n - kMaxCountBits > kMinChunkBits ? n - kMaxCountBits : kMinChunkBits);
     it.count = 1ull << (n - it.chunk_bits);
     const bool odd = (chain & kChainOdd) != 0;
     it.partial = scratch + (odd ? kPartialSpan : 0);
@@ -2749,14 +2825,18 @@ hipError_t launch_flip_items(int dtype, const OpItem& one, const OpItem* d_items
 hipError_t launch_diffuse_items(int dtype, const OpItem& one, const OpItem* d_items, int nitems, uint64_t total,
                                 hipStream_t stream) {
     const unsigned grid = static_cast<unsigned>(total < 65536 ? total : 65536);
+    // a lone item whose sums are already there needs no pass 1
+    const bool partial = d_items != nullptr || (one.flags & kChainReady) == 0;
     if (dtype == BFLY_C64) {
-        hipLaunchKernelGGL(diffuse_partial_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one, d_items,
-                           nitems, total);
+        if (partial)
+            hipLaunchKernelGGL(diffuse_partial_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
+                               d_items, nitems, total);
         hipLaunchKernelGGL(diffuse_apply_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one, d_items,
                            nitems, total);
     } else {
-        hipLaunchKernelGGL(diffuse_partial_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
-                           d_items, nitems, total);
+        if (partial)
+            hipLaunchKernelGGL(diffuse_partial_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
+                               d_items, nitems, total);
         hipLaunchKernelGGL(diffuse_apply_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream, one,
                            d_items, nitems, total);
     }
