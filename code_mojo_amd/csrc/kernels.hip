@@ -205,26 +205,42 @@ __device__ __forceinline__ void slot_pairs(T* ar, T* ai, uint32_t cslots, const 
     }
 }
 
-__device__ __forceinline__ void flip_sign(double& x, uint32_t on) {
-    uint64_t b = __double_as_longlong(x);
-    b ^= static_cast<uint64_t>(on) << 63;
-    x = __longlong_as_double(b);
-}
-__device__ __forceinline__ void flip_sign(float& x, uint32_t on) {
-    uint32_t b = __float_as_uint(x);
-    b ^= on << 31;
-    x = __uint_as_float(b);
-}
+// Fused multiply-add in the state's precision (a single rounding).
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Z family: flip the sign bit of the members selected in sel1 (exactly the
-// product by -1).
+// Z family: multiply the members selected in sel1 by -1 and the others by 1
+// (exact: the products flip or keep the sign bit).  sel1 is the same in
+// every lane, so the factors are scalars; whole-register products keep the
+// register arrays in place, where a sign-bit XOR on the high half of each
+// double costs a copy of the low half.
 template <typename T, int G>
 __device__ __forceinline__ void neg_apply(T* ar, T* ai, uint32_t sel1) {
 #pragma unroll
     for (int m = 0; m < G; ++m) {
-        const uint32_t on = (sel1 >> m) & 1u;
-        flip_sign(ar[m], on);
-        flip_sign(ai[m], on);
+        const T f = ((sel1 >> m) & 1u) ? T(-1) : T(1);
+        ar[m] *= f;
+        ai[m] *= f;
+    }
+}
+
+// Z family with the target bit outside the slots: `bit` (per lane or per
+// tile) is the target's value; members in mm1 flip when it is 1, members in
+// mm0 when it is 0 (mm1 and mm0 are the same in every lane).  With
+// F = -1 where bit is 1 and 1 elsewhere, member m's factor is a F + b with
+// scalar (a, b): (1, 0) for mm1 only, (-1, 0) for mm0 only, (0, -1) for
+// both, (0, 1) for neither; every value is exact.
+template <typename T, int G>
+__device__ __forceinline__ void neg_apply_bit(T* ar, T* ai, uint32_t mm1, uint32_t mm0, uint32_t bit) {
+    const T F = bit ? T(-1) : T(1);
+#pragma unroll
+    for (int m = 0; m < G; ++m) {
+        const uint32_t f1 = (mm1 >> m) & 1u, f0 = (mm0 >> m) & 1u;
+        const T a = f1 == f0 ? T(0) : (f1 ? T(1) : T(-1));
+        const T b = f1 != f0 ? T(0) : (f1 ? T(-1) : T(1));
+        const T f = fmat(a, F, b);
+        ar[m] *= f;
+        ai[m] *= f;
     }
 }
 
@@ -247,9 +263,6 @@ __device__ __forceinline__ float fabs_t(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ bool signbit_t(double x) { return __builtin_signbit(x); }
 __device__ __forceinline__ bool signbit_t(float x) { return __builtin_signbitf(x); }
 
-// Fused multiply-add in the state's precision (a single rounding).
-__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // Register copy of the per-sweep part of a TileInstr (parameters in the
 // state's precision): 5 + 8 or 16 scalar registers.
@@ -417,22 +430,20 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
                 } else if (kind == T_PAIR) {
                     BFLY_SLOT_SWITCH(S, slot, (slot_pairs<T, G, SL>(ar, ai, cslots, g)))
                 } else {
-                    uint32_t sel1 = in.mm & 0xffffu, sel0 = in.mm >> 16;
+                    uint32_t sel1 = __builtin_amdgcn_readfirstlane(in.mm & 0xffffu);
+                    uint32_t sel0 = __builtin_amdgcn_readfirstlane(in.mm >> 16);
                     if (slot < 0) {
                         // target outside the slots: one bit per thread (local) or per tile (outer)
                         const int obitpos = sbyte(in.w1, 0);
                         const uint32_t bit = dlocal >= 0 ? ((jb >> dlocal) & 1u)
                                                          : static_cast<uint32_t>((base >> obitpos) & 1ull);
                         if (kind == T_NEG) {
-                            sel1 = bit ? sel1 : sel0;
+                            neg_apply_bit<T, G>(ar, ai, sel1, sel0, bit);
                         } else {
-                            sel1 = bit ? sel1 : 0u;
-                            sel0 = bit ? 0u : sel0;
+                            diag_apply<T, G>(ar, ai, bit ? sel1 : 0u, bit ? 0u : sel0, g);
                         }
-                    }
-                    if (kind == T_NEG) {
-                        if (slot >= 0) neg_apply<T, G>(ar, ai, __builtin_amdgcn_readfirstlane(sel1));
-                        else neg_apply<T, G>(ar, ai, sel1);
+                    } else if (kind == T_NEG) {
+                        neg_apply<T, G>(ar, ai, sel1);
                     } else {
                         diag_apply<T, G>(ar, ai, sel1, sel0, g);
                     }

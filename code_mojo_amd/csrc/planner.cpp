@@ -254,15 +254,25 @@ bool blocks_phase(const Micro& u, int q) {
     return u.q == q && (u.kind == Micro::ROT || (u.kind == Micro::GATE && !u.pg->c.diag));
 }
 
+// Phases (factor on target bit 1) carried from one tile pass to a later one.
+using PhaseCarry = std::vector<std::pair<int, Cx>>;
+
 // Builds the instruction list of a tile pass: uncontrolled one-qubit unitaries
 // are factored (T_ROT between phases), phases on one qubit are moved forward
 // through everything that commutes with them (diagonal gates, gates on other
 // targets, gates controlled by that qubit) and multiplied together, and the
 // pass's scale factors are applied by T_SCALE whenever their product falls
-// below 2^-16 and at the end.
-std::vector<Micro> micro_ops(const std::vector<PendingGate>& gates, bool factor) {
+// below 2^-16 and at the end.  Phases carried in from earlier passes go
+// first; a phase with no non-diagonal action on its qubit after it in this
+// pass leaves through `carry_out` when `may_carry(q)` (a later pass applies
+// it, usually as the pre-phase of the qubit's next rotation, which costs
+// nothing) instead of becoming an instruction of its own.
+template <typename MayCarry>
+std::vector<Micro> micro_ops(const std::vector<PendingGate>& gates, bool factor, const PhaseCarry& carry_in,
+                             PhaseCarry* carry_out, MayCarry&& may_carry) {
     std::vector<Micro> u;
-    u.reserve(gates.size() * 3 + 4);
+    u.reserve(gates.size() * 3 + carry_in.size() + 4);
+    for (const auto& c : carry_in) u.push_back({Micro::PH, c.first, nullptr, c.second, 0, 0.0});
     for (const PendingGate& pg : gates) {
         const bfly_op& op = *pg.op;
         const bool gate = op.kind == BFLY_OP_GATE && factor;
@@ -315,6 +325,10 @@ std::vector<Micro> micro_ops(const std::vector<PendingGate>& gates, bool factor)
             long anchor = -1;
             for (size_t j = i + 1; j < N && anchor < 0; ++j)
                 if (live[j] && blocks_phase(u[j], u[i].q)) anchor = 2 * static_cast<long>(j) - 1;
+            if (anchor < 0 && carry_out && may_carry(u[i].q)) {
+                carry_out->push_back({u[i].q, u[i].z});
+                continue;
+            }
             for (size_t j = i; j-- > 0 && anchor < 0;)
                 if (live[j] && blocks_phase(u[j], u[i].q)) anchor = 2 * static_cast<long>(j) + 1;
             if (anchor >= 0) key = anchor;
@@ -442,8 +456,9 @@ std::vector<bfly_op> rewrite_grover(const bfly_op* ops, uint64_t nops, int n, bo
     return out;
 }
 
+template <typename MayCarry>
 void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, const std::vector<PendingGate>& gates,
-                    bool factor) {
+                    bool factor, const PhaseCarry& carry_in, PhaseCarry* carry_out, MayCarry&& may_carry) {
     Step st{};
     st.kind = S_TILE;
     TilePass& tp = st.tile;
@@ -460,7 +475,7 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
     tp.nouter = oi;
     tp.first_sweep = static_cast<int32_t>(plan.sweeps.size());
 
-    const std::vector<Micro> micro = micro_ops(gates, factor);
+    const std::vector<Micro> micro = micro_ops(gates, factor, carry_in, carry_out, may_carry);
     int slots[kSlots] = {0, 0, 0, 0};
     int nslots = 0;
     std::vector<const Micro*> cur;
@@ -482,6 +497,50 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
             }
         }
         plan.instrs.push_back(in);
+    };
+    // Negation where (index & mask) == pattern: the pattern's slot bits
+    // select members, the others are tested per lane / tile.
+    auto flip_instr = [&](uint64_t mask, uint64_t pattern) {
+        TileInstr in{};
+        in.dlocal = -1;
+        in.obitpos = -1;
+        // negation where (index & mask) == pattern: the pattern's slot
+        // bits select members, the others are tested per lane / tile
+        in.kind = T_NEG;
+        int tq = -1;  // the bit that plays the target: a slot if possible
+        for (int b = 0; b < n && tq < 0; ++b)
+            if (((mask >> b) & 1) && local_of[b] >= 0 && slot_of(local_of[b]) >= 0) tq = b;
+        for (int b = 0; b < n && tq < 0; ++b)
+            if (((mask >> b) & 1) && local_of[b] >= 0) tq = b;
+        for (int b = 0; b < n && tq < 0; ++b)
+            if ((mask >> b) & 1) tq = b;
+        uint32_t cvals = 0;
+        for (int b = 0; b < n; ++b) {
+            if (!((mask >> b) & 1) || b == tq) continue;
+            const uint64_t v = (pattern >> b) & 1;
+            const int lb = local_of[b];
+            if (lb < 0) { in.omask |= 1ull << b; in.oval |= v << b; }
+            else if (slot_of(lb) >= 0) { in.cslots |= 1u << slot_of(lb); cvals |= static_cast<uint32_t>(v) << slot_of(lb); }
+            else { in.lmask |= 1u << lb; in.lval |= static_cast<uint32_t>(v) << lb; }
+        }
+        const uint32_t tv = static_cast<uint32_t>((pattern >> tq) & 1);
+        const int tl = local_of[tq];
+        in.slot = tl >= 0 ? slot_of(tl) : -1;
+        uint16_t sel = 0;
+        for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m) {
+            if ((m & in.cslots) != cvals) continue;
+            if (in.slot >= 0 && ((m >> in.slot) & 1u) != tv) continue;
+            sel |= static_cast<uint16_t>(1u << m);
+        }
+        if (in.slot >= 0) {
+            in.mm1 = sel;
+        } else {
+            if (tl < 0) in.obitpos = static_cast<int8_t>(tq);
+            else in.dlocal = tl;
+            (tv ? in.mm1 : in.mm0) = sel;
+        }
+        in.cslots = 0;  // folded into the member sets
+        return in;
     };
     auto close = [&]() {
         for (int b = 0; nslots < S && b < k; ++b)
@@ -559,43 +618,14 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
             const PendingGate* pg = mu->pg;
             const bfly_op& op = *pg->op;
             if (op.kind == BFLY_OP_PHASE_FLIP) {
-                // negation where (index & mask) == pattern: the pattern's slot
-                // bits select members, the others are tested per lane / tile
-                in.kind = T_NEG;
-                int tq = -1;  // the bit that plays the target: a slot if possible
-                for (int b = 0; b < n && tq < 0; ++b)
-                    if (((op.aux >> b) & 1) && local_of[b] >= 0 && slot_of(local_of[b]) >= 0) tq = b;
-                for (int b = 0; b < n && tq < 0; ++b)
-                    if (((op.aux >> b) & 1) && local_of[b] >= 0) tq = b;
-                for (int b = 0; b < n && tq < 0; ++b)
-                    if ((op.aux >> b) & 1) tq = b;
-                uint32_t cvals = 0;
-                for (int b = 0; b < n; ++b) {
-                    if (!((op.aux >> b) & 1) || b == tq) continue;
-                    const uint64_t v = (op.controls >> b) & 1;
-                    const int lb = local_of[b];
-                    if (lb < 0) { in.omask |= 1ull << b; in.oval |= v << b; }
-                    else if (slot_of(lb) >= 0) { in.cslots |= 1u << slot_of(lb); cvals |= static_cast<uint32_t>(v) << slot_of(lb); }
-                    else { in.lmask |= 1u << lb; in.lval |= static_cast<uint32_t>(v) << lb; }
-                }
-                const uint32_t tv = static_cast<uint32_t>((op.controls >> tq) & 1);
-                const int tl = local_of[tq];
-                in.slot = tl >= 0 ? slot_of(tl) : -1;
-                uint16_t sel = 0;
-                for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m) {
-                    if ((m & in.cslots) != cvals) continue;
-                    if (in.slot >= 0 && ((m >> in.slot) & 1u) != tv) continue;
-                    sel |= static_cast<uint16_t>(1u << m);
-                }
-                if (in.slot >= 0) {
-                    in.mm1 = sel;
-                } else {
-                    if (tl < 0) in.obitpos = static_cast<int8_t>(tq);
-                    else in.dlocal = tl;
-                    (tv ? in.mm1 : in.mm0) = sel;
-                }
-                in.cslots = 0;  // folded into the member sets
-                push(in, sw.first);
+                push(flip_instr(op.aux, op.controls), sw.first);
+                continue;
+            }
+            if (pg->c.kind == I_NEG1) {
+                // a sign gate is the phase flip of its controls and target all 1
+                // (symmetric: the flip's own choice of target prefers a slot)
+                const uint64_t qm = op.controls | (1ull << op.target);
+                push(flip_instr(qm, qm), sw.first);
                 continue;
             }
             const int ik = pg->c.kind;
@@ -669,16 +699,43 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
         cur.clear();
         nslots = 0;
     };
+    // Sweep assignment first, emission second.  A diagonal gate (controlled
+    // phase, sign gate, phase flip) commutes with everything but the
+    // non-diagonal actions on its own qubits, so it may join any sweep from
+    // the one holding the last such action (after it) to the current one: it
+    // joins the one where most of its qubits are slots (the latest on a tie),
+    // which turns per-lane tests into member sets and lets sign flips merge.
+    struct SweepBuild {
+        int slots[kSlots];
+        int nslots;
+        std::vector<const Micro*> ops;
+    };
+    std::vector<SweepBuild> built;
+    built.push_back(SweepBuild{{0, 0, 0, 0}, 0, {}});
+    std::vector<int> last_nd(static_cast<size_t>(n), 0);  // sweep of the last non-diagonal action per qubit
     auto need_slot = [&](int lt) {
-        if (lt < 0 || slot_of(lt) >= 0) return;
-        if (nslots == S) close();
-        slots[nslots++] = lt;
+        SweepBuild& b = built.back();
+        for (int s2 = 0; s2 < b.nslots; ++s2)
+            if (b.slots[s2] == lt) return;
+        if (lt < 0) return;
+        if (b.nslots == S) built.push_back(SweepBuild{{0, 0, 0, 0}, 0, {}});
+        SweepBuild& c = built.back();
+        c.slots[c.nslots++] = lt;
+    };
+    auto is_slot_in = [&](const SweepBuild& b, int q) {
+        const int lt = local_of[q];
+        if (lt < 0) return false;
+        for (int s2 = 0; s2 < b.nslots; ++s2)
+            if (b.slots[s2] == lt) return true;
+        return false;
     };
     for (size_t i = 0; i < micro.size(); ++i) {
         const Micro& mu = micro[i];
-        if (mu.kind == Micro::ROT || (mu.kind == Micro::GATE && !mu.pg->c.diag)) {
+        const bool nondiag = mu.kind == Micro::ROT || (mu.kind == Micro::GATE && !mu.pg->c.diag);
+        if (nondiag) {
             need_slot(local_of[mu.q]);
-        } else if (mu.kind == Micro::PH && local_of[mu.q] >= 0 && slot_of(local_of[mu.q]) < 0) {
+            last_nd[static_cast<size_t>(mu.q)] = static_cast<int>(built.size()) - 1;
+        } else if (mu.kind == Micro::PH && local_of[mu.q] >= 0) {
             // a phase anchored before a rotation of its qubit opens the
             // rotation's slot now; otherwise it only takes a free slot
             bool before_rot = false;
@@ -686,11 +743,32 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
                 if (blocks_phase(micro[j], mu.q)) before_rot = true;
                 else if (micro[j].kind != Micro::PH) break;
             }
-            if (before_rot || nslots < S) need_slot(local_of[mu.q]);
+            if (before_rot || built.back().nslots < S) need_slot(local_of[mu.q]);
+        } else if (mu.kind == Micro::GATE && mu.pg->c.diag) {
+            const bfly_op& op = *mu.pg->op;
+            const uint64_t qm = op.kind == BFLY_OP_PHASE_FLIP ? op.aux : (op.controls | (1ull << op.target));
+            int lo = 0;
+            for (int q = 0; q < n; ++q)
+                if ((qm >> q) & 1) lo = std::max(lo, last_nd[static_cast<size_t>(q)]);
+            int best = static_cast<int>(built.size()) - 1, best_score = -1;
+            for (int b2 = static_cast<int>(built.size()) - 1; b2 >= lo; --b2) {
+                int score = 0;
+                for (int q = 0; q < n; ++q)
+                    if (((qm >> q) & 1) && is_slot_in(built[static_cast<size_t>(b2)], q)) ++score;
+                if (score > best_score) { best_score = score; best = b2; }
+            }
+            built[static_cast<size_t>(best)].ops.push_back(&mu);
+            continue;
         }
-        cur.push_back(&mu);
+        built.back().ops.push_back(&mu);
     }
-    close();
+    for (const SweepBuild& b : built) {
+        if (b.ops.empty() && b.nslots == 0) continue;
+        for (int s2 = 0; s2 < kSlots; ++s2) slots[s2] = b.slots[s2];
+        nslots = b.nslots;
+        cur = b.ops;
+        close();
+    }
     tp.nsweeps = static_cast<int32_t>(plan.sweeps.size()) - tp.first_sweep;
     st.gates = static_cast<int32_t>(gates.size());
     plan.steps.push_back(st);
@@ -1341,6 +1419,7 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
     size_t head = 0;  // remaining[head..] are still to schedule (compacted each pass)
     std::vector<uint32_t> rest;
     std::vector<PendingGate> placed;
+    PhaseCarry carry;  // phases waiting for a later tile pass
     constexpr size_t kScanWindow = 8192;  // gates examined past the first deferral
     while (head < remaining.size()) {
         const bfly_op& front = ops[remaining[head]];
@@ -1390,21 +1469,49 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
         }
         bool flips_only = true;
         for (const PendingGate& pg : placed) flips_only = flips_only && pg.op->kind == BFLY_OP_PHASE_FLIP;
-        if (flips_only) {
-            // a pass of phase flips alone: each touches 2^(n - |mask|) amplitudes
-            for (const PendingGate& pg : placed) emit_standalone(*pg.op);
-        } else {
-            // complete the tile with the lowest unused qubits (any k-subset works)
-            for (int b = 0; popc(tile) < k && b < n; ++b) tile |= 1ull << b;
-            emit_tile_pass(plan, n, k, S, tile, placed, !generic);
-        }
         // compact: deferred ops keep their order, followed by the unscanned tail
         std::vector<uint32_t> next;
         next.reserve(rest.size() + (remaining.size() - idx));
         next.insert(next.end(), rest.begin(), rest.end());
         next.insert(next.end(), remaining.begin() + static_cast<long>(idx), remaining.end());
+        // A trailing phase on q may wait for a later pass when the first
+        // remaining op that acts on q non-diagonally is a gate with target q:
+        // that gate stays ahead of every later non-diagonal action on q, and
+        // the phase commutes with everything before it (diagonal gates, phase
+        // flips, gates controlled by q).  A measurement, diffuser or other
+        // stand-alone op on q first, or none at all, takes the phase now.
+        uint64_t carry_ok = 0, decided = 0;
+        for (const uint32_t oi : next) {
+            const bfly_op& op = ops[oi];
+            if (!placeable(op)) {
+                decided |= all;   // conservatively every qubit
+                break;
+            }
+            if (op.kind == BFLY_OP_GATE && !cls[oi].diag) {
+                const uint64_t t = 1ull << op.target;
+                if (!(decided & t)) carry_ok |= t;
+                decided |= t;
+                if ((decided & all) == all) break;
+            }
+        }
+        if (flips_only) {
+            // a pass of phase flips alone: each touches 2^(n - |mask|) amplitudes;
+            // the phases waiting for later passes commute with them
+            for (const PendingGate& pg : placed) emit_standalone(*pg.op);
+        } else {
+            // complete the tile with the lowest unused qubits (any k-subset works)
+            for (int b = 0; popc(tile) < k && b < n; ++b) tile |= 1ull << b;
+            PhaseCarry carry_next;
+            emit_tile_pass(plan, n, k, S, tile, placed, !generic, carry, generic ? nullptr : &carry_next,
+                           [&](int q) { return ((carry_ok >> q) & 1) != 0; });
+            carry.swap(carry_next);
+        }
         remaining.swap(next);
         head = 0;
+    }
+    if (!carry.empty()) {
+        err = "internal planner error: a carried phase was not applied";
+        return BFLY_EINVAL;
     }
     return BFLY_OK;
 }

@@ -5,7 +5,9 @@ import numpy as np
 import pytest
 
 from code_mojo_amd import _lib, algos, lower_circuit
+from code_mojo_amd import gates as G
 from code_mojo_amd import workloads as W
+from code_mojo_amd.lowering import OpList
 from code_mojo_amd.state import Plan
 from oracle import oracle as O
 
@@ -127,3 +129,56 @@ def test_invalid_ops_rejected():
         Plan(n, np.float64, bad, ud, _lib.opts())
     with pytest.raises(_lib.BflyError, match="execution mode"):
         Plan(n, np.float64, ops, ud, _lib.opts(mode=7))
+
+
+def _u3_layer(ops, rng, n, qubits=None):
+    for q in (range(n) if qubits is None else qubits):
+        g = G.Custom_Gate(G.U3(*(float(x) for x in rng.uniform(0.1, 3.0, size=3))))
+        ops.gate(q, [], g.kind, g.gate, 0.0)
+
+
+def test_trailing_phases_ride_on_later_rotations():
+    """A one-qubit unitary factors into phase, real rotation, phase; the last
+    phase of each qubit in a tile pass waits for the qubit's next rotation in
+    a later pass (its pre-phase) instead of becoming an instruction of its
+    own.  Counted on a 14-qubit layered circuit with 2^6 tiles (many passes per
+    layer), and replayed against the oracle."""
+    n = 14
+    ops, ud = lower_circuit(W.random_layered(n, 8, seed=4))
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=6))
+    text = plan.dump()
+    rot = text.count("instr kind=3 ") + text.count("instr kind=6 ")
+    ph = text.count("instr kind=4 ") + text.count("instr kind=2 ")
+    assert plan.stats()["passes"] > 8
+    assert ph * 4 <= rot, (ph, rot)
+    psi = W.normal_state(n, 2).to_complex()
+    assert np.abs(run_plan(text, psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
+
+
+@pytest.mark.parametrize("barrier", ["bitrev", "diffuse", "measure", "end"])
+def test_waiting_phases_are_applied_before_a_barrier(barrier):
+    """Phases that wait for a later pass are applied before any op that does
+    not commute with them (a bit reversal, a diffuser, a measurement) and by
+    the end of the circuit, even when the qubit has more rotations after it."""
+    n = 10
+    rng = np.random.default_rng(11)
+    ops = OpList(n)
+    for _ in range(3):
+        _u3_layer(ops, rng, n)
+        for a in range(0, n - 1, 2):
+            ops.gate(a + 1, [a], G.GateKind.names.index("Z"), G.GateInfo.of(G.GateKind.names.index("Z"), None).gate, 0.0)
+    if barrier == "bitrev":
+        ops.bitrev((1 << n) - 1)
+    elif barrier == "diffuse":
+        ops.diffuse((1 << n) - 1, 0)
+    elif barrier == "measure":
+        ops.measure([0, 5], [None, None], seed=12345)
+    if barrier != "end":
+        _u3_layer(ops, rng, n, qubits=[1, 4, 7])
+    ops, ud = ops.finish()
+    psi = W.normal_state(n, 6).to_complex()
+    ref = oracle_run(n, ops, ud, psi)
+    for tb in (4, 6):
+        plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
+        got = run_plan(plan.dump(), psi, ud)
+        assert np.abs(got - ref).max() < 1e-12, (barrier, tb)
