@@ -48,6 +48,8 @@ enum TileKind : int32_t {
     T_ROTS = 6,    // (3-slot sweeps) T_ROT on every slot s in the mask `cslots`, in slot order (they
                    // commute): m[3s] = t (negative, or -0.0, for form 1),
                    // m[3s + 1 .. 3s + 2] = pre-phase
+    T_ROT3 = 7,    // (3-slot sweeps) T_ROTS on all three slots, every rotation in form 0
+                   // (straight-line code: no per-slot or per-form branches)
 };
 
 // Amplitudes per thread in a tile sweep: 2^S, spanned by S local bits (the

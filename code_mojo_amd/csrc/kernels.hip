@@ -410,9 +410,19 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
                 const uint32_t cslots = ubyte(in.w0, 2);
                 const int dlocal = sbyte(in.w0, 3);
                 const T* g = in.g;
-                if (kind == T_ROT) {
-                    const bool f1 = g[1] != T(0);
-                    BFLY_SLOT_SWITCH(S, slot, (slot_rot<T, G, SL>(ar, ai, g[0], f1, g[2], g[3])))
+                if (S == 3 && kind == T_ROT3) {
+                    // every slot, form 0: straight-line, so the register
+                    // groups need no copies at control-flow joins
+                    const T* h = sizeof(T) == 8 ? reinterpret_cast<const T*>(ip[q].m) : reinterpret_cast<const T*>(ip[q].mf);
+                    slot_rot<T, G, 0>(ar, ai, h[0], false, h[1], h[2]);
+                    slot_rot<T, G, 1>(ar, ai, h[3], false, h[4], h[5]);
+                    if constexpr (S > 2) slot_rot<T, G, 2>(ar, ai, h[6], false, h[7], h[8]);
+                } else if (kind == T_ROT) {
+                    if (g[1] == T(0)) {
+                        BFLY_SLOT_SWITCH(S, slot, (slot_rot<T, G, SL>(ar, ai, g[0], false, g[2], g[3])))
+                    } else {
+                        BFLY_SLOT_SWITCH(S, slot, (slot_rot<T, G, SL>(ar, ai, g[0], true, g[2], g[3])))
+                    }
                 } else if (S == 3 && kind == T_ROTS) {
                     const T* h = sizeof(T) == 8 ? reinterpret_cast<const T*>(ip[q].m) : reinterpret_cast<const T*>(ip[q].mf);
                     if (cslots & 1u) slot_rot<T, G, 0>(ar, ai, fabs_t(h[0]), signbit_t(h[0]), h[1], h[2]);

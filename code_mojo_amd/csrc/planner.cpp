@@ -209,9 +209,14 @@ bool is_minus_one(Cx z) { return z.r == -1.0 && z.i == 0.0; }
 
 // Factors a 2x2 unitary as M = sigma * PH(ephi) * R * PH(elam), where
 // PH(z) = diag(1, z) and R is the real rotation [[c, -s], [s, c]] written as
-// c * [[1, -t], [t, 1]] (form 0, t = s/c <= 1) or s * [[t, -1], [1, t]]
-// (form 1, t = c/s < 1); sigma collects e^{i alpha} and c or s.  Returns
-// false when M is not unitary to 1e-12 (then the generic pair product runs).
+// c * [[1, -t], [t, 1]] (form 0, t = s/c <= 2^20) or s * [[t, -1], [1, t]]
+// (form 1, t = c/s < 2^-20: rotations within about 1e-6 of a quarter turn,
+// such as X and Y); sigma collects e^{i alpha} and c or s.  A large t costs
+// no accuracy: each output is one FMA rounded relative to its own size, and
+// the pass's scale product is applied before it falls below 2^-16.  Keeping
+// almost every rotation in form 0 lets the kernels run them without
+// branches.  Returns false when M is not unitary to 1e-12 (then the generic
+// pair product runs).
 bool factor_unitary(const double* m, Cx& sigma, Cx& ephi, Cx& elam, int& form, double& t) {
     const Cx m00{m[0], m[1]}, m01{m[2], m[3]}, m10{m[4], m[5]}, m11{m[6], m[7]};
     const double c = std::hypot(m00.r, m00.i), s = std::hypot(m10.r, m10.i);
@@ -227,7 +232,7 @@ bool factor_unitary(const double* m, Cx& sigma, Cx& ephi, Cx& elam, int& form, d
         elam = cdiv_unit({-m01.r / s, -m01.i / s}, ea);
     }
     double scale;
-    if (c >= s) { form = 0; t = s / c; scale = c; }
+    if (c >= s * 0x1p-20) { form = 0; t = s / c; scale = c; }
     else { form = 1; t = c / s; scale = s; }
     sigma = {ea.r * scale, ea.i * scale};
     // rebuild ea * [[c, -elam s], [ephi s, ephi elam c]] and compare
@@ -674,7 +679,9 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
                     continue;
                 }
                 TileInstr r{};
-                r.kind = T_ROTS;
+                bool all_form0 = true;
+                for (size_t q = i; q < j; ++q) all_form0 = all_form0 && plan.instrs[q].m[1] == 0.0;
+                r.kind = used_slots == 7u && all_form0 ? T_ROT3 : T_ROTS;
                 r.slot = -1;
                 r.dlocal = -1;
                 r.obitpos = -1;

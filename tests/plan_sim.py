@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 I_GEN, I_H, I_X, I_RY, I_DIAG, I_PHASE1, I_NEG1 = range(7)
-T_PAIR, T_NEG, T_DIAG, T_ROT, T_PH, T_SCALE, T_ROTS = range(7)
+T_PAIR, T_NEG, T_DIAG, T_ROT, T_PH, T_SCALE, T_ROTS, T_ROT3 = range(8)
 SQ_HALF = 0.7071067811865476
 
 
@@ -168,7 +168,10 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
                     for si in range(S):
                         member |= ((j >> sw["s"][si]) & 1) << si
                     m = ins["m"]
-                    if ins["kind"] == T_ROTS:
+                    if ins["kind"] == T_ROT3:
+                        assert S == 3 and ins["cslots"] == 7
+                        assert all(math.copysign(1.0, m[3 * sl]) > 0 for sl in range(3))
+                    if ins["kind"] in (T_ROTS, T_ROT3):
                         for sl in range(S):
                             if not (ins["cslots"] >> sl) & 1:
                                 continue
