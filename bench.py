@@ -82,15 +82,19 @@ def bench_batching(args) -> int:
     [10, 22]: half Grover search (one seeded mark), half 20-layer random
     U3 + CZ brick circuits.  Timed: one hipGraph launch running every round
     of the batch (states reset between timed runs), against the same
-    circuits executed one after another.  Every 8th final state is compared
-    with a complex128 run of its circuit."""
-    from code_mojo_amd import serve
+    circuits executed one after another.  Every circuit compiles with the
+    options the autotuner picked for its class (family, qubits; tuned on the
+    GPU before anything is timed; --no-autotune for the defaults).  Every 8th
+    final state is compared with a complex128 run of its circuit."""
+    from code_mojo_amd import autotune, serve
 
     count = args.circuits or 4096
     t0 = time.perf_counter()
     jobs = serve.batching_jobs(count, args.seed)
     log(f"[bench] {count} batching circuits built in {time.perf_counter() - t0:.1f}s")
-    res = serve.bench_batching(jobs, graph=True, repeats=max(args.steps, 1), check=True, log=log, check_every=8)
+    tuner = None if args.no_autotune else autotune.Autotuner()
+    res = serve.bench_batching(jobs, graph=True, repeats=max(args.steps, 1), check=True, log=log, check_every=8,
+                               tuner=tuner)
     secs = res["batch_s"]
     out = {
         "metric": "circuits/s, batched persistent launches (complex64, 10-22 qubits)",
@@ -100,7 +104,8 @@ def bench_batching(args) -> int:
         "data": "synthetic (seeded Grover and random layered circuits)",
         "config": {"workload": "batching_grover_layered", "circuits": count, "qubits": "10-22 log-uniform",
                    "seed": args.seed, "graph": True, "tile_passes": res["passes"], "rounds": res["rounds"],
-                   "launches": res["launches"]},
+                   "launches": res["launches"], "autotuned": tuner is not None},
+        "tuned": res.get("tuned"), "tune_s": round(res["tune_s"], 2) if "tune_s" in res else None,
         "one_circuit_per_launch_circuits_per_s": round(res["sequential_circuits_per_s"], 2),
         "speedup_vs_one_circuit_per_launch": round(res["speedup"], 3),
         "max_l2_vs_c128": res.get("max_l2_vs_c128"), "min_fidelity_vs_c128": res.get("min_fidelity_vs_c128"),
@@ -315,6 +320,8 @@ def main() -> int:
     ap.add_argument("--cpu-layers", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batching: compile every circuit with the default options instead of the tuned ones")
     ap.add_argument("--workload", choices=["layered", "batching", "serving", "precision"], default="layered",
                     help="layered: the headline state-GB/s workload (default); batching: many independent "
                          "circuits in batched persistent launches; serving: a Poisson request stream "

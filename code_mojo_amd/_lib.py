@@ -19,6 +19,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libbfly_hip.so"
 BFLY_OK, BFLY_EINVAL, BFLY_ENOMEM, BFLY_EHIP, BFLY_EUNSUPPORTED, BFLY_ENUMERIC = 0, -1, -2, -3, -4, -5
 C128, C64 = 0, 1
 MODE_FUSED, MODE_UNFUSED, MODE_BLOCK, MODE_AUTO = 0, 1, 2, 3
+TUNE_ALL = (1 << MODE_FUSED) | (1 << MODE_BLOCK) | (1 << MODE_AUTO)
 FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS = 1, 2, 4
 BATCH_GRAPH = 1
 
@@ -116,6 +117,11 @@ _SIGS = {
                                 ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_double)]),
     "bfly_sample": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.c_int32, ct.c_int64, ct.c_uint64,
                                ct.POINTER(ct.c_uint64)]),
+    "bfly_autotune_candidates": (ct.c_int, [ct.c_int32, ct.c_int32, ct.POINTER(ExecOpts), ct.c_int32,
+                                            ct.POINTER(ExecOpts), ct.c_int32, ct.POINTER(ct.c_int32)]),
+    "bfly_autotune": (ct.c_int, [ct.c_int32, ct.c_int32, ct.c_void_p, ct.c_uint64, ct.c_void_p, ct.c_uint64,
+                                 ct.POINTER(ExecOpts), ct.c_int32, ct.c_int32, ct.POINTER(ExecOpts),
+                                 ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "bfly_profile_reset": (ct.c_int, []),
     "bfly_profile_get": (ct.c_int, [ct.c_int32, ct.POINTER(ct.c_char_p), ct.POINTER(ct.c_int64),
                                     ct.POINTER(ct.c_double)]),

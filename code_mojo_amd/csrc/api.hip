@@ -581,6 +581,120 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
     return BFLY_OK;
 }
 
+int bfly_autotune_candidates(int32_t num_qubits, int32_t dtype, const bfly_exec_opts* base, int32_t modes,
+                             bfly_exec_opts* out, int32_t cap, int32_t* count) {
+    if (!base || !count || (cap > 0 && !out)) return fail(BFLY_EINVAL, "null argument");
+    if (num_qubits < 1 || num_qubits > kMaxQubits) return fail(BFLY_EINVAL, "qubit count out of range");
+    if (dtype != BFLY_C128 && dtype != BFLY_C64) return fail(BFLY_EINVAL, "unknown dtype");
+    std::vector<bfly_exec_opts> c;
+    auto same = [](const bfly_exec_opts& a, const bfly_exec_opts& b) {
+        return a.mode == b.mode && a.tile_bits == b.tile_bits && a.max_gates_per_pass == b.max_gates_per_pass &&
+               a.low_bits == b.low_bits && a.block_qubits == b.block_qubits && a.renorm_every == b.renorm_every &&
+               a.threads == b.threads && a.flags == b.flags;
+    };
+    auto add = [&](bfly_exec_opts o) {
+        // a tile wider than the state compiles like the whole state
+        const int d = default_tile_bits(dtype);
+        if (o.tile_bits == d || (o.tile_bits >= num_qubits && d >= num_qubits)) o.tile_bits = 0;
+        for (const auto& x : c)
+            if (same(x, o)) return;
+        c.push_back(o);
+    };
+    add(*base);
+    if (modes & (1 << BFLY_MODE_FUSED)) {
+        const int d = default_tile_bits(dtype);
+        for (int k = d - 1; k <= d + 1; ++k) {
+            if (k < kMinTileBits || k > kMaxTileBits || (k > num_qubits && k != d)) continue;
+            for (int cap2 : {0, 2 * k}) {
+                bfly_exec_opts o = *base;
+                o.mode = BFLY_MODE_FUSED;
+                o.tile_bits = k;
+                o.max_gates_per_pass = cap2;
+                add(o);
+            }
+        }
+    }
+    if ((modes & (1 << BFLY_MODE_BLOCK)) && num_qubits >= 9) {
+        bfly_exec_opts o = *base;
+        o.mode = BFLY_MODE_BLOCK;
+        o.tile_bits = 0;
+        o.max_gates_per_pass = 0;
+        add(o);
+    }
+    if (modes & (1 << BFLY_MODE_AUTO)) {
+        bfly_exec_opts o = *base;
+        o.mode = BFLY_MODE_AUTO;
+        o.tile_bits = 0;
+        o.max_gates_per_pass = 0;
+        add(o);
+    }
+    *count = static_cast<int32_t>(c.size());
+    for (int32_t i = 0; i < cap && i < *count; ++i) out[i] = c[static_cast<size_t>(i)];
+    return BFLY_OK;
+}
+
+int bfly_autotune(int32_t num_qubits, int32_t dtype, const bfly_op* ops, uint64_t nops, const double* udata,
+                  uint64_t udata_len, const bfly_exec_opts* base, int32_t modes, int32_t repeats,
+                  bfly_exec_opts* best, double* best_ms, double* base_ms) {
+    if (!base || !best || !best_ms || !base_ms) return fail(BFLY_EINVAL, "null argument");
+    if (repeats < 1 || repeats > 100) return fail(BFLY_EINVAL, "repeats must be in 1..100");
+    int32_t count = 0;
+    int rc = bfly_autotune_candidates(num_qubits, dtype, base, modes, nullptr, 0, &count);
+    if (rc) return rc;
+    std::vector<bfly_exec_opts> cand(static_cast<size_t>(count));
+    if ((rc = bfly_autotune_candidates(num_qubits, dtype, base, modes, cand.data(), count, &count))) return rc;
+    bfly_state* s = nullptr;
+    if ((rc = bfly_state_create(&s, num_qubits, dtype))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        bfly_state_destroy(s);
+        return hip_fail(he, "hipEventCreate");
+    }
+    double best_t = -1.0, base_t = -1.0;
+    int found = -1;
+    for (int32_t i = 0; i < count && rc == BFLY_OK; ++i) {
+        bfly_exec_opts o = cand[static_cast<size_t>(i)];
+        o.flags &= ~(BFLY_FLAG_SYNC | BFLY_FLAG_PROFILE);
+        bfly_plan* p = nullptr;
+        const int prc = bfly_plan_create(&p, num_qubits, dtype, ops, nops, udata, udata_len, &o);
+        if (prc != BFLY_OK) {
+            if (i == 0) { rc = prc; break; }   // the base options must compile: keep their error
+            continue;
+        }
+        std::vector<double> t;
+        for (int r = 0; r <= repeats && rc == BFLY_OK; ++r) {
+            if ((rc = bfly_state_reset(s))) break;
+            he = hipEventRecord(e0, s->stream);
+            if (he == hipSuccess && (rc = bfly_plan_execute(p, s)) != BFLY_OK) break;
+            if (he == hipSuccess) he = hipEventRecord(e1, s->stream);
+            if (he == hipSuccess) he = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+            if (he != hipSuccess) { rc = hip_fail(he, "autotune timing"); break; }
+            if (r > 0) t.push_back(static_cast<double>(ms));   // run 0 warms the kernels
+        }
+        (void)hipStreamSynchronize(s->stream);   // no queued kernel may outlive the plan's records
+        bfly_plan_destroy(p);
+        if (rc != BFLY_OK) break;
+        std::sort(t.begin(), t.end());
+        const double med = t[t.size() / 2];
+        if (i == 0) base_t = med;
+        if (found < 0 || med < best_t) { best_t = med; found = i; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const std::string keep = rc ? g_last_error : std::string();
+    bfly_state_destroy(s);
+    if (rc) return fail(rc, keep);
+    *best = cand[static_cast<size_t>(found)];
+    *best_ms = best_t;
+    *base_ms = base_t;
+    return BFLY_OK;
+}
+
 int bfly_apply(bfly_state* s, const bfly_op* ops, uint64_t nops, const double* udata, uint64_t udata_len,
                const bfly_exec_opts* opts) {
     int rc = check_state(s);

@@ -60,7 +60,10 @@ class Engine:
     """Python handle of a bfly_engine."""
 
     def __init__(self, budget_bytes: int = 0, max_inflight_rounds: int = 0, large_qubits: int = 0,
-                 max_round_requests: int = 0, planner_threads: int = 0):
+                 max_round_requests: int = 0, planner_threads: int = 0, tuner=None):
+        """`tuner` (an autotune.Autotuner): requests submitted without options
+        compile with the options tuned for their class instead of AUTO."""
+        self.tuner = tuner
         lib = _lib.load()
         opts = _lib.EngineOpts(budget_bytes, max_inflight_rounds, large_qubits, max_round_requests, 0,
                                planner_threads, 0)
@@ -82,7 +85,9 @@ class Engine:
         if job.init_kind == _lib.INIT_HOST:
             rq.init_re = job.init.re.ctypes.data
             rq.init_im = job.init.im.ctypes.data
-        rq.exec = exec_opts or default_exec_opts()
+        if exec_opts is None:
+            exec_opts = self.tuner.opts_for(job) if self.tuner is not None else default_exec_opts()
+        rq.exec = exec_opts
         rid = ct.c_uint64()
         check(_lib.load().bfly_engine_submit(self._h, ct.byref(rq), ct.byref(rid)))
         return rid.value
@@ -356,7 +361,8 @@ def fidelity(a: QuantumState, b: QuantumState) -> Tuple[float, float]:
 
 
 def bench_batching(jobs: List[Job], graph: bool = True, repeats: int = 1, check: bool = True,
-                   exec_opts: Optional[_lib.ExecOpts] = None, log=None, check_every: int = 1) -> dict:
+                   exec_opts: Optional[_lib.ExecOpts] = None, log=None, check_every: int = 1,
+                   tuner=None) -> dict:
     """Run `jobs` (i) as one batch: every round's tile passes of all circuits
     in one persistent launch per tile size, all rounds captured in one
     hipGraph, and (ii) one circuit after another, each circuit's launches
@@ -364,12 +370,19 @@ def bench_batching(jobs: List[Job], graph: bool = True, repeats: int = 1, check:
     from their recipe before every timed run; plans are compiled outside the
     timed region for both.  With `check`, every final state is compared with
     a complex128 run of the same circuit (every `check_every`-th circuit
-    against its complex128 run; all of them against the one-by-one run)."""
+    against its complex128 run; all of them against the one-by-one run).
+    With `tuner` (an autotune.Autotuner) each circuit compiles with the
+    options tuned for its class (tile shape, gates per pass, schedule mode),
+    tuned before anything is timed; both runs use the same plans."""
     eo = exec_opts or _lib.opts()
     say = log or (lambda m: None)
     t0 = time.perf_counter()
+    job_opts = [tuner.opts_for(j) for j in jobs] if tuner is not None else [eo] * len(jobs)
+    if tuner is not None:
+        say(f"batching: tuned {len(tuner.cache)} circuit classes in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
     states = [_init_state(j) for j in jobs]
-    plans = [Plan(j.n, j.dtype, j.ops, j.udata, eo) for j in jobs]
+    plans = [Plan(j.n, j.dtype, j.ops, j.udata, o) for j, o in zip(jobs, job_opts)]
     say(f"batching: {len(jobs)} states + plans in {time.perf_counter() - t0:.1f}s")
     passes = sum(p.stats()["launches"] for p in plans)
     algo_bytes = sum(p.stats()["launches"] * 2 * (1 << j.n) * np.dtype(j.dtype).itemsize * 2
@@ -420,11 +433,14 @@ def bench_batching(jobs: List[Job], graph: bool = True, repeats: int = 1, check:
            "circuits_per_s": len(jobs) / min(t_batch), "sequential_circuits_per_s": len(jobs) / t_seq,
            "speedup": t_seq / min(t_batch), "passes": passes, "algorithmic_bytes": algo_bytes,
            "rounds": bstats["rounds"], "launches": bstats["launches"], "graph": graph}
+    if tuner is not None:
+        out["tuned"] = tuner.summary()
+        out["tune_s"] = tuner.seconds()
     if check:
         l2s, infid = [], []
         for j, r in list(zip(jobs, results))[::check_every]:
             g = GpuState(j.n, np.float64)
-            p = Plan(j.n, np.float64, j.ops, j.udata, eo)
+            p = Plan(j.n, np.float64, j.ops, j.udata, _lib.opts())
             _reinit(g, j)
             p.execute(g)
             ref = g.download()

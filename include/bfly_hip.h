@@ -166,6 +166,27 @@ int bfly_plan_stats(const bfly_plan* p, int64_t* passes, int64_t* launches, int6
 int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed);
 /* Run a compiled plan on a state (asynchronous unless BFLY_FLAG_SYNC was set). */
 int bfly_plan_execute(bfly_plan* p, bfly_state* s);
+/* ---- autotuner: tile shape and fusion width ------------------------------
+   No counterpart in the reference (its strategies take fixed layouts:
+   compute_grid_layout, executors.mojo:213-228).  Candidate schedules are
+   the base options plus, for each mode whose bit (1 << BFLY_MODE_*) is set in
+   `modes`: FUSED at the default tile size and one size either side, each
+   with no cap and with a cap of 2k gates per pass (k = tile bits); BLOCK;
+   AUTO.  Options that compile to an identical schedule are kept once. */
+#define BFLY_TUNE_ALL ((1 << BFLY_MODE_FUSED) | (1 << BFLY_MODE_BLOCK) | (1 << BFLY_MODE_AUTO))
+/* Host only: the candidate list for an n-qubit circuit (at most `cap`
+   written; *count = how many there are). */
+int bfly_autotune_candidates(int32_t num_qubits, int32_t dtype, const bfly_exec_opts* base, int32_t modes,
+                             bfly_exec_opts* out, int32_t cap, int32_t* count);
+/* Times every candidate's schedule of this op list on a scratch state of the
+   circuit's size (allocated and freed here; |0..0> start, one untimed run,
+   then the median of `repeats` runs bracketed by HIP events on the state's
+   stream) and returns the fastest options in *best, its time in *best_ms and
+   the base options' time in *base_ms.  Candidates that fail to compile are
+   skipped. */
+int bfly_autotune(int32_t num_qubits, int32_t dtype, const bfly_op* ops, uint64_t nops,
+                  const double* udata, uint64_t udata_len, const bfly_exec_opts* base, int32_t modes,
+                  int32_t repeats, bfly_exec_opts* best, double* best_ms, double* base_ms);
 /* Plan + execute + destroy in one call. */
 int bfly_apply(bfly_state* s, const bfly_op* ops, uint64_t nops,
                const double* udata, uint64_t udata_len, const bfly_exec_opts* opts);
