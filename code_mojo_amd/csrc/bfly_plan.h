@@ -87,13 +87,25 @@ static_assert(sizeof(TileInstr) == 240, "TileInstr layout");
 // A sweep: every group of 16 amplitudes spanned by the kSlots local bits
 // `s` is loaded from LDS into registers, all the sweep's gates run in
 // registers, and the group is written back.  32 bytes.
+// XOR swizzle of a tile's LDS index (elements of `elem_bytes` bytes): bits
+// 4.. (8-byte elements) or 5.. (4-byte elements) are folded onto the low bits,
+// which spreads the strided accesses of a sweep over the LDS banks.  Linear
+// over GF(2): swz(a | b) = swz(a) ^ swz(b) for disjoint a, b.
+constexpr uint32_t lds_swizzle(uint32_t j, int elem_bytes) {
+    return elem_bytes == 8 ? j ^ ((j >> 4) & 15u) : j ^ ((j >> 5) & 31u);
+}
+
 struct alignas(16) TileSweep {
     int32_t s[kSlots]; // local bit positions, distinct (the first S are used)
     int32_t ninstr;
     int32_t first;     // index of the first instruction
     int32_t pad[2];
+    // byte offset in an LDS tile of member m's slot bits, swizzled:
+    // lds_swizzle(sum over slots a of bit a of m << s[a]) * element size; a
+    // member's address is the thread's own swizzled offset XOR doff[m]
+    uint32_t doff[kGroup];
 };
-static_assert(sizeof(TileSweep) == 32, "TileSweep layout");
+static_assert(sizeof(TileSweep) == 96, "TileSweep layout");
 
 // Launch descriptor of one tile pass (passed by value as a kernel argument).
 struct TilePass {

@@ -154,8 +154,7 @@ struct Vec<float> { using type = float4; static constexpr int N = 4; };
 // the LDS banks.  S = 4 for 8-byte elements, 5 for 4-byte elements.
 template <typename T>
 __device__ __forceinline__ uint32_t swz(uint32_t j) {
-    constexpr int S = sizeof(T) == 8 ? 4 : 5;
-    return j ^ ((j >> S) & ((1u << S) - 1));
+    return lds_swizzle(j, sizeof(T));
 }
 
 template <int K>
@@ -373,9 +372,9 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
     for (int s = 0; s < nsweeps; ++s) {
         const TileSweep sw = sweeps[first_sweep + s];
         if (tid < NG) {
-            int sl[S], srt[S];
+            int srt[S];
 #pragma unroll
-            for (int a = 0; a < S; ++a) sl[a] = srt[a] = sw.s[a];
+            for (int a = 0; a < S; ++a) srt[a] = sw.s[a];
 #pragma unroll
             for (int a = 0; a < S; ++a)
 #pragma unroll
@@ -384,20 +383,21 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
             uint32_t jb = tid;
 #pragma unroll
             for (int a = 0; a < S; ++a) jb = insert_zero32(jb, srt[a]);
-            // member m of the group sits at local index jb | dep(m); dep is
-            // uniform, so it lives in scalar registers
-            auto dep = [&](int m) {
-                uint32_t d = 0;
+            // member m of the group sits at local index jb | dep(m), dep(m) the
+            // bits of m placed at the slots; swz(jb | dep(m)) = swz(jb) ^
+            // swz(dep(m)) (swz is linear, the bits are disjoint), and the
+            // planner stores the uniform second term in bytes (sw.doff), so
+            // each member's address is one XOR
+            constexpr int kLogT = sizeof(T) == 8 ? 3 : 2;
+            const uint32_t jbs = swz<T>(jb) << kLogT;
+            uint32_t off[G];
 #pragma unroll
-                for (int a = 0; a < S; ++a) d |= ((static_cast<uint32_t>(m) >> a) & 1u) << sl[a];
-                return d;
-            };
+            for (int m = 0; m < G; ++m) off[m] = jbs ^ sw.doff[m];
             T ar[G], ai[G];
 #pragma unroll
             for (int m = 0; m < G; ++m) {
-                const uint32_t jm = swz<T>(jb | dep(m));
-                ar[m] = s_re[jm];
-                ai[m] = s_im[jm];
+                ar[m] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_re) + off[m]);
+                ai[m] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_im) + off[m]);
             }
             const TileInstr* ip = instrs + sw.first;
             const int ni = sw.ninstr;
@@ -453,7 +453,16 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
                             diag_apply<T, G>(ar, ai, bit ? sel1 : 0u, bit ? 0u : sel0, g);
                         }
                     } else if (kind == T_NEG) {
-                        neg_apply<T, G>(ar, ai, sel1);
+                        if constexpr (S == 3) {
+                            // member factors (+-1, exact) come with the record
+#pragma unroll
+                            for (int m = 0; m < G; ++m) {
+                                ar[m] *= g[m];
+                                ai[m] *= g[m];
+                            }
+                        } else {
+                            neg_apply<T, G>(ar, ai, sel1);
+                        }
                     } else {
                         diag_apply<T, G>(ar, ai, sel1, sel0, g);
                     }
@@ -461,9 +470,8 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
             }
 #pragma unroll
             for (int m = 0; m < G; ++m) {
-                const uint32_t jm = swz<T>(jb | dep(m));
-                s_re[jm] = ar[m];
-                s_im[jm] = ai[m];
+                *reinterpret_cast<T*>(reinterpret_cast<char*>(s_re) + off[m]) = ar[m];
+                *reinterpret_cast<T*>(reinterpret_cast<char*>(s_im) + off[m]) = ai[m];
             }
         }
         __syncthreads();

@@ -490,7 +490,15 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
         return -1;
     };
     // Sign flips commute and compose by XOR: a Z-family instruction whose
-    // selection depends on slot bits only merges into a preceding one.
+    // selection depends on slot bits only merges into a preceding one.  In
+    // 3-slot sweeps a slot-target flip also carries its member factors
+    // (-1 where mm1 selects, else 1) in m[0..7], which the kernel reads with
+    // the record's other parameters instead of deriving them.
+    auto set_factors = [&](TileInstr& in) {
+        if (in.kind != T_NEG || in.slot < 0 || S != 3) return;
+        for (int m = 0; m < 8; ++m) in.m[m] = ((in.mm1 >> m) & 1u) ? -1.0 : 1.0;
+        fill_floats(in.m, in.mf);
+    };
     auto push = [&](TileInstr in, int32_t first) {
         in.guard = in.omask != 0 ? 1 : 0;
         const bool uniform_neg = in.kind == T_NEG && in.slot >= 0 && in.lmask == 0 && in.omask == 0;
@@ -498,9 +506,11 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
             TileInstr& prev = plan.instrs.back();
             if (prev.kind == T_NEG && prev.slot >= 0 && prev.lmask == 0 && prev.omask == 0) {
                 prev.mm1 = static_cast<uint16_t>(prev.mm1 ^ in.mm1);
+                set_factors(prev);
                 return;
             }
         }
+        set_factors(in);
         plan.instrs.push_back(in);
     };
     // Negation where (index & mask) == pattern: the pattern's slot bits
@@ -552,6 +562,12 @@ void emit_tile_pass(HostPlan& plan, int n, int k, int S, uint64_t tile_mask, con
             if (slot_of(b) < 0) slots[nslots++] = b;
         TileSweep sw{};
         for (int s = 0; s < kSlots; ++s) sw.s[s] = s < S ? slots[s] : -1;
+        const int eb = plan.dtype == BFLY_C64 ? 4 : 8;
+        for (uint32_t m = 0; m < static_cast<uint32_t>(G); ++m) {
+            uint32_t d = 0;
+            for (int a = 0; a < S; ++a) d |= ((m >> a) & 1u) << slots[a];
+            sw.doff[m] = lds_swizzle(d, eb) * static_cast<uint32_t>(eb);
+        }
         sw.first = static_cast<int32_t>(plan.instrs.size());
         // a phase of a rotation's qubit just before it in the sweep rides on
         // the rotation's instruction (pre-phase; 1 when there is none)

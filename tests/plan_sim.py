@@ -236,7 +236,10 @@ def run_plan(text: str, psi: np.ndarray, udata: np.ndarray) -> np.ndarray:
                             s1 = sel1[None, :] * np.ones((ntiles, 1), np.int64)
                             s0 = sel0[None, :] * np.ones((ntiles, 1), np.int64)
                         active = tile_ok[:, None] & lane_ok[None, :]
-                        if ins["kind"] == T_NEG:
+                        if ins["kind"] == T_NEG and ins["slot"] >= 0 and S == 3:
+                            # 3-slot kernels multiply by the record's member factors
+                            f = np.asarray(m[:8])[member][None, :] * np.ones((ntiles, 1))
+                        elif ins["kind"] == T_NEG:
                             f = np.where(s1 == 1, -1.0, 1.0)
                         else:
                             f = np.where(s1 == 1, complex(m[2], m[3]), np.where(s0 == 1, complex(m[0], m[1]), 1.0))
