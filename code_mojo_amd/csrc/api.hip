@@ -603,7 +603,12 @@ int bfly_autotune_candidates(int32_t num_qubits, int32_t dtype, const bfly_exec_
     add(*base);
     if (modes & (1 << BFLY_MODE_FUSED)) {
         const int d = default_tile_bits(dtype);
-        for (int k = d - 1; k <= d + 1; ++k) {
+        std::vector<int> ks = {d - 1, d, d + 1};
+        // a state that fits one default tile runs in one workgroup: offer
+        // tiles one and two bits narrower than the state (several workgroups,
+        // and Grover iterations as phase flips and diffusers)
+        if (num_qubits <= d) ks.insert(ks.end(), {num_qubits - 1, num_qubits - 2});
+        for (int k : ks) {
             if (k < kMinTileBits || k > kMaxTileBits || (k > num_qubits && k != d)) continue;
             for (int cap2 : {0, 2 * k}) {
                 bfly_exec_opts o = *base;

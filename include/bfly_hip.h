@@ -170,9 +170,11 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s);
    No counterpart in the reference (its strategies take fixed layouts:
    compute_grid_layout, executors.mojo:213-228).  Candidate schedules are
    the base options plus, for each mode whose bit (1 << BFLY_MODE_*) is set in
-   `modes`: FUSED at the default tile size and one size either side, each
-   with no cap and with a cap of 2k gates per pass (k = tile bits); BLOCK;
-   AUTO.  Options that compile to an identical schedule are kept once. */
+   `modes`: FUSED at the default tile size and one size either side (and, for
+   a state no wider than the default tile, one and two bits narrower than the
+   state), each with no cap and with a cap of 2k gates per pass (k = tile
+   bits); BLOCK; AUTO.  Options that compile to an identical schedule are kept
+   once. */
 #define BFLY_TUNE_ALL ((1 << BFLY_MODE_FUSED) | (1 << BFLY_MODE_BLOCK) | (1 << BFLY_MODE_AUTO))
 /* Host only: the candidate list for an n-qubit circuit (at most `cap`
    written; *count = how many there are). */

@@ -26,7 +26,12 @@ def test_complex64_default_tile_and_small_states():
     c = [sig(o) for o in autotune.candidates(24, np.float32)]
     # default 2^13 (spelled 0) is the widest tile: 2^12 is the only other size
     assert {(k, g) for m, k, g in c if m == _lib.MODE_FUSED} == {(12, 0), (12, 24), (0, 0), (0, 26)}
+    # a state no wider than the default tile also gets tiles narrower than itself
+    # (a 2^12 tile on a 12-qubit state is the whole state: spelled 0)
+    c12 = [sig(o) for o in autotune.candidates(12, np.float32)]
+    assert {k for m, k, g in c12 if m == _lib.MODE_FUSED} == {0, 11, 10}
     small = [sig(o) for o in autotune.candidates(8, np.float64)]
+    assert {k for m, k, g in small if m == _lib.MODE_FUSED} == {0, 7, 6}
     assert all(m != _lib.MODE_BLOCK for m, k, g in small)   # BLOCK needs 9 qubits
     assert all(k <= 8 or k == 0 for m, k, g in small)
 
