@@ -1354,6 +1354,17 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
     // BLOCK mode below 9 qubits (fewer than 16 columns of 32) runs fused tiles
     const bool fused = (opts.mode == BFLY_MODE_FUSED || opts.mode == BFLY_MODE_BLOCK) && n >= kMinTileBits;
     if (k > n) k = n;
+    if (opts.tile_bits <= 0 && k == n && n >= 10) {
+        // A state that fits one tile runs every pass in one workgroup.  Long
+        // circuits (at least 48 gates per qubit: Grover searches) run faster
+        // on tiles two bits narrower than the state, which spread each pass
+        // over several workgroups and let Grover iterations become phase
+        // flips and diffusers (measured with the autotuner on 10-13 qubit
+        // complex64 Grover: 1.4-2.3x).
+        uint64_t ngates = 0;
+        for (uint64_t i = 0; i < nops; ++i) ngates += ops[i].kind == BFLY_OP_GATE ? 1 : 0;
+        if (ngates >= 48ull * static_cast<uint64_t>(n)) k = n - 2;
+    }
     int L = opts.low_bits > 0 ? opts.low_bits : default_low_bits(dtype);
     // the tile kernel stages 16-byte vectors: 2 (f64) or 4 (f32) consecutive
     // amplitudes must be consecutive local indices
