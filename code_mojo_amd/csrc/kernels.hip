@@ -403,6 +403,11 @@ __device__ __forceinline__ void run_sweeps(T* s_re, T* s_im, const TilePass& P, 
             const int ni = sw.ninstr;
             for (int q = 0; q < ni; ++q) {
                 const InstrRegs<T> in = load_instr<T>(ip + q);
+                // the record's header and parameters arrive in one scalar-load
+                // round trip (otherwise the parameter load is issued only after
+                // the header tests, a second round trip per record)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) asm volatile("" ::"s"(in.g[e]));
                 if (ubyte(in.w1, 1) && (base & ip[q].omask) != ip[q].oval) continue;
                 if ((jb & in.lmask) != in.lval) continue;
                 const int kind = sbyte(in.w0, 0);
