@@ -182,3 +182,22 @@ def test_waiting_phases_are_applied_before_a_barrier(barrier):
         plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
         got = run_plan(plan.dump(), psi, ud)
         assert np.abs(got - ref).max() < 1e-12, (barrier, tb)
+
+
+def test_long_single_tile_circuits_get_narrower_tiles():
+    """A state that fits one default tile would run each pass in one
+    workgroup: a long circuit (>= 48 gates per qubit, here a 12-qubit Grover
+    search) compiles to tiles two bits narrower than the state, so its
+    iterations become phase flips and diffusers; a short one (a 12-qubit,
+    4-layer circuit) keeps the single tile; an explicit tile size is kept."""
+    from code_mojo_amd import serve
+    job = serve.grover_job(12, 3, np.float32)
+    text = Plan(12, np.float32, job.ops, job.udata, _lib.opts()).dump()
+    assert "tile k=10 " in text and "diffuse" in text
+    ops, ud = lower_circuit(W.random_layered(12, 4, seed=5))
+    assert "tile k=12 " in Plan(12, np.float32, ops, ud, _lib.opts()).dump()
+    assert "tile k=12 " in Plan(12, np.float32, job.ops, job.udata, _lib.opts(tile_bits=12)).dump()
+    psi = np.zeros(1 << 12, dtype=complex)
+    psi[0] = 1
+    got = run_plan(text, psi, job.udata)
+    assert np.abs(got - oracle_run(12, job.ops, job.udata, psi)).max() < 1e-12
