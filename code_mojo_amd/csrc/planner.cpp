@@ -135,9 +135,10 @@ int default_tile_bits(int32_t dtype) { return dtype == BFLY_C64 ? 13 : 12; }
 // 3-slot sweeps (8 amplitudes per thread) from 2^10-amplitude tiles for both
 // precisions: twice the threads of 4-slot sweeps and straight-line three-slot
 // rotation records.  For complex64 2^13 tiles (1024 threads, two blocks per
-// CU) the kernel spills 28 bytes per thread and still runs 8 % faster than
-// with 4 slots (30q layered 435 vs 471 ms); one block per CU without spills
-// is slower (560 ms).
+// CU) the kernel spills 28 bytes per thread (the tile's global addressing,
+// stored once per tile and reloaded for its stage-out, none in the gate loop)
+// and still runs 8 % faster than with 4 slots (30q layered 435 vs 471 ms);
+// one block per CU without spills is slower (560 ms).
 int default_sweep_slots(int32_t dtype, int k) { (void)dtype; return k >= kMinSlots3Bits ? 3 : 4; }
 int default_low_bits(int32_t dtype) { return dtype == BFLY_C64 ? 5 : 4; }
 
