@@ -2316,30 +2316,31 @@ __global__ __launch_bounds__(kDiffChunkThreads) void diffuse_apply_kernel(const 
             V* rv = reinterpret_cast<V*>(re);
             V* iv = reinterpret_cast<V*>(im);
             const uint64_t nv = len / E;
-            // two vectors of re and im in flight per step, updated in order
-            for (uint64_t k = threadIdx.x; k < nv; k += 2 * kDiffChunkThreads) {
-                const uint64_t k2 = k + kDiffChunkThreads;
-                const bool two = k2 < nv;
-                V xr0 = rv[k], xi0 = iv[k], xr1{}, xi1{};
-                if (two) {
-                    xr1 = rv[k2];
-                    xi1 = iv[k2];
+            // four vectors of re and im in flight per step (loaded before any
+            // store: re and im may not be assumed apart), updated in order
+            constexpr int U = 4;
+            for (uint64_t k0 = threadIdx.x; k0 < nv; k0 += U * kDiffChunkThreads) {
+                V xr[U], xi[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t k = k0 + static_cast<uint64_t>(u) * kDiffChunkThreads;
+                    if (k < nv) {
+                        xr[u] = rv[k];
+                        xi[u] = iv[k];
+                    }
                 }
-                T* a = reinterpret_cast<T*>(&xr0);
-                T* b = reinterpret_cast<T*>(&xi0);
 #pragma unroll
-                for (int j = 0; j < E; ++j)
-                    diff_update(a[j], b[j], ar, ai, variant, flip, g0 + k * E + j, fmask, fvalue, next, v);
-                rv[k] = xr0;
-                iv[k] = xi0;
-                if (two) {
-                    a = reinterpret_cast<T*>(&xr1);
-                    b = reinterpret_cast<T*>(&xi1);
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t k = k0 + static_cast<uint64_t>(u) * kDiffChunkThreads;
+                    if (k < nv) {
+                        T* a = reinterpret_cast<T*>(&xr[u]);
+                        T* b = reinterpret_cast<T*>(&xi[u]);
 #pragma unroll
-                    for (int j = 0; j < E; ++j)
-                        diff_update(a[j], b[j], ar, ai, variant, flip, g0 + k2 * E + j, fmask, fvalue, next, v);
-                    rv[k2] = xr1;
-                    iv[k2] = xi1;
+                        for (int j = 0; j < E; ++j)
+                            diff_update(a[j], b[j], ar, ai, variant, flip, g0 + k * E + j, fmask, fvalue, next, v);
+                        rv[k] = xr[u];
+                        iv[k] = xi[u];
+                    }
                 }
             }
         } else {
