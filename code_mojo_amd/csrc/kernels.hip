@@ -1798,48 +1798,62 @@ __device__ __forceinline__ uint64_t rev_bits64(uint64_t x, int nbits) {
     return nbits == 0 ? 0ull : (__brevll(x) >> (64 - nbits));
 }
 
-constexpr int kRevB = 5;  // 32 x 32 tiles
-
 // Index i = (h: top B bits | m: middle M bits | l: low B bits); its reversal is
 // (rev l | rev m | rev h).  Block m (with m <= rev m) swaps tile m with tile
-// rev m through LDS; loads and stores run along l (32 contiguous amplitudes).
-template <typename T>
+// rev m through LDS, re first and then im (two W x W buffers, padded rows);
+// loads and stores are 16-byte vectors along l (W contiguous amplitudes: 512
+// bytes per row for complex128 at W = 64).  Pure data movement: bit-exact.
+template <typename T, int B>
 __global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* __restrict__ re, T* __restrict__ im, int n) {
-    constexpr int B = kRevB;
     constexpr int W = 1 << B;
-    __shared__ T a_re[W][W + 1], a_im[W][W + 1], b_re[W][W + 1], b_im[W][W + 1];
+    constexpr int VN = 16 / static_cast<int>(sizeof(T));   // amplitudes per 16-byte vector
+    constexpr int RV = W / VN;                              // vectors per row
+    using V = typename Vec<T>::type;
+    __shared__ T ta[W][W + 1], tb[W][W + 1];
     const int M = n - 2 * B;
     const uint64_t m = blockIdx.x;
     const uint64_t rm = rev_bits64(m, M);
     if (m > rm) return;
     const bool self = (m == rm);
     const int hs = n - B;
-    for (int e = threadIdx.x; e < W * W; e += 256) {
-        const int h = e >> B, l = e & (W - 1);
-        const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | l;
-        a_re[h][l] = re[ia];
-        a_im[h][l] = im[ia];
-        if (!self) {
-            const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | l;
-            b_re[h][l] = re[ib];
-            b_im[h][l] = im[ib];
+    for (int arr = 0; arr < 2; ++arr) {
+        T* p = arr == 0 ? re : im;
+        for (int e = threadIdx.x; e < W * RV; e += 256) {
+            const int h = e / RV, l0 = (e % RV) * VN;
+            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+            const V va = *reinterpret_cast<const V*>(p + ia);
+            const T* xa = reinterpret_cast<const T*>(&va);
+#pragma unroll
+            for (int j = 0; j < VN; ++j) ta[h][l0 + j] = xa[j];
+            if (!self) {
+                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                const V vb = *reinterpret_cast<const V*>(p + ib);
+                const T* xb = reinterpret_cast<const T*>(&vb);
+#pragma unroll
+                for (int j = 0; j < VN; ++j) tb[h][l0 + j] = xb[j];
+            }
         }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < W * W; e += 256) {
-        const int h = e >> B, l = e & (W - 1);
-        const int rh = static_cast<int>(rev_bits32(h, B)), rl = static_cast<int>(rev_bits32(l, B));
-        const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | l;
-        if (self) {
-            re[ia] = a_re[rl][rh];
-            im[ia] = a_im[rl][rh];
-        } else {
-            re[ia] = b_re[rl][rh];
-            im[ia] = b_im[rl][rh];
-            const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | l;
-            re[ib] = a_re[rl][rh];
-            im[ib] = a_im[rl][rh];
+        __syncthreads();
+        for (int e = threadIdx.x; e < W * RV; e += 256) {
+            const int h = e / RV, l0 = (e % RV) * VN;
+            const int rh = static_cast<int>(rev_bits32(static_cast<uint32_t>(h), B));
+            V oa, ob;
+            T* ya = reinterpret_cast<T*>(&oa);
+            T* yb = reinterpret_cast<T*>(&ob);
+#pragma unroll
+            for (int j = 0; j < VN; ++j) {
+                const int rl = static_cast<int>(rev_bits32(static_cast<uint32_t>(l0 + j), B));
+                ya[j] = self ? ta[rl][rh] : tb[rl][rh];
+                yb[j] = ta[rl][rh];
+            }
+            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+            *reinterpret_cast<V*>(p + ia) = oa;
+            if (!self) {
+                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                *reinterpret_cast<V*>(p + ib) = ob;
+            }
         }
+        __syncthreads();   // the next array reuses the buffers
     }
 }
 
@@ -2464,9 +2478,15 @@ hipError_t launch_gate_t(T* re, T* im, const GateLaunch& G, hipStream_t st) {
 template <typename T>
 hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
     const uint64_t size = 1ull << n;
-    if (mask == 0 && n >= 2 * kRevB) {
-        const uint64_t blocks = 1ull << (n - 2 * kRevB);
-        hipLaunchKernelGGL(bitrev_tiled_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im, n);
+    if (mask == 0 && n >= 12 && sizeof(T) == 4) {
+        const uint64_t blocks = 1ull << (n - 12);   // 64 x 64 tiles (complex64: 33 KiB of LDS)
+        hipLaunchKernelGGL((bitrev_tiled_kernel<T, 6>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
+                           n);
+    } else if (mask == 0 && n >= 10) {
+        const uint64_t blocks = 1ull << (n - 10);   // 32 x 32 tiles (complex128 at 64 x 64 would hold
+                                                     // 65 KiB of LDS: two blocks per CU, slower)
+        hipLaunchKernelGGL((bitrev_tiled_kernel<T, 5>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
+                           n);
     } else {
         if (mask == 0) mask = size - 1;
         hipLaunchKernelGGL(bitrev_simple_kernel<T>, dim3(grid_for(size)), dim3(256), 0, st, re, im, size, mask, n);
