@@ -1097,6 +1097,28 @@ struct MeasureSetup {
     std::vector<double> P;
 };
 
+// Outcome o of the target list (bit k = target k) from the probabilities Pu of
+// the distinct unforced free qubits uq: 0 when o disagrees with a forced value
+// or gives one qubit two values.
+void measure_spread(MeasureSetup& ms, int f, const std::vector<int32_t>& uq, const std::vector<double>& Pu) {
+    for (uint64_t o = 0; o < (1ull << f); ++o) {
+        uint64_t assigned = 0, bits = 0;
+        bool ok = true;
+        for (int k = 0; k < f && ok; ++k) {
+            const int q = ms.free_q[static_cast<size_t>(k)];
+            const uint64_t b = (o >> k) & 1;
+            if ((ms.forced_mask >> q) & 1) ok = ((ms.forced_val >> q) & 1) == b;
+            else if ((assigned >> q) & 1) ok = ((bits >> q) & 1) == b;
+            assigned |= 1ull << q;
+            bits |= b << q;
+        }
+        if (!ok) continue;
+        uint64_t u = 0;
+        for (size_t j = 0; j < uq.size(); ++j) u |= ((bits >> uq[j]) & 1) << j;
+        ms.P[o] = Pu[u];
+    }
+}
+
 int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, int m, MeasureSetup& ms) {
     bool conflict = false;
     for (int k = 0; k < m; ++k) {
@@ -1114,20 +1136,42 @@ int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, 
         }
     }
     const int f = static_cast<int>(ms.free_q.size());
-    if (f > 20) return fail(BFLY_EUNSUPPORTED, "more than 20 free measured targets");
+    if (f > kOutcomeDirectMaxBits)
+        return fail(BFLY_EUNSUPPORTED, "more than " + std::to_string(kOutcomeDirectMaxBits) + " free measured targets");
     std::vector<int32_t> uq;
     uint64_t umask = 0;
     for (int32_t q : ms.free_q)
         if (!((ms.forced_mask >> q) & 1) && !((umask >> q) & 1)) { uq.push_back(q); umask |= 1ull << q; }
-    if (static_cast<int>(uq.size()) > kOutcomeMaxBits)
-        return fail(BFLY_EUNSUPPORTED, "more than 12 distinct free measured qubits");
     ms.P.assign(size_t(1) << f, 0.0);
     if (conflict) return BFLY_OK;  // no amplitude matches the forced values
+    const uint64_t all = (1ull << s->n) - 1;
+    std::vector<double> Pu(size_t(1) << uq.size());
+    if (static_cast<int>(uq.size()) > kOutcomeMaxBits) {
+        // many outcomes: one thread per outcome
+        OutcomeDirect dp{};
+        dp.f = static_cast<int32_t>(uq.size());
+        for (size_t j = 0; j < uq.size(); ++j) dp.fpos[j] = uq[j];
+        dp.fval = ms.forced_val;
+        dp.rmask = all & ~(ms.forced_mask | umask);
+        dp.nrest = 1ull << __builtin_popcountll(dp.rmask);
+        const int R = outcome_direct_chunks(dp);
+        HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+        double* buf = nullptr;
+        const size_t parts = R > 1 ? static_cast<size_t>(R) * Pu.size() : 0;
+        HIP_TRY(hipMalloc(&buf, (Pu.size() + parts) * sizeof(double)), "hipMalloc(measurement)");
+        hipError_t e = launch_outcome_direct(s->dtype, s->re, s->im, dp, R, buf + Pu.size(), buf, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(Pu.data(), buf, Pu.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        (void)hipFree(buf);
+        if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
+        measure_spread(ms, f, uq, Pu);
+        return BFLY_OK;
+    }
     OutcomeSpec sp{};
     sp.f = static_cast<int32_t>(uq.size());
     for (size_t j = 0; j < uq.size(); ++j) sp.fpos[j] = uq[j];
     sp.fval = ms.forced_val;
-    const uint64_t all = (1ull << s->n) - 1;
     sp.rmask = all & ~(ms.forced_mask | umask);
     const uint64_t rest_count = 1ull << __builtin_popcountll(sp.rmask);
     const int nb = outcome_blocks(sp, rest_count);
@@ -1136,7 +1180,6 @@ int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, 
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
     double* buf = nullptr;
     HIP_TRY(hipMalloc(&buf, (static_cast<size_t>(nb) * nout + nout) * sizeof(double)), "hipMalloc(measurement)");
-    std::vector<double> Pu(nout);
     hipError_t e = launch_outcome_probs(s->dtype, s->re, s->im, sp, nb, buf, buf + static_cast<size_t>(nb) * nout,
                                         s->stream);
     if (e == hipSuccess)
@@ -1145,22 +1188,7 @@ int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, 
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
-    for (uint64_t o = 0; o < (1ull << f); ++o) {
-        uint64_t assigned = 0, bits = 0;
-        bool ok = true;
-        for (int k = 0; k < f && ok; ++k) {
-            const int q = ms.free_q[k];
-            const uint64_t b = (o >> k) & 1;
-            if ((ms.forced_mask >> q) & 1) ok = ((ms.forced_val >> q) & 1) == b;
-            else if ((assigned >> q) & 1) ok = ((bits >> q) & 1) == b;
-            assigned |= 1ull << q;
-            bits |= b << q;
-        }
-        if (!ok) continue;
-        uint64_t u = 0;
-        for (size_t j = 0; j < uq.size(); ++j) u |= ((bits >> uq[j]) & 1) << j;
-        ms.P[o] = Pu[u];
-    }
+    measure_spread(ms, f, uq, Pu);
     return BFLY_OK;
 }
 

@@ -58,6 +58,23 @@ struct OutcomeSpec {
     uint64_t rest_per_block;   // rest values per block
 };
 int outcome_blocks(const OutcomeSpec& sp, uint64_t rest_count);
+// More than kOutcomeMaxBits (and at most kOutcomeDirectMaxBits) distinct free
+// qubits: one thread per outcome and rest chunk sums its amplitudes (rest
+// values in increasing order, TwoSum in FP64), then the chunks are added in
+// order: deterministic.
+constexpr int kOutcomeDirectMaxBits = 24;
+struct OutcomeDirect {
+    int32_t f;                 // free qubits (outcome bit k = qubit fpos[k])
+    int32_t pad;
+    int32_t fpos[kOutcomeDirectMaxBits];
+    uint64_t fval;             // forced bits
+    uint64_t rmask;            // every other qubit
+    uint64_t nrest;            // 2^popcount(rmask)
+};
+int outcome_direct_chunks(const OutcomeDirect& sp);   // R, a power of two dividing nrest
+// part: R * 2^f doubles (unused when R == 1); out: 2^f doubles
+hipError_t launch_outcome_direct(int dtype, const void* re, const void* im, const OutcomeDirect& sp, int R,
+                                 double* part, double* out, hipStream_t stream);
 hipError_t launch_outcome_probs(int dtype, const void* re, const void* im, const OutcomeSpec& sp, int nblocks,
                                 double* part, double* out, hipStream_t stream);
 // Amplitudes with (i & tmask) == tval times inv (in the state's precision), the others 0.

@@ -2773,6 +2773,68 @@ hipError_t launch_outcome_t(const T* re, const T* im, const OutcomeSpec& sp, int
     return hipGetLastError();
 }
 
+// Grid (outcome blocks, R rest chunks): chunk y sums rest values
+// [y*nrest/R, (y+1)*nrest/R) of outcome o into part[y][o]; with R == 1 part is
+// the result.
+template <typename T>
+__global__ __launch_bounds__(256) void outcome_direct_kernel(const T* __restrict__ re, const T* __restrict__ im,
+                                                             const OutcomeDirect sp, uint64_t per_chunk,
+                                                             double* __restrict__ part) {
+    const uint64_t nout = 1ull << sp.f;
+    const uint64_t o = static_cast<uint64_t>(blockIdx.x) * 256u + threadIdx.x;
+    if (o >= nout) return;
+    uint64_t base = sp.fval;
+    for (int j = 0; j < sp.f; ++j) base |= ((o >> j) & 1ull) << sp.fpos[j];
+    // first rest value of this chunk, deposited on rmask
+    const uint64_t r0 = static_cast<uint64_t>(blockIdx.y) * per_chunk;
+    uint64_t dep = 0, m = sp.rmask;
+    for (int b = 0; m; ++b, m &= m - 1)
+        if ((r0 >> b) & 1ull) dep |= m & (~m + 1ull);
+    double s = 0.0, c = 0.0;
+    for (uint64_t r = 0; r < per_chunk; ++r) {
+        const uint64_t i = base | dep;
+        const double a = static_cast<double>(re[i]), b = static_cast<double>(im[i]);
+        two_sum_add(s, c, a * a + b * b);
+        dep = ((dep | ~sp.rmask) + 1ull) & sp.rmask;   // next rest value
+    }
+    part[static_cast<uint64_t>(blockIdx.y) * nout + o] = s + c;
+}
+
+// out[o] = the R partial sums of outcome o, added in chunk order.
+__global__ __launch_bounds__(256) void outcome_direct_combine_kernel(const double* __restrict__ part, uint64_t nout,
+                                                                     int R, double* __restrict__ out) {
+    const uint64_t o = static_cast<uint64_t>(blockIdx.x) * 256u + threadIdx.x;
+    if (o >= nout) return;
+    double s = 0.0, c = 0.0;
+    for (int y = 0; y < R; ++y) two_sum_add(s, c, part[static_cast<uint64_t>(y) * nout + o]);
+    out[o] = s + c;
+}
+
+int outcome_direct_chunks(const OutcomeDirect& sp) {
+    const uint64_t gx = ((1ull << sp.f) + 255) / 256;
+    int R = 1;
+    while (gx * static_cast<uint64_t>(R) < 2048 && static_cast<uint64_t>(R) * 2 <= sp.nrest && R < 1024) R *= 2;
+    return R;
+}
+
+hipError_t launch_outcome_direct(int dtype, const void* re, const void* im, const OutcomeDirect& sp, int R,
+                                 double* part, double* out, hipStream_t st) {
+    if (sp.f < 1 || sp.f > kOutcomeDirectMaxBits || R < 1 || sp.nrest % static_cast<uint64_t>(R))
+        return hipErrorInvalidValue;
+    const uint64_t nout = 1ull << sp.f;
+    const unsigned gx = static_cast<unsigned>((nout + 255) / 256);
+    const uint64_t per = sp.nrest / static_cast<uint64_t>(R);
+    double* dst = R == 1 ? out : part;
+    if (dtype == BFLY_C128)
+        hipLaunchKernelGGL(outcome_direct_kernel<double>, dim3(gx, R), dim3(256), 0, st,
+                           static_cast<const double*>(re), static_cast<const double*>(im), sp, per, dst);
+    else
+        hipLaunchKernelGGL(outcome_direct_kernel<float>, dim3(gx, R), dim3(256), 0, st,
+                           static_cast<const float*>(re), static_cast<const float*>(im), sp, per, dst);
+    if (R > 1) hipLaunchKernelGGL(outcome_direct_combine_kernel, dim3(gx), dim3(256), 0, st, part, nout, R, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_outcome_probs(int dtype, const void* re, const void* im, const OutcomeSpec& sp, int nblocks,
                                 double* part, double* out, hipStream_t st) {
     if (sp.f < 0 || sp.f > kOutcomeMaxBits) return hipErrorInvalidValue;

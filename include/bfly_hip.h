@@ -304,7 +304,7 @@ int bfly_scale(bfly_state* s, double scale);
    targets[k] with values[k] = -1 (free) or 0 / 1 (forced).  Outcome
    probabilities of the free targets (bit k of an outcome = the k-th free
    target) among the amplitudes matching the forced values are FP64
-   compensated sums on the device (at most 12 distinct free qubits); the
+   compensated sums on the device (at most 24 free targets); the
    outcome is chosen on the host exactly as the reference does: one MINSTD
    step rng = (seed mod 2147483647) * 48271 mod 2147483647,
    r = rng / 2147483647 * total in the state's precision, the first outcome
