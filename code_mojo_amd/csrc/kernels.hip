@@ -1811,49 +1811,51 @@ __global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* __restrict__ re, T
     using V = typename Vec<T>::type;
     __shared__ T ta[W][W + 1], tb[W][W + 1];
     const int M = n - 2 * B;
-    const uint64_t m = blockIdx.x;
-    const uint64_t rm = rev_bits64(m, M);
-    if (m > rm) return;
-    const bool self = (m == rm);
     const int hs = n - B;
-    for (int arr = 0; arr < 2; ++arr) {
-        T* p = arr == 0 ? re : im;
-        for (int e = threadIdx.x; e < W * RV; e += 256) {
-            const int h = e / RV, l0 = (e % RV) * VN;
-            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
-            const V va = *reinterpret_cast<const V*>(p + ia);
-            const T* xa = reinterpret_cast<const T*>(&va);
+    // blocks walk the middle indices (grid capped by the launcher)
+    for (uint64_t m = blockIdx.x; m < (1ull << M); m += gridDim.x) {
+        const uint64_t rm = rev_bits64(m, M);
+        if (m > rm) continue;   // uniform across the block
+        const bool self = (m == rm);
+        for (int arr = 0; arr < 2; ++arr) {
+            T* p = arr == 0 ? re : im;
+            for (int e = threadIdx.x; e < W * RV; e += 256) {
+                const int h = e / RV, l0 = (e % RV) * VN;
+                const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+                const V va = *reinterpret_cast<const V*>(p + ia);
+                const T* xa = reinterpret_cast<const T*>(&va);
 #pragma unroll
-            for (int j = 0; j < VN; ++j) ta[h][l0 + j] = xa[j];
-            if (!self) {
-                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
-                const V vb = *reinterpret_cast<const V*>(p + ib);
-                const T* xb = reinterpret_cast<const T*>(&vb);
+                for (int j = 0; j < VN; ++j) ta[h][l0 + j] = xa[j];
+                if (!self) {
+                    const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                    const V vb = *reinterpret_cast<const V*>(p + ib);
+                    const T* xb = reinterpret_cast<const T*>(&vb);
 #pragma unroll
-                for (int j = 0; j < VN; ++j) tb[h][l0 + j] = xb[j];
+                    for (int j = 0; j < VN; ++j) tb[h][l0 + j] = xb[j];
+                }
             }
+            __syncthreads();
+            for (int e = threadIdx.x; e < W * RV; e += 256) {
+                const int h = e / RV, l0 = (e % RV) * VN;
+                const int rh = static_cast<int>(rev_bits32(static_cast<uint32_t>(h), B));
+                V oa, ob;
+                T* ya = reinterpret_cast<T*>(&oa);
+                T* yb = reinterpret_cast<T*>(&ob);
+#pragma unroll
+                for (int j = 0; j < VN; ++j) {
+                    const int rl = static_cast<int>(rev_bits32(static_cast<uint32_t>(l0 + j), B));
+                    ya[j] = self ? ta[rl][rh] : tb[rl][rh];
+                    yb[j] = ta[rl][rh];
+                }
+                const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+                *reinterpret_cast<V*>(p + ia) = oa;
+                if (!self) {
+                    const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                    *reinterpret_cast<V*>(p + ib) = ob;
+                }
+            }
+            __syncthreads();   // the next array reuses the buffers
         }
-        __syncthreads();
-        for (int e = threadIdx.x; e < W * RV; e += 256) {
-            const int h = e / RV, l0 = (e % RV) * VN;
-            const int rh = static_cast<int>(rev_bits32(static_cast<uint32_t>(h), B));
-            V oa, ob;
-            T* ya = reinterpret_cast<T*>(&oa);
-            T* yb = reinterpret_cast<T*>(&ob);
-#pragma unroll
-            for (int j = 0; j < VN; ++j) {
-                const int rl = static_cast<int>(rev_bits32(static_cast<uint32_t>(l0 + j), B));
-                ya[j] = self ? ta[rl][rh] : tb[rl][rh];
-                yb[j] = ta[rl][rh];
-            }
-            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
-            *reinterpret_cast<V*>(p + ia) = oa;
-            if (!self) {
-                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
-                *reinterpret_cast<V*>(p + ib) = ob;
-            }
-        }
-        __syncthreads();   // the next array reuses the buffers
     }
 }
 
@@ -2475,16 +2477,21 @@ hipError_t launch_gate_t(T* re, T* im, const GateLaunch& G, hipStream_t st) {
     return hipGetLastError();
 }
 
+// One block per middle index up to 30 qubits (complex128); beyond, blocks
+// loop, which keeps blocks x threads within the 2^32 launch limit.
+constexpr uint64_t kBitrevMaxBlocks = 1ull << 20;
+
 template <typename T>
 hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
     const uint64_t size = 1ull << n;
     if (mask == 0 && n >= 12 && sizeof(T) == 4) {
-        const uint64_t blocks = 1ull << (n - 12);   // 64 x 64 tiles (complex64: 33 KiB of LDS)
+        const uint64_t blocks = std::min<uint64_t>(1ull << (n - 12), kBitrevMaxBlocks);   // 64 x 64 tiles
+                                                                                      // (complex64: 33 KiB of LDS)
         hipLaunchKernelGGL((bitrev_tiled_kernel<T, 6>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
                            n);
     } else if (mask == 0 && n >= 10) {
-        const uint64_t blocks = 1ull << (n - 10);   // 32 x 32 tiles (complex128 at 64 x 64 would hold
-                                                     // 65 KiB of LDS: two blocks per CU, slower)
+        const uint64_t blocks = std::min<uint64_t>(1ull << (n - 10), kBitrevMaxBlocks);   // 32 x 32 tiles
+                                          // (complex128 at 64 x 64 would hold 65 KiB of LDS: two blocks per CU, slower)
         hipLaunchKernelGGL((bitrev_tiled_kernel<T, 5>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
                            n);
     } else {
