@@ -5,7 +5,8 @@ No CPU oracle runs at this size, so the check is the value-encoding known
 answer: H on every qubit, the phases of a seeded 33-bit integer v and the
 inverse QFT with bit reversal give exactly the basis state |rev_33(v)>.  A
 wrong index anywhere in 2^33 amplitudes (a 32-bit overflow, a wrapped tile
-or block number) moves probability away from that one index."""
+or block number) moves probability away from that one index.  Measuring
+some of its qubits then returns their bits with probability 1."""
 import numpy as np
 import pytest
 
@@ -40,7 +41,14 @@ def test_33_qubit_value_encoding(gpu_lib, dtype, mode, tol):
     g.synchronize()
     pr = g.prob_mask((1 << N) - 1, rev)
     nrm = g.norm2()
+    # measuring three qubits of the basis state returns their bits of rev with
+    # probability 1 (outcome and collapse kernels over all 2^33 amplitudes)
+    targets = [0, N - 1, 17]
+    outcome, prob = g.measure(targets, None, 12345)
+    nrm_after = g.norm2()
     p.free()
     g.free()
     assert pr > 1 - tol, 1 - pr
     assert abs(nrm - 1) < tol, nrm
+    assert outcome == sum(((rev >> t) & 1) << k for k, t in enumerate(targets)), outcome
+    assert abs(prob - 1) < tol and abs(nrm_after - 1) < tol, (prob, nrm_after)
