@@ -194,3 +194,30 @@ def test_batches_and_engine_reject_measurement(gpu_lib):
         eng.release(rid)
     p.free()
     g.free()
+
+
+def test_sample_many_targets(gpu_lib):
+    """16 sampled targets (2^16 outcomes, the per-outcome kernel): shot 0 is
+    the outcome measure() picks with the same seed, and every shot lands on
+    an outcome with nonzero probability of the FP64 marginal."""
+    n = 18
+    st = W.normal_state(n, 23)
+    # zero the amplitudes with qubit 5 set: outcomes with that bit have p = 0
+    idx = np.arange(1 << n)
+    re, im = st.re.copy(), st.im.copy()
+    re[(idx >> 5) & 1 == 1] = 0.0
+    im[(idx >> 5) & 1 == 1] = 0.0
+    nrm = np.sqrt(np.sum(re ** 2 + im ** 2))
+    re, im = re / nrm, im / nrm
+    g = GpuState(n)
+    g.upload_arrays(re, im)
+    targets = [17, 5, 0, 1, 2, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14]
+    assert len(set(targets)) == 16
+    seed = 5550123
+    shots = g.sample(targets, 5000, seed)
+    P = fp64_outcome_probs(n, re, im, targets, [-1] * 16)
+    assert np.all(P[shots.astype(np.int64)] > 0)
+    assert not np.any((shots >> np.uint64(1)) & np.uint64(1))       # qubit 5 (bit 1) never set
+    outcome, _ = g.measure(targets, None, seed)
+    assert outcome == int(shots[0])
+    g.free()
