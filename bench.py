@@ -315,6 +315,11 @@ def main() -> int:
     ap.add_argument("--mode", choices=["fused", "unfused"], default="fused")
     ap.add_argument("--tile-bits", type=int, default=0)
     ap.add_argument("--max-gates", type=int, default=0, help="cap on gates per tile pass (0 = planner default)")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="tile-pass workgroup size (0 = default; 2^(k-4) selects 16-amplitude register sweeps)")
+    ap.add_argument("--jit", choices=["on", "off"], default="on",
+                    help="layered: compile every tile pass into its own kernel at plan time (BFLY_FLAG_JIT; "
+                         "the compile is part of plan building, outside the timed region)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-qubits", type=int, default=26)
     ap.add_argument("--cpu-layers", type=int, default=4)
@@ -371,11 +376,13 @@ def main() -> int:
     qc = W.random_layered(n, args.layers, args.seed)
     ops, ud = lower_circuit(qc)
     mode = _lib.MODE_FUSED if args.mode == "fused" else _lib.MODE_UNFUSED
+    flags = _lib.FLAG_PROFILE | (_lib.FLAG_JIT if args.jit == "on" else 0)
     plan = Plan(n, dtype, ops, ud, _lib.opts(mode=mode, tile_bits=args.tile_bits, max_gates_per_pass=args.max_gates,
-                                         flags=_lib.FLAG_PROFILE))
+                                         threads=args.threads, flags=flags))
+    plan_s = time.perf_counter() - t0
     st = plan.stats()
     log(f"[bench] lowered {len(ops)} gates -> {st['passes']} passes ({st['launches']} launches) "
-        f"in {time.perf_counter() - t0:.2f}s")
+        f"in {plan_s:.2f}s (jit {args.jit})")
     g = GpuState(n, dtype)
     log(f"[bench] state allocated ({g.nbytes() / 2**30:.1f} GiB); warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -426,8 +433,10 @@ def main() -> int:
         "config": {"workload": "random_layered_u3_czbrick_cx", "qubits": n, "layers": args.layers,
                    "state": "complex128" if args.dtype == "c128" else "complex64", "seed": args.seed,
                    "mode": args.mode, "tile_bits": args.tile_bits, "max_gates_per_pass": args.max_gates,
+                   "threads": args.threads,
                    "gates": int(len(ops)), "passes_per_circuit": st["passes"],
-                   "launches_per_circuit": st["launches"]},
+                   "launches_per_circuit": st["launches"], "jit": args.jit == "on",
+                   "plan_build_s": round(plan_s, 2)},
         "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "circuits_per_s": round(1e3 / ms_per_step, 4),
         "gate_layers_per_s": round(args.layers * 1e3 / ms_per_step, 3),

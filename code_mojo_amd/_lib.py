@@ -20,7 +20,7 @@ BFLY_OK, BFLY_EINVAL, BFLY_ENOMEM, BFLY_EHIP, BFLY_EUNSUPPORTED, BFLY_ENUMERIC =
 C128, C64 = 0, 1
 MODE_FUSED, MODE_UNFUSED, MODE_BLOCK, MODE_AUTO = 0, 1, 2, 3
 TUNE_ALL = (1 << MODE_FUSED) | (1 << MODE_BLOCK) | (1 << MODE_AUTO)
-FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS = 1, 2, 4
+FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS, FLAG_JIT = 1, 2, 4, 8
 BATCH_GRAPH = 1
 
 ERROR_NAMES = {BFLY_EINVAL: "EINVAL", BFLY_ENOMEM: "ENOMEM", BFLY_EHIP: "EHIP",
@@ -78,6 +78,7 @@ _SIGS = {
     "bfly_plan_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64),
                                    ct.POINTER(ct.c_int64)]),
     "bfly_plan_dump": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
+    "bfly_plan_jit_info": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "bfly_plan_execute": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "bfly_apply": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_uint64, ct.c_void_p, ct.c_uint64,
                               ct.POINTER(ExecOpts)]),

@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libbfly_hip.so"
 ARCH = os.environ.get("BFLY_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "api.hip", "engine.hip", "planner.cpp", "qasm.cpp"]
+SOURCES = ["kernels.hip", "api.hip", "engine.hip", "planner.cpp", "qasm.cpp", "jit.cpp"]
 
 # Packed-FP32 instructions are disabled for every device pass (the host pass
 # warns that it does not know the feature; that is harmless).
@@ -38,13 +38,19 @@ def build(verbose: bool = False) -> Path:
     """Compile every source to an object, link the shared library."""
     out_dir = PKG / "_build"
     out_dir.mkdir(exist_ok=True)
+    # tile_ops.h as a string literal: the generated tile-pass kernels
+    # (jit.cpp) hand it to hipRTC
+    ops = (CSRC / "tile_ops.h").read_text()
+    if ")BFLYSRC\"" in ops:
+        raise RuntimeError("tile_ops.h contains the raw-string delimiter")
+    (out_dir / "tile_ops_src.inc").write_text('R"BFLYSRC(' + ops + ')BFLYSRC"\n')
     objs = []
     cc = hipcc()
     for src in SOURCES:
         obj = out_dir / (src + ".o")
         cmd = [cc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
         if src.endswith(".cpp"):
-            cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", str(CSRC / src), "-o", str(obj)]
+            cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{out_dir}", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         res = subprocess.run(cmd, capture_output=True, text=True)
@@ -52,7 +58,8 @@ def build(verbose: bool = False) -> Path:
             raise RuntimeError(f"compile failed for {src}:\n{res.stdout}\n{res.stderr}")
         objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp), "-L/opt/rocm/lib", "-lhiprtc",
+           "-pthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

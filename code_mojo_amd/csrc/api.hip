@@ -121,6 +121,16 @@ int upload_plan(bfly_plan* p, int device) {
         HIP_TRY(hipMemcpy(p->d_btab, hp.btab.data(), hp.btab.size() * sizeof(double), hipMemcpyHostToDevice),
                 "hipMemcpy(plan angle tables)");
     }
+    if (p->jit) {
+        std::string err;
+        const int rc = jit_load(*p->jit, device, err);
+        if (rc) return fail(rc, err);
+        if (!p->jit->consts.empty()) {
+            HIP_TRY(hipMalloc(&p->jit->d_consts, p->jit->consts.size()), "hipMalloc(pass parameters)");
+            HIP_TRY(hipMemcpy(p->jit->d_consts, p->jit->consts.data(), p->jit->consts.size(), hipMemcpyHostToDevice),
+                    "hipMemcpy(pass parameters)");
+        }
+    }
     p->device = device;
     return BFLY_OK;
 }
@@ -350,6 +360,14 @@ int bfly_plan_create(bfly_plan** out, int32_t num_qubits, int32_t dtype, const b
         delete p;
         return fail(rc, err);
     }
+    if (o.flags & BFLY_FLAG_JIT) {
+        p->jit = new JitPlan();
+        if ((rc = jit_build(*p->jit, p->hp, err))) {
+            delete p->jit;
+            delete p;
+            return fail(rc, err);
+        }
+    }
     *out = p;
     return BFLY_OK;
 }
@@ -364,6 +382,13 @@ int bfly_plan_destroy(bfly_plan* p) {
         if (p->d_bmat) (void)hipFree(p->d_bmat);
         if (p->d_btab) (void)hipFree(p->d_btab);
     }
+    if (p->jit) {
+        if (p->jit->d_consts) {
+            if (p->jit->device >= 0) (void)hipSetDevice(p->jit->device);
+            (void)hipFree(p->jit->d_consts);
+        }
+        delete p->jit;
+    }
     delete p;
     return BFLY_OK;
 }
@@ -373,6 +398,13 @@ int bfly_plan_stats(const bfly_plan* p, int64_t* passes, int64_t* launches, int6
     if (passes) *passes = p->hp.passes;
     if (launches) *launches = static_cast<int64_t>(p->hp.steps.size());
     if (gates) *gates = p->hp.gates;
+    return BFLY_OK;
+}
+
+int bfly_plan_jit_info(const bfly_plan* p, int64_t* kernels, int64_t* compiled) {
+    if (!p) return fail(BFLY_EINVAL, "null plan");
+    if (kernels) *kernels = p->jit ? static_cast<int64_t>(p->jit->passes.size()) : 0;
+    if (compiled) *compiled = p->jit ? p->jit->compiled : 0;
     return BFLY_OK;
 }
 
@@ -515,12 +547,19 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
     if ((rc = upload_plan(p, s->device))) return rc;
     ProfScope prof((p->hp.opts.flags & BFLY_FLAG_PROFILE) != 0, s->stream);
-    for (const Step& st : p->hp.steps) {
+    for (size_t si = 0; si < p->hp.steps.size(); ++si) {
+        const Step& st = p->hp.steps[si];
         hipError_t e = hipSuccess;
         switch (st.kind) {
             case S_TILE:
                 prof.begin(F_TILE);
-                e = launch_tile_pass(s->dtype, s->re, s->im, st.tile, p->d_sweeps, p->d_instrs, s->stream);
+                if (p->jit && p->jit->device == s->device && p->jit->step_pass[si] >= 0) {
+                    const JitPass& jp = p->jit->passes[static_cast<size_t>(p->jit->step_pass[si])];
+                    e = jit_launch(jp, s->re, s->im, static_cast<const char*>(p->jit->d_consts) + jp.const_off,
+                                   s->stream);
+                } else {
+                    e = launch_tile_pass(s->dtype, s->re, s->im, st.tile, p->d_sweeps, p->d_instrs, s->stream);
+                }
                 break;
             case S_GATE:
                 prof.begin(F_GATE);

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "bfly_plan.h"
+#include "jit.h"
 #include "kernels.h"
 
 #include <vector>
@@ -29,6 +30,7 @@ struct bfly_plan {
     float* d_bmat = nullptr;             // BLOCK mode block matrices
     double* d_btab = nullptr;              // BLOCK mode column factors and outer angles
     bool borrowed = false;   // device records live in a buffer the plan does not own
+    bfly::JitPlan* jit = nullptr;   // BFLY_FLAG_JIT: generated tile-pass kernels
 };
 
 namespace bfly {

@@ -31,6 +31,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "tile_ops.h"
 
 namespace bfly {
 
@@ -41,10 +42,6 @@ __device__ __forceinline__ uint64_t insert_zero64(uint64_t x, int pos) {
     return ((x >> pos) << (pos + 1)) | lo;
 }
 
-__device__ __forceinline__ uint32_t insert_zero32(uint32_t x, int pos) {
-    const uint32_t lo = x & ((1u << pos) - 1);
-    return ((x >> pos) << (pos + 1)) | lo;
-}
 
 template <typename T>
 __device__ __forceinline__ void load_params(const double* m, const float* mf, T* g);
@@ -57,16 +54,6 @@ template <>
 __device__ __forceinline__ void load_params<float>(const double*, const float* mf, float* g) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) g[i] = mf[i];
-}
-
-// process_pair_scalar (transformations_scalar.mojo:10-40).
-template <typename T>
-__device__ __forceinline__ void pair_gen(T& ur, T& ui, T& vr, T& vi, const T* g) {
-    const T a = ur, b = ui, c = vr, d = vi;
-    ur = a * g[0] - b * g[1] + c * g[2] - d * g[3];
-    ui = a * g[1] + b * g[0] + c * g[3] + d * g[2];
-    vr = a * g[4] - b * g[5] + c * g[6] - d * g[7];
-    vi = a * g[5] + b * g[4] + c * g[7] + d * g[6];
 }
 
 // transform_h_scalar's (u +- v) * 0.7071067811865476 (transformations_scalar.mojo:241-283).
@@ -88,14 +75,6 @@ __device__ __forceinline__ void pair_ry(T& ur, T& ui, T& vr, T& vi, const T* g) 
     ui = b * g[0] - d * g[1];
     vr = a * g[1] + c * g[0];
     vi = b * g[1] + d * g[0];
-}
-
-// v *= (dr + i di) as in transform_p_scalar (transformations_scalar.mojo:330-333).
-template <typename T>
-__device__ __forceinline__ void cmul(T& vr, T& vi, T dr, T di) {
-    const T a = vr, b = vi;
-    vr = a * dr - b * di;
-    vi = a * di + b * dr;
 }
 
 template <typename T>
@@ -165,104 +144,6 @@ __device__ __forceinline__ uint64_t local_to_global(uint32_t j, const TilePass& 
     return g;
 }
 
-// A 2x2 matrix on slot SLOT of a 16-amplitude register group: pairs
-// (m0, m0 | 2^SLOT).  Pairs whose slot controls do not hold take the
-// identity; the test is on the member index only, so the selection is scalar
-// and no register array is copied.
-template <typename T, int G, int SLOT>
-__device__ __forceinline__ void slot_pairs_all(T* ar, T* ai, const T* g) {
-#pragma unroll
-    for (int p = 0; p < G / 2; ++p) {
-        constexpr int lo_mask = (1 << SLOT) - 1;
-        const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
-        pair_gen(ar[m0], ai[m0], ar[m0 | (1 << SLOT)], ai[m0 | (1 << SLOT)], g);
-    }
-}
-
-template <typename T, int G, int SLOT>
-__device__ __forceinline__ void slot_pairs(T* ar, T* ai, uint32_t cslots, const T* g) {
-    if (cslots == 0) {  // no slot controls: every pair, no selection
-        slot_pairs_all<T, G, SLOT>(ar, ai, g);
-        return;
-    }
-#pragma unroll
-    for (int p = 0; p < G / 2; ++p) {
-        constexpr int lo_mask = (1 << SLOT) - 1;
-        const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
-        const int m1 = m0 | (1 << SLOT);
-        const bool on = (static_cast<uint32_t>(m0) & cslots) == cslots;
-        T h[8];
-        h[0] = on ? g[0] : T(1);
-        h[1] = on ? g[1] : T(0);
-        h[2] = on ? g[2] : T(0);
-        h[3] = on ? g[3] : T(0);
-        h[4] = on ? g[4] : T(0);
-        h[5] = on ? g[5] : T(0);
-        h[6] = on ? g[6] : T(1);
-        h[7] = on ? g[7] : T(0);
-        pair_gen(ar[m0], ai[m0], ar[m1], ai[m1], h);
-    }
-}
-
-// Fused multiply-add in the state's precision (a single rounding).
-__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-// Z family: multiply the members selected in sel1 by -1 and the others by 1
-// (exact: the products flip or keep the sign bit).  sel1 is the same in
-// every lane, so the factors are scalars; whole-register products keep the
-// register arrays in place, where a sign-bit XOR on the high half of each
-// double costs a copy of the low half.
-template <typename T, int G>
-__device__ __forceinline__ void neg_apply(T* ar, T* ai, uint32_t sel1) {
-#pragma unroll
-    for (int m = 0; m < G; ++m) {
-        const T f = ((sel1 >> m) & 1u) ? T(-1) : T(1);
-        ar[m] *= f;
-        ai[m] *= f;
-    }
-}
-
-// Z family with the target bit outside the slots: `bit` (per lane or per
-// tile) is the target's value; members in mm1 flip when it is 1, members in
-// mm0 when it is 0 (mm1 and mm0 are the same in every lane).  With
-// F = -1 where bit is 1 and 1 elsewhere, member m's factor is a F + b with
-// scalar (a, b): (1, 0) for mm1 only, (-1, 0) for mm0 only, (0, -1) for
-// both, (0, 1) for neither; every value is exact.
-template <typename T, int G>
-__device__ __forceinline__ void neg_apply_bit(T* ar, T* ai, uint32_t mm1, uint32_t mm0, uint32_t bit) {
-    const T F = bit ? T(-1) : T(1);
-#pragma unroll
-    for (int m = 0; m < G; ++m) {
-        const uint32_t f1 = (mm1 >> m) & 1u, f0 = (mm0 >> m) & 1u;
-        const T a = f1 == f0 ? T(0) : (f1 ? T(1) : T(-1));
-        const T b = f1 != f0 ? T(0) : (f1 ? T(-1) : T(1));
-        const T f = fmat(a, F, b);
-        ar[m] *= f;
-        ai[m] *= f;
-    }
-}
-
-// Phase family: members in sel1 are multiplied by d1, in sel0 by d0, the
-// others keep their value (factor 1 + 0i, exact).
-template <typename T, int G>
-__device__ __forceinline__ void diag_apply(T* ar, T* ai, uint32_t sel1, uint32_t sel0, const T* g) {
-#pragma unroll
-    for (int m = 0; m < G; ++m) {
-        const bool s1 = (sel1 >> m) & 1u;
-        const bool s0 = (sel0 >> m) & 1u;
-        const T fr = s1 ? g[2] : (s0 ? g[0] : T(1));
-        const T fi = s1 ? g[3] : (s0 ? g[1] : T(0));
-        cmul(ar[m], ai[m], fr, fi);
-    }
-}
-
-__device__ __forceinline__ double fabs_t(double x) { return __builtin_fabs(x); }
-__device__ __forceinline__ float fabs_t(float x) { return __builtin_fabsf(x); }
-__device__ __forceinline__ bool signbit_t(double x) { return __builtin_signbit(x); }
-__device__ __forceinline__ bool signbit_t(float x) { return __builtin_signbitf(x); }
-
-
 // Register copy of the per-sweep part of a TileInstr (parameters in the
 // state's precision): 5 + 8 or 16 scalar registers.
 template <typename T>
@@ -289,64 +170,6 @@ __device__ __forceinline__ InstrRegs<T> load_instr(const TileInstr* __restrict__
 
 __device__ __forceinline__ int sbyte(uint32_t w, int i) { return static_cast<int8_t>((w >> (8 * i)) & 0xffu); }
 __device__ __forceinline__ uint32_t ubyte(uint32_t w, int i) { return (w >> (8 * i)) & 0xffu; }
-
-// T_ROT on slot SLOT: v <- v * (zr + i zi) (the pre-phase, exact when it
-// is 1 or -1), then the scaled real rotation of a factored one-qubit
-// unitary, one FMA per output component: form 0: (u, v) <- (u - t v, v + t u);
-// form 1: (u, v) <- (t u - v, u + t v).
-template <typename T, int G, int SLOT>
-__device__ __forceinline__ void slot_rot(T* ar, T* ai, T t, bool form1, T zr, T zi) {
-    constexpr int lo_mask = (1 << SLOT) - 1;
-    if (!form1) {
-        const T nt = -t;
-#pragma unroll
-        for (int p = 0; p < G / 2; ++p) {
-            const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
-            const int m1 = m0 | (1 << SLOT);
-            const T ur = ar[m0], ui = ai[m0];
-            const T vr = ar[m1] * zr - ai[m1] * zi, vi = ar[m1] * zi + ai[m1] * zr;
-            ar[m0] = fmat(nt, vr, ur);
-            ai[m0] = fmat(nt, vi, ui);
-            ar[m1] = fmat(t, ur, vr);
-            ai[m1] = fmat(t, ui, vi);
-        }
-    } else {
-#pragma unroll
-        for (int p = 0; p < G / 2; ++p) {
-            const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
-            const int m1 = m0 | (1 << SLOT);
-            const T ur = ar[m0], ui = ai[m0];
-            const T vr = ar[m1] * zr - ai[m1] * zi, vi = ar[m1] * zi + ai[m1] * zr;
-            ar[m0] = fmat(t, ur, -vr);
-            ai[m0] = fmat(t, ui, -vi);
-            ar[m1] = fmat(t, vr, ur);
-            ai[m1] = fmat(t, vi, ui);
-        }
-    }
-}
-
-// T_PH on slot SLOT: members whose slot bit is 1 times (zr + i zi).
-template <typename T, int G, int SLOT>
-__device__ __forceinline__ void slot_phase(T* ar, T* ai, T zr, T zi) {
-#pragma unroll
-    for (int m = 0; m < G; ++m)
-        if ((m >> SLOT) & 1) cmul(ar[m], ai[m], zr, zi);
-}
-
-// T_SCALE: every member times (zr + i zi); a real factor takes two products.
-template <typename T, int G>
-__device__ __forceinline__ void scale_all(T* ar, T* ai, T zr, T zi) {
-    if (zi == T(0)) {
-#pragma unroll
-        for (int m = 0; m < G; ++m) {
-            ar[m] *= zr;
-            ai[m] *= zr;
-        }
-    } else {
-#pragma unroll
-        for (int m = 0; m < G; ++m) cmul(ar[m], ai[m], zr, zi);
-    }
-}
 
 // Sweeps of one tile pass on the tile staged in LDS (s_re, s_im): each of
 // the 2^(K-S) groups of 2^S amplitudes spanned by the sweep's S slot bits

@@ -237,6 +237,13 @@ class Plan:
         check(_lib.load().bfly_plan_stats(self._h, ct.byref(p), ct.byref(l), ct.byref(g)))
         return {"passes": p.value, "launches": l.value, "gates": g.value}
 
+    def jit_info(self) -> dict:
+        """Generated tile-pass kernels of a BFLY_FLAG_JIT plan and how many
+        were compiled for it (the rest came from the process cache)."""
+        k, c = ct.c_int64(), ct.c_int64()
+        check(_lib.load().bfly_plan_jit_info(self._h, ct.byref(k), ct.byref(c)))
+        return {"kernels": k.value, "compiled": c.value}
+
     def dump(self) -> str:
         """Text form of the compiled schedule (bfly_plan_dump)."""
         lib = _lib.load()

@@ -117,6 +117,10 @@ typedef struct bfly_exec_opts {
 #define BFLY_FLAG_GENERIC_PAIRS 4 /* fused mode: apply every gate as its 2x2 product (no factoring of
                                      uncontrolled one-qubit gates into phases and a scaled rotation,
                                      no Grover peephole): the reference's per-gate arithmetic */
+#define BFLY_FLAG_JIT       8 /* fused mode: compile every tile pass into its own straight-line kernel
+                                 (hipRTC, at bfly_plan_create; cached per process by pass structure);
+                                 same amplitudes as the interpreting tile kernel.  Worth it for plans
+                                 executed many times: compiling takes about 0.3 s per new pass */
 
 typedef struct bfly_state bfly_state;   /* opaque: device-resident SoA state */
 typedef struct bfly_plan  bfly_plan;    /* opaque: compiled pass schedule    */
@@ -164,6 +168,11 @@ int bfly_plan_stats(const bfly_plan* p, int64_t* passes, int64_t* launches, int6
    (including the NUL) in *needed.  No counterpart in the reference: the
    schedule replaces its per-transformation dispatch. */
 int bfly_plan_dump(const bfly_plan* p, char* buf, uint64_t len, uint64_t* needed);
+/* BFLY_FLAG_JIT plans: tile passes that run as their own generated kernel, and how many of
+   those were compiled when the plan was created (the others came from the process-wide
+   cache).  Plans built without the flag report 0 / 0.  No counterpart in the reference
+   (plan introspection, like bfly_plan_stats). */
+int bfly_plan_jit_info(const bfly_plan* p, int64_t* kernels, int64_t* compiled);
 /* Run a compiled plan on a state (asynchronous unless BFLY_FLAG_SYNC was set). */
 int bfly_plan_execute(bfly_plan* p, bfly_state* s);
 /* ---- autotuner: tile shape and fusion width ------------------------------

@@ -1,0 +1,60 @@
+"""CPU tests of BFLY_FLAG_JIT plans: every tile pass is generated as its own
+kernel and compiled with hipRTC for gfx950 when the plan is created (no GPU
+needed), and a second plan of the same structure takes every kernel from the
+process cache."""
+import numpy as np
+import pytest
+
+from code_mojo_amd import _lib, lower_circuit
+from code_mojo_amd import workloads as W
+from code_mojo_amd.state import Plan
+
+from .helpers import random_ops
+
+
+def _plan(n, dtype, ops, ud, **kw):
+    return Plan(n, dtype, ops, ud, _lib.opts(flags=_lib.FLAG_JIT, **kw))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_jit_plan_compiles_every_tile_pass(dtype):
+    n = 14
+    ops, ud = lower_circuit(W.random_layered(n, 4, 11))
+    p = _plan(n, dtype, ops, ud, tile_bits=10)
+    info = p.jit_info()
+    assert info["kernels"] == p.stats()["passes"] > 1
+    assert info["compiled"] <= info["kernels"]
+
+
+def test_jit_cache_reuses_kernels_of_the_same_structure():
+    n = 13
+    ops, ud = lower_circuit(W.random_layered(n, 3, 5))
+    first = _plan(n, np.float64, ops, ud, tile_bits=9).jit_info()
+    second = _plan(n, np.float64, ops, ud, tile_bits=9).jit_info()
+    assert second["kernels"] == first["kernels"] > 0
+    assert second["compiled"] == 0
+    # new angles, same structure: the generated sources do not depend on the
+    # gate parameters, only on kinds, slots, forms and masks
+    ops2, ud2 = lower_circuit(W.random_layered(n, 3, 6))
+    assert _plan(n, np.float64, ops2, ud2, tile_bits=9).jit_info()["kernels"] > 0
+
+
+@pytest.mark.parametrize("n,tile_bits,k", [(6, 4, 4), (9, 6, 6), (12, 0, 12), (13, 11, 11)])
+def test_jit_handles_every_instruction_form(n, tile_bits, k):
+    # random op lists: controlled pairs, diagonal gates on lane / tile bits,
+    # phase flips, scale records and both sweep widths (threads 2^(k-4):
+    # 16-amplitude sweeps; the default is 8-amplitude sweeps from k = 10)
+    rng = np.random.default_rng(40 + n)
+    ops, ud = random_ops(rng, n, 80)
+    for threads in (0, 1 << (k - 4) if k >= 10 else 64):
+        p = _plan(n, np.float64, ops, ud, tile_bits=tile_bits, threads=threads)
+        assert p.jit_info()["kernels"] == sum(1 for line in p.dump().splitlines() if line.startswith("tile "))
+
+
+def test_plans_without_the_flag_report_no_kernels():
+    n = 10
+    ops, ud = lower_circuit(W.qft_circuit(n))
+    p = Plan(n, np.float64, ops, ud, _lib.opts())
+    assert p.jit_info() == {"kernels": 0, "compiled": 0}
+    pu = Plan(n, np.float64, ops, ud, _lib.opts(mode=_lib.MODE_UNFUSED, flags=_lib.FLAG_JIT))
+    assert pu.jit_info()["kernels"] == 0
