@@ -14,6 +14,7 @@ from .circuit import (Circuit, ClassicalReplacementKind, ClassicalTransformation
                       permute_qubits, replace_quantum_with_classical)
 from .context import ExecContext, ExecutionStrategy, execution_strategy_name
 from .executors import compile_plan, execute, execute_gpu, executor_registry, get_executor
+from .fusion import fuse_circuit, fuse_circuit_specialized_pairs
 from .gates import (Custom_Gate, GateInfo, GateKind, H, P, P_Gate, RX, RX_Gate, RY, RY_Gate, RZ,
                     RZ_Gate, U1, U2, U3, X, Y, Z, H_Gate, X_Gate, Y_Gate, Z_Gate)
 from .lowering import LoweringError, OP_DTYPE, lower_circuit

@@ -96,6 +96,8 @@ _SIGS = {
                                       ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "bfly_engine_wait": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_double]),
     "bfly_engine_fetch": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_void_p, ct.c_void_p, ct.c_uint64]),
+    "bfly_engine_fetch_range": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_uint64, ct.c_uint64, ct.c_void_p,
+                                            ct.c_void_p]),
     "bfly_engine_release": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "bfly_engine_cancel": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "bfly_engine_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64),

@@ -46,6 +46,7 @@ def build(verbose: bool = False) -> Path:
     (out_dir / "tile_ops_src.inc").write_text('R"BFLYSRC(' + ops + ')BFLYSRC"\n')
     objs = []
     cc = hipcc()
+    jobs = []
     for src in SOURCES:
         obj = out_dir / (src + ".o")
         cmd = [cc, *HIPCC_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
@@ -53,10 +54,16 @@ def build(verbose: bool = False) -> Path:
             cmd = [cc, "-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{out_dir}", "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        res = subprocess.run(cmd, capture_output=True, text=True)
-        if res.returncode != 0:
-            raise RuntimeError(f"compile failed for {src}:\n{res.stdout}\n{res.stderr}")
+        jobs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
         objs.append(str(obj))
+    # the translation units compile concurrently (one process each)
+    failed = []
+    for src, proc in jobs:
+        out, err = proc.communicate()
+        if proc.returncode != 0:
+            failed.append(f"compile failed for {src}:\n{out}\n{err}")
+    if failed:
+        raise RuntimeError("\n".join(failed))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp), "-L/opt/rocm/lib", "-lhiprtc",
            "-pthread"]

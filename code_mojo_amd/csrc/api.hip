@@ -7,6 +7,8 @@
 // the device except bfly_state_create and a plan's first execute.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cstdio>
 #include <algorithm>
 #include <chrono>
@@ -303,18 +305,32 @@ int bfly_state_copy(bfly_state* dst, const bfly_state* src) {
     int rc = check_state(dst);
     if (rc) return rc;
     if ((rc = check_state(src))) return rc;
+    if (dst == src) return fail(BFLY_EINVAL, "bfly_state_copy: source and destination are the same state");
     if (dst->n != src->n) return fail(BFLY_EINVAL, "state shape mismatch");
+    if (dst->device != src->device) return fail(BFLY_EINVAL, "bfly_state_copy: states live on different devices");
     HIP_TRY(hipSetDevice(dst->device), "hipSetDevice");
-    HIP_TRY(hipStreamSynchronize(src->stream), "hipStreamSynchronize");
-    if (dst->dtype != src->dtype) {
-        // precision conversion (complex64 <-> complex128), element by element
-        HIP_TRY(launch_convert(src->dtype, src->re, src->im, dst->re, dst->im, 1ull << dst->n, dst->stream),
-                "convert launch");
-        return BFLY_OK;
+    // order: src's pending work -> the copy on dst's stream -> src's later work
+    // (which may overwrite what the copy still reads)
+    hipEvent_t before = nullptr, after = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&before, hipEventDisableTiming), "hipEventCreateWithFlags");
+    hipError_t e = hipEventCreateWithFlags(&after, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(before, src->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(dst->stream, before, 0);
+    if (e == hipSuccess) {
+        if (dst->dtype != src->dtype) {
+            // precision conversion (complex64 <-> complex128), element by element
+            e = launch_convert(src->dtype, src->re, src->im, dst->re, dst->im, 1ull << dst->n, dst->stream);
+        } else {
+            const size_t bytes = (1ull << dst->n) * elem_size(dst->dtype);
+            e = hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, dst->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, dst->stream);
+        }
     }
-    const size_t bytes = (1ull << dst->n) * elem_size(dst->dtype);
-    HIP_TRY(hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy re)");
-    HIP_TRY(hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, dst->stream), "hipMemcpyAsync(copy im)");
+    if (e == hipSuccess) e = hipEventRecord(after, dst->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(src->stream, after, 0);
+    (void)hipEventDestroy(before);
+    if (after) (void)hipEventDestroy(after);
+    if (e != hipSuccess) return hip_fail(e, "bfly_state_copy");
     return BFLY_OK;
 }
 
@@ -978,7 +994,16 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
             e = hipMemcpy(b->d_ops, b->ops.data(), b->ops.size() * sizeof(OpItem), hipMemcpyHostToDevice);
         if (e != hipSuccess) { free_batch(b); return hip_fail(e, "batch op items"); }
     }
-    if (flags & BFLY_BATCH_GRAPH) {
+    // Under a rocprofiler-sdk tool (rocprofv3 sets ROCP_TOOL_LIBRARIES for
+    // the program it runs) a batch launches its kernels directly on its
+    // stream instead of replaying a captured graph: a kernel-trace run of a
+    // graph batch of about 1,600 rounds ended in a host-side SIGSEGV inside
+    // hipGraphLaunch, and no fault was found in this graph's construction
+    // (kernel arguments are captured by value, the item buffers live as long
+    // as the batch, nothing but kernels is captured).  The direct launches are
+    // the same kernels in the same order.
+    const bool profiled = std::getenv("ROCP_TOOL_LIBRARIES") != nullptr;
+    if ((flags & BFLY_BATCH_GRAPH) && !profiled) {
         e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
         if (e == hipSuccess) {
             hipError_t e2 = enqueue_batch(b);

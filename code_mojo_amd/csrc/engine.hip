@@ -53,6 +53,7 @@ struct Request {
     bool large = false;
     bool issued_all = false;
     bool cancel = false;
+    int64_t touched = -1;          // last issue attempt that enqueued work for this request
     std::string message;
     double t_submit = 0, t_admit = 0, t_done = 0;
 };
@@ -89,6 +90,7 @@ struct bfly_engine {
     std::map<std::pair<int, int>, std::vector<PoolEntry>> pool;
     uint64_t pool_bytes = 0;
     int64_t rounds = 0, launches = 0, completed = 0, admitted = 0;
+    int64_t attempts = 0;        // issue_round calls (marks the requests each one touched)
     // ring of item slots: host (pinned) -> device copy per round
     int nslots = 0;
     int large_running = 0;
@@ -142,6 +144,21 @@ void free_request_resources(bfly_engine* e, Request* r) {
     }
     std::vector<uint8_t>().swap(r->init_re);
     std::vector<uint8_t>().swap(r->init_im);
+}
+
+// A failed request's state may still be written by launches queued before
+// the failure: wait for its stream, then free the state instead of pooling it.
+void discard_request_resources(bfly_engine* e, Request* r) {
+    if (r->state) {
+        (void)hipStreamSynchronize(r->state->stream);
+        bfly_state_destroy(r->state);
+        if (r->plan_buf) (void)hipFree(r->plan_buf);
+        e->bytes_in_use -= r->bytes;
+        r->state = nullptr;
+        r->plan_buf = nullptr;
+        r->plan_cap = 0;
+    }
+    free_request_resources(e, r);
 }
 
 void destroy_entry(PoolEntry& pe) {
@@ -282,6 +299,7 @@ void admit_pending(bfly_engine* e) {
 int issue_round(bfly_engine* e) {
     std::vector<Request*> small;
     const size_t cap = kSlotItems * 4;  // items per ring slot (each kind)
+    const int64_t attempt = ++e->attempts;
     for (uint64_t id : e->running) {
         Request* r = e->reqs[id].get();
         if (r->issued_all) continue;
@@ -293,6 +311,7 @@ int issue_round(bfly_engine* e) {
             continue;
         }
         if (r->large) {
+            r->touched = attempt;
             for (; r->next_step < steps.size(); ++r->next_step) {
                 hipError_t err = launch_step(steps[r->next_step], r->state, r->plan, r->state->stream);
                 if (err != hipSuccess) return hip_fail(err, "large request launch");
@@ -427,6 +446,7 @@ int issue_round(bfly_engine* e) {
             acts.push_back(a);
         }
     }
+    for (Request* r : small) r->touched = attempt;   // this round's launches carry their steps
     if (oused)
         HIP_TRY(hipMemcpyAsync(dop, ho, oused * sizeof(OpItem), hipMemcpyHostToDevice, e->stream), "round op items");
     if (used)
@@ -554,14 +574,17 @@ void worker_loop(bfly_engine* e) {
         const int64_t before = e->rounds;
         int rc = issue_round(e);
         if (rc != BFLY_OK) {
-            // fail every request that had work in flight in this round
+            // fail the requests the failed round enqueued work for (the others
+            // keep running); their states are discarded, not pooled
             const std::string msg = bfly_last_error();
             (void)hipStreamSynchronize(e->stream);
             for (uint64_t id : e->running) {
                 Request* r = e->reqs[id].get();
+                if (r->touched != e->attempts) continue;
                 if (r->large && r->issued_all) continue;
                 r->message = msg;
-                free_request_resources(e, r);
+                if (r->large) e->large_running -= 1;
+                discard_request_resources(e, r);
                 finish(e, r, BFLY_REQ_FAILED);
             }
             e->running.erase(std::remove_if(e->running.begin(), e->running.end(),
@@ -700,6 +723,32 @@ int bfly_engine_fetch(bfly_engine* e, uint64_t id, void* re, void* im, uint64_t 
     free_request_resources(e, r);
     e->reqs.erase(it);
     e->cv_work.notify_all();
+    return BFLY_OK;
+}
+
+int bfly_engine_fetch_range(bfly_engine* e, uint64_t id, uint64_t first, uint64_t count, void* re, void* im) {
+    if (!e) return fail(BFLY_EINVAL, "null engine");
+    if (count && (!re || !im)) return fail(BFLY_EINVAL, "null output buffer");
+    std::unique_lock<std::mutex> lk(e->mu);
+    auto it = e->reqs.find(id);
+    if (it == e->reqs.end()) return fail(BFLY_EINVAL, "unknown request " + std::to_string(id));
+    Request* r = it->second.get();
+    if (r->status != BFLY_REQ_DONE) return fail(BFLY_ESTATE, "request " + std::to_string(id) + " is not done");
+    const uint64_t size = 1ull << r->n;
+    if (first > size || count > size - first)
+        return fail(BFLY_EINVAL, "amplitude range [" + std::to_string(first) + ", +" + std::to_string(count) +
+                                     ") outside a state of " + std::to_string(size));
+    if (!count) return BFLY_OK;
+    bfly_state* st = r->state;
+    const size_t esz = st->dtype == BFLY_C64 ? sizeof(float) : sizeof(double);
+    HIP_TRY(hipSetDevice(st->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(re, static_cast<char*>(st->re) + first * esz, count * esz, hipMemcpyDeviceToHost,
+                           st->stream),
+            "hipMemcpyAsync(range re)");
+    HIP_TRY(hipMemcpyAsync(im, static_cast<char*>(st->im) + first * esz, count * esz, hipMemcpyDeviceToHost,
+                           st->stream),
+            "hipMemcpyAsync(range im)");
+    HIP_TRY(hipStreamSynchronize(st->stream), "hipStreamSynchronize");
     return BFLY_OK;
 }
 

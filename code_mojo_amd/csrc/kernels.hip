@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "kernels.h"
 #include "tile_ops.h"
@@ -491,21 +492,37 @@ __global__ __launch_bounds__(tile_threads(K, S), batch_waves_per_eu(sizeof(T), K
 
 
 // Kernel attributes (dynamic LDS beyond 64 KiB) and the persistent grid size
-// (blocks that fit on the device at once) for one kernel instance.
+// (blocks that fit on the device at once) for one kernel instance, cached per
+// device.  The cache entries are atomics: threads launching on separate
+// handles may race to fill an entry and compute the same value, which is
+// harmless; a grid sized for one device is never reused on another.
+constexpr int kGridCacheDevices = 64;
+struct GridCache {
+    std::atomic<int> cap[kGridCacheDevices] = {};
+};
+
 template <typename KernelT>
-hipError_t persistent_grid(KernelT kernel, int nt, size_t lds, int* grid_cap) {
-    if (*grid_cap > 0) return hipSuccess;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+hipError_t persistent_grid(KernelT kernel, int nt, size_t lds, GridCache& cache, int* grid_cap) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    int per_cu = 0, dev = 0, cus = 0;
+    if (dev < 0 || dev >= kGridCacheDevices) return hipErrorInvalidDevice;
+    const int cached = cache.cap[dev].load(std::memory_order_acquire);
+    if (cached > 0) {
+        *grid_cap = cached;
+        return hipSuccess;
+    }
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    int per_cu = 0, cus = 0;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, lds);
-    if (e != hipSuccess) return e;
-    e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    *grid_cap = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    const int cap = (per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
+    cache.cap[dev].store(cap, std::memory_order_release);
+    *grid_cap = cap;
     return hipSuccess;
 }
 
@@ -513,8 +530,9 @@ template <typename T, int K, int S>
 hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, const TileInstr* in, hipStream_t st) {
     constexpr int NT = tile_threads(K, S);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(tile_pass_kernel<T, K, S>, NT, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(tile_pass_kernel<T, K, S>, NT, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     if (!re) return hipSuccess;  // preparation only
     const uint64_t ntiles = 1ull << (P.n - P.k);
@@ -527,8 +545,9 @@ template <typename T, int K, int S>
 hipError_t launch_batch_k(const BatchItem* items, int nitems, uint64_t total_tiles, hipStream_t st) {
     constexpr int NT = tile_threads(K, S);
     const size_t lds = 2ull * (1ull << K) * sizeof(T);
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(tile_batch_kernel<T, K, S>, NT, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(tile_batch_kernel<T, K, S>, NT, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     if (!items) return hipSuccess;  // preparation only
     const unsigned grid =
@@ -1065,8 +1084,9 @@ template <int K>
 hipError_t launch_block64_k(double* re, double* im, const TilePass& P, const BlockPass& B, const double* mat,
                             const double* tab, const double* norm2, hipStream_t st) {
     const size_t lds = 2ull * (1ull << K) * sizeof(double) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 2 * 6 * 16;
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(block_mfma64_kernel<K>, 256, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(block_mfma64_kernel<K>, 256, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
@@ -1335,8 +1355,9 @@ __global__ __launch_bounds__(512) void block_ws_kernel(float* __restrict__ re, f
 hipError_t launch_block_ws(float* re, float* im, const TilePass& P, const BlockPass& B, const float* mat,
                            const double* tab, const double* norm2, hipStream_t st) {
     const size_t lds = ws_lds_bytes();
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(block_ws_kernel, 512, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(block_ws_kernel, 512, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
@@ -1588,8 +1609,9 @@ __global__ __launch_bounds__(512) void block_ws64_kernel(double* __restrict__ re
 hipError_t launch_block_ws64(double* re, double* im, const TilePass& P, const BlockPass& B, const double* mat,
                              const double* tab, const double* norm2, hipStream_t st) {
     const size_t lds = ws64_lds_bytes();
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(block_ws64_kernel, 512, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(block_ws64_kernel, 512, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);
@@ -1603,8 +1625,9 @@ hipError_t launch_block_k(float* re, float* im, const TilePass& P, const BlockPa
     // the tile (re, im), the pre / post column factors (6 complex FP64 per
     // column) and two buffers of the tile's 2 x 6 factors
     const size_t lds = 2ull * (1ull << K) * sizeof(float) + 2ull * ((1ull << K) / kBlockDim) * 6 * 16 + 2 * 2 * 6 * 16;
-    static int grid_cap = 0;
-    hipError_t e = persistent_grid(block_mfma_kernel<K>, 256, lds, &grid_cap);
+    static GridCache grid_cache;
+    int grid_cap = 0;
+    hipError_t e = persistent_grid(block_mfma_kernel<K>, 256, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     const uint64_t ntiles = 1ull << (P.n - P.k);
     const unsigned grid = static_cast<unsigned>(ntiles < static_cast<uint64_t>(grid_cap) ? ntiles : grid_cap);

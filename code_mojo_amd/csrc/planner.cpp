@@ -367,6 +367,87 @@ std::vector<Micro> micro_ops(const std::vector<PendingGate>& gates, bool factor,
     return out;
 }
 
+// -- small dense unitaries as gates ---------------------------------------
+// Fused mode runs a UNITARY op as a pass of its own (unitary_kernel).  The
+// reference's fusion rewrites emit two kinds that are plain gates in
+// disguise (transformations_fusion.mojo:334-384,539-660): m = 1 blocks (a run
+// of gates on one target, with or without one control) and m = 2 blocks of
+// uncontrolled gates on adjacent qubits, which are Kronecker products
+// G_hi (x) G_lo.  Both become 2x2 gate records that join the tile passes: an
+// m = 1 block directly, an m = 2 block when it factors to 1e-14 (a controlled
+// G_hi (x) G_lo is controlled-G_hi times controlled-G_lo, so the controls carry
+// over).  Other blocks stay stand-alone steps.
+bfly_op gate_from_2x2(const std::complex<double>* g, int target, uint64_t controls) {
+    bfly_op o{};
+    o.kind = BFLY_OP_GATE;
+    o.gate = BFLY_GATE_CUSTOM;
+    o.target = target;
+    o.controls = controls;
+    for (int i = 0; i < 4; ++i) {
+        o.mat[2 * i] = g[i].real();
+        o.mat[2 * i + 1] = g[i].imag();
+    }
+    return o;
+}
+
+// U (4 x 4, row-major, index b0 + 2 b1) = A (x) B with A on b1, B on b0?
+bool factor_kron(const std::complex<double>* U, std::complex<double>* A, std::complex<double>* B) {
+    using C = std::complex<double>;
+    auto blk = [&](int i1, int j1, int r0, int c0) { return U[(r0 + 2 * i1) * 4 + (c0 + 2 * j1)]; };
+    int bi = 0, bj = 0;
+    double best = -1.0;
+    for (int i1 = 0; i1 < 2; ++i1)
+        for (int j1 = 0; j1 < 2; ++j1) {
+            double f = 0;
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 2; ++c) f += std::norm(blk(i1, j1, r, c));
+            if (f > best) { best = f; bi = i1; bj = j1; }
+        }
+    if (!(best > 0)) return false;
+    const double scale = std::sqrt(2.0 / best);   // a unitary 2x2 has Frobenius norm sqrt(2)
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) B[r * 2 + c] = blk(bi, bj, r, c) * scale;
+    for (int i1 = 0; i1 < 2; ++i1)
+        for (int j1 = 0; j1 < 2; ++j1) {
+            C t = 0;   // tr(B^H U_blk) / 2
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 2; ++c) t += std::conj(B[r * 2 + c]) * blk(i1, j1, r, c);
+            A[i1 * 2 + j1] = t * 0.5;
+        }
+    for (int i1 = 0; i1 < 2; ++i1)
+        for (int j1 = 0; j1 < 2; ++j1)
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 2; ++c)
+                    if (std::abs(A[i1 * 2 + j1] * B[r * 2 + c] - blk(i1, j1, r, c)) > 1e-14) return false;
+    return true;
+}
+
+std::vector<bfly_op> unitaries_as_gates(const bfly_op* ops, uint64_t nops, const double* udata, bool& changed) {
+    std::vector<bfly_op> out;
+    out.reserve(nops + 8);
+    changed = false;
+    for (uint64_t i = 0; i < nops; ++i) {
+        const bfly_op& op = ops[i];
+        if (op.kind == BFLY_OP_UNITARY && (op.arg_i == 1 || op.arg_i == 2)) {
+            const auto* u = reinterpret_cast<const std::complex<double>*>(udata + 2 * op.aux);
+            if (op.arg_i == 1) {
+                out.push_back(gate_from_2x2(u, op.target, op.controls));
+                changed = true;
+                continue;
+            }
+            std::complex<double> A[4], B[4];
+            if (factor_kron(u, A, B)) {
+                out.push_back(gate_from_2x2(B, op.target, op.controls));
+                out.push_back(gate_from_2x2(A, op.target + 1, op.controls));
+                changed = true;
+                continue;
+            }
+        }
+        out.push_back(op);
+    }
+    return out;
+}
+
 // -- Grover peephole -------------------------------------------------------
 // Recognises, in a gate list, the sequences that the reference's classical
 // replacement rules recognise (grover_diffuser_replacement_rule /
@@ -1443,7 +1524,13 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
     for (uint64_t i = 0; i < nops; ++i)
         if (ops[i].kind == BFLY_OP_GATE) plan.gates += 1;
     const bool generic = (opts.flags & BFLY_FLAG_GENERIC_PAIRS) != 0;
-    std::vector<bfly_op> rewritten;
+    std::vector<bfly_op> as_gates, rewritten;
+    bool small_unitaries = false;
+    if (udata) as_gates = unitaries_as_gates(ops, nops, udata, small_unitaries);
+    if (small_unitaries) {   // plan.gates keeps counting the input's gate ops
+        ops = as_gates.data();
+        nops = as_gates.size();
+    }
     if (!generic) {
         rewritten = rewrite_grover(ops, nops, n, n > k);
         ops = rewritten.data();

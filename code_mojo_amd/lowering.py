@@ -136,8 +136,9 @@ def _lower_gate(out: OpList, g: C.GateTransformation) -> None:
     out.gate(g.target, g.controls, g.gate_info.kind, g.gate_info.gate, theta)
 
 
-def _lower_permute(out: OpList, order: List[int]) -> None:
-    """apply_permute_qubits (state.mojo:393-431): selection of swaps."""
+def _lower_permute(out: OpList, order: List[int], inverse: bool = False) -> None:
+    """apply_permute_qubits (state.mojo:393-431): selection of swaps; with
+    `inverse` the same swaps in reverse order (the inverse permutation)."""
     n = len(order)
     if n == 0:
         return
@@ -151,6 +152,7 @@ def _lower_permute(out: OpList, order: List[int]) -> None:
             raise LoweringError(f"Duplicate permutation value: {v}")
         seen[v] = True
     perm = list(range(n))
+    swaps = []
     for i in range(n):
         if perm[i] == order[i]:
             continue
@@ -159,8 +161,10 @@ def _lower_permute(out: OpList, order: List[int]) -> None:
             j += 1
         if j >= n:
             raise LoweringError("Invalid permutation order")
-        out.swap(i, j)
+        swaps.append((i, j))
         perm[i], perm[j] = perm[j], perm[i]
+    for a, b in (reversed(swaps) if inverse else swaps):
+        out.swap(a, b)
 
 
 def _lower_sign_flip(out: OpList, indices: List[int]) -> None:
@@ -250,7 +254,15 @@ def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None,
                     raise LoweringError("Control overlaps target range")
             out.unitary(tr.u, tr.target, tr.m, ctrls)
         elif isinstance(tr, C.ClassicalTransformation):
-            name = tr.name[:-4] if tr.name.endswith("_inv") else tr.name
+            # NAME_inv: the reference renames a classical transform on inversion
+            # when it carries an inverse_apply (circuit.mojo:1000-1010); BITREV,
+            # SWAP, sign flips and inversion-0 are their own inverses, PERMUTE's
+            # inverse is its swap sequence reversed
+            inverse = tr.name.endswith("_inv")
+            name = tr.name[:-4] if inverse else tr.name
+            if inverse and name == "GROVER_DIFFUSER":
+                raise LoweringError("GROVER_DIFFUSER_inv: the classical diffuser (im <- -2 mean_im - im) is not "
+                                    "its own inverse and has no GPU lowering")
             if name == "BITREV":
                 mask = 0
                 for q in tr.targets:
@@ -261,7 +273,7 @@ def lower_circuit(circuit: C.Circuit, num_qubits: int | None = None,
                 elif len(tr.targets) > 1:
                     out.bitrev(mask)
             elif name == "PERMUTE":
-                _lower_permute(out, list(tr.targets))
+                _lower_permute(out, list(tr.targets), inverse)
             elif name == "SWAP":
                 if len(tr.targets) != 2:
                     raise LoweringError("SWAP expects 2 targets")

@@ -105,6 +105,14 @@ class Engine:
         check(_lib.load().bfly_engine_fetch(self._h, rid, out.re.ctypes.data, out.im.ctypes.data, out.size()))
         return out
 
+    def fetch_range(self, rid: int, first: int, count: int, dtype) -> np.ndarray:
+        """Amplitudes [first, first + count) of a finished request, as complex
+        numbers (the request stays until fetched or released)."""
+        re = np.empty(count, dtype=dtype)
+        im = np.empty(count, dtype=dtype)
+        check(_lib.load().bfly_engine_fetch_range(self._h, rid, first, count, re.ctypes.data, im.ctypes.data))
+        return re.astype(np.float64) + 1j * im.astype(np.float64)
+
     def release(self, rid: int) -> None:
         check(_lib.load().bfly_engine_release(self._h, rid))
 

@@ -143,9 +143,11 @@ int bfly_state_reset(bfly_state* s);
 /* Copy host SoA arrays (len = 2^n elements of the state's dtype) in / out. */
 int bfly_state_upload(bfly_state* s, const void* re, const void* im, uint64_t len);
 int bfly_state_download(bfly_state* s, void* re, void* im, uint64_t len);
-/* Copy one state into another of the same shape and dtype (device to device). */
-int bfly_state_copy(bfly_state* dst, const bfly_state* src);   /* same qubit count; converts
-                                                                  complex64 <-> complex128 */
+/* Copy one state into another of the same qubit count on the same device (device to
+   device; converts complex64 <-> complex128).  Asynchronous on dst's stream, ordered after
+   src's pending work; src's later work waits for the copy, so the copy always reads src as
+   it was at the call.  dst == src is rejected. */
+int bfly_state_copy(bfly_state* dst, const bfly_state* src);
 int bfly_state_info(const bfly_state* s, int32_t* num_qubits, int32_t* dtype);
 /* Device pointers of the re / im arrays (for callers that share memory). */
 int bfly_state_device_ptrs(bfly_state* s, void** re, void** im);
@@ -213,7 +215,9 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
    size are one persistent launch.  States and plans are borrowed and must
    outlive the batch; each state gets its own plan. */
 typedef struct bfly_batch bfly_batch;
-#define BFLY_BATCH_GRAPH 1    /* capture every round into one hipGraph, replayed by bfly_batch_launch */
+#define BFLY_BATCH_GRAPH 1    /* capture every round into one hipGraph, replayed by bfly_batch_launch
+                                 (not under a rocprofiler-sdk tool, e.g. rocprofv3: there the same
+                                 kernels launch directly on the batch's stream) */
 int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* const* plans, int32_t count,
                       int32_t flags);
 /* Enqueue all rounds (ordered after pending work on every state's stream;
@@ -279,6 +283,11 @@ int bfly_engine_status(bfly_engine* e, uint64_t id, int32_t* status, double* t_s
 int bfly_engine_wait(bfly_engine* e, uint64_t id, double timeout_s);
 /* Copy a DONE request's final state to the host and free its HBM. */
 int bfly_engine_fetch(bfly_engine* e, uint64_t id, void* re, void* im, uint64_t len);
+/* Copy amplitudes [first, first + count) of a DONE request's state to the host (re, im:
+   count elements of the request's precision each); the request stays in the engine until
+   fetched or released.  For results too large to copy whole (a 33-qubit complex128 state is
+   128 GiB).  No counterpart in the reference (its states live in host memory). */
+int bfly_engine_fetch_range(bfly_engine* e, uint64_t id, uint64_t first, uint64_t count, void* re, void* im);
 /* Free a finished request without copying it out. */
 int bfly_engine_release(bfly_engine* e, uint64_t id);
 /* Queued: dropped at once; running: no further steps are issued. */

@@ -101,6 +101,37 @@ def test_lowering_permute_is_swap_sequence():
         assert all(((src[i] >> order[q]) & 1) == ((i >> q) & 1) for q in range(4))
 
 
+def test_inverse_of_a_permute_with_inverse_apply_is_the_inverse_permutation():
+    # a PERMUTE carrying an inverse_apply becomes PERMUTE_inv on inversion
+    # (circuit.mojo:1000-1010); for a 3-cycle (not its own inverse) the
+    # lowered inverse must undo the forward permutation
+    from code_mojo_amd.circuit import ClassicalTransformation
+    from oracle import oracle as O
+
+    def simulate(n, ops, ud, psi):
+        re, im = psi.real.copy(), psi.imag.copy()
+        O.execute(n, re, im, ops, ud)
+        return re + 1j * im
+    n = 3
+    order = [1, 2, 0]
+    qc = QuantumCircuit(n)
+    qc.transformations.append(ClassicalTransformation("PERMUTE", order, apply=None, inverse_apply=lambda *a: None))
+    inv = qc.inverse()
+    assert inv.transformations[0].name == "PERMUTE_inv"
+    fwd_ops, _ = lower_circuit(qc)
+    inv_ops, _ = lower_circuit(inv)
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=8) + 1j * rng.normal(size=8)
+    after_fwd = simulate(n, fwd_ops, np.zeros(0), psi)
+    assert not np.allclose(simulate(n, fwd_ops, np.zeros(0), after_fwd), psi)   # the 3-cycle is no involution
+    assert np.allclose(simulate(n, inv_ops, np.zeros(0), after_fwd), psi, atol=0)
+    # the classical diffuser has no known inverse
+    qd = QuantumCircuit(n)
+    qd.transformations.append(ClassicalTransformation("GROVER_DIFFUSER", [0, 1, 2], inverse_apply=lambda *a: None))
+    with pytest.raises(LoweringError):
+        lower_circuit(qd.inverse())
+
+
 def test_lowering_errors():
     qc = QuantumCircuit(3)
     qc.h(5)
