@@ -110,6 +110,8 @@ _SIGS = {
     "bfly_swap_qubits": (ct.c_int, [ct.c_void_p, ct.c_int32, ct.c_int32]),
     "bfly_norm2": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_double)]),
     "bfly_prob_mask": (ct.c_int, [ct.c_void_p, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "bfly_prob_mask_noalloc": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_uint64, ct.c_uint64,
+                                           ct.c_void_p, ct.c_void_p]),
     "bfly_marginal": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_int32), ct.c_int32, ct.POINTER(ct.c_double)]),
     "bfly_inner": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
     "bfly_diff_norm2": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double)]),

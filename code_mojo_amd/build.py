@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libbfly_hip.so"
 ARCH = os.environ.get("BFLY_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "api.hip", "engine.hip", "planner.cpp", "qasm.cpp", "jit.cpp"]
+SOURCES = ["kernels.hip", "api.hip", "engine.hip", "noalloc.hip", "planner.cpp", "qasm.cpp", "jit.cpp"]
 
 # Packed-FP32 instructions are disabled for every device pass (the host pass
 # warns that it does not know the feature; that is harmless).

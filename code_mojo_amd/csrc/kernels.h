@@ -131,6 +131,12 @@ size_t reduce_scratch_doubles();
 hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre,
                          const void* bim, uint64_t size, uint64_t mask, uint64_t value,
                          double* scratch, hipStream_t stream);
+// The same reduction with the per-block partials in `partial` (2 * channels *
+// reduce_blocks(size) doubles) and the result(s) written to `out` (device).
+hipError_t launch_reduce_into(int dtype, int mode, const void* are, const void* aim, const void* bre, const void* bim,
+                              uint64_t size, uint64_t mask, uint64_t value, double* partial, double* out,
+                              hipStream_t stream);
+unsigned reduce_blocks(uint64_t size);
 // marginal distribution over m <= 6 target bits; 2^m results in scratch[0..)
 hipError_t launch_marginal(int dtype, const void* re, const void* im, uint64_t size, const int* targets,
                            int m, double* scratch, hipStream_t stream);

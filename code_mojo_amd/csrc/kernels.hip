@@ -2717,35 +2717,43 @@ size_t reduce_scratch_doubles() {
 namespace {
 template <typename T, int MODE>
 void launch_reduce_t(const void* are, const void* aim, const void* bre, const void* bim, uint64_t size, uint64_t mask,
-                     uint64_t value, double* scratch, unsigned nb, hipStream_t st) {
+                     uint64_t value, double* partial, unsigned nb, hipStream_t st) {
     hipLaunchKernelGGL((reduce_kernel<T, MODE>), dim3(nb), dim3(kRedThreads), 0, st, static_cast<const T*>(are),
                        static_cast<const T*>(aim), static_cast<const T*>(bre), static_cast<const T*>(bim), size, mask,
-                       value, scratch + 8);
+                       value, partial);
 }
 }  // namespace
 
-hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre, const void* bim,
-                         uint64_t size, uint64_t mask, uint64_t value, double* scratch, hipStream_t stream) {
+unsigned reduce_blocks(uint64_t size) { return grid_for(size, kRedThreads * 4, kRedBlocks); }
+
+hipError_t launch_reduce_into(int dtype, int mode, const void* are, const void* aim, const void* bre, const void* bim,
+                              uint64_t size, uint64_t mask, uint64_t value, double* partial, double* out,
+                              hipStream_t stream) {
     // the block count depends only on the size: same size -> same tree -> same bits
-    const unsigned nb = grid_for(size, kRedThreads * 4, kRedBlocks);
+    const unsigned nb = reduce_blocks(size);
     const bool f = dtype == BFLY_C64;
     switch (mode) {
-        case R_NORM2: f ? launch_reduce_t<float, R_NORM2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
-                        : launch_reduce_t<double, R_NORM2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
-        case R_PROB_MASK: f ? launch_reduce_t<float, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
-                            : launch_reduce_t<double, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
-        case R_DIFF2: f ? launch_reduce_t<float, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
-                        : launch_reduce_t<double, R_DIFF2>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
-        case R_INNER: f ? launch_reduce_t<float, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream)
-                        : launch_reduce_t<double, R_INNER>(are, aim, bre, bim, size, mask, value, scratch, nb, stream); break;
+        case R_NORM2: f ? launch_reduce_t<float, R_NORM2>(are, aim, bre, bim, size, mask, value, partial, nb, stream)
+                        : launch_reduce_t<double, R_NORM2>(are, aim, bre, bim, size, mask, value, partial, nb, stream); break;
+        case R_PROB_MASK: f ? launch_reduce_t<float, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, partial, nb, stream)
+                            : launch_reduce_t<double, R_PROB_MASK>(are, aim, bre, bim, size, mask, value, partial, nb, stream); break;
+        case R_DIFF2: f ? launch_reduce_t<float, R_DIFF2>(are, aim, bre, bim, size, mask, value, partial, nb, stream)
+                        : launch_reduce_t<double, R_DIFF2>(are, aim, bre, bim, size, mask, value, partial, nb, stream); break;
+        case R_INNER: f ? launch_reduce_t<float, R_INNER>(are, aim, bre, bim, size, mask, value, partial, nb, stream)
+                        : launch_reduce_t<double, R_INNER>(are, aim, bre, bim, size, mask, value, partial, nb, stream); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nch = mode == R_INNER ? 2 : 1;
-    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, scratch + 8, static_cast<int>(nb), nch,
-                       scratch);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(kRedThreads), 0, stream, partial, static_cast<int>(nb), nch,
+                       out);
     return hipGetLastError();
+}
+
+hipError_t launch_reduce(int dtype, int mode, const void* are, const void* aim, const void* bre, const void* bim,
+                         uint64_t size, uint64_t mask, uint64_t value, double* scratch, hipStream_t stream) {
+    return launch_reduce_into(dtype, mode, are, aim, bre, bim, size, mask, value, scratch + 8, scratch, stream);
 }
 
 OpItem flip_item(void* re, void* im, int n, uint64_t mask, uint64_t value) {

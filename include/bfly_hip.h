@@ -312,6 +312,18 @@ int bfly_swap_qubits(bfly_state* s, int32_t a, int32_t b);
         (benchmark_verify.mojo:15-67); FP64 compensated, fixed order ---- */
 int bfly_norm2(bfly_state* s, double* out);                       /* sum |a|^2 */
 int bfly_prob_mask(bfly_state* s, uint64_t mask, uint64_t value, double* out); /* sum over (i & mask) == value */
+/* Allocation-free variant of bfly_prob_mask.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  re, im: device arrays of 2^num_qubits amplitudes
+   of `dtype`; out: a DEVICE address receiving one double, sum |a_i|^2 over (i & mask) ==
+   value (the same kernels and bits as bfly_prob_mask); stream: a hipStream_t (NULL = the
+   null stream).  The call enqueues two kernels and returns: no hipMalloc / hipFree, no
+   memset, no host synchronisation.  The per-block partials live in a static device array of
+   2048 doubles (16 KiB) that each GPU holds until the process ends; every call on a GPU
+   shares it, so calls that overlap on one GPU race.  The first call on a device looks up
+   that array's address (hipGetSymbolAddress; no allocation).  num_qubits 1..40. */
+int bfly_prob_mask_noalloc(const void* re, const void* im, int32_t num_qubits, int32_t dtype, uint64_t mask,
+                           uint64_t value, double* out, void* stream);
 int bfly_marginal(bfly_state* s, const int32_t* targets, int32_t m, double* out); /* 2^m probabilities */
 int bfly_inner(bfly_state* a, bfly_state* b, double* re, double* im);          /* <a|b> */
 int bfly_diff_norm2(bfly_state* a, bfly_state* b, double* out);                /* sum |a-b|^2 */
