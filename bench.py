@@ -36,9 +36,24 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, layers: int, seed: int, budget_s: float, threads: int) -> dict:
+def cpu_info() -> dict:
+    """Host description for the CPU baseline: logical CPUs, the CPU model."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(n: int, layers: int, seed: int, threads: int) -> dict:
     """GRID_PARALLEL_FUSED restatement on the host cores: the first `layers`
-    layers of the same circuit family at `n` qubits (bounded sample)."""
+    layers of the headline circuit (same generator and seed) at `n` qubits,
+    with the reference's default grid (slack 3: 8 rows) and with slack
+    log2(threads) (rows for every thread, the reference's own knob)."""
     from code_mojo_amd import lower_circuit
     from code_mojo_amd import workloads as W
     from oracle import oracle as O
@@ -49,35 +64,31 @@ def cpu_baseline(n: int, layers: int, seed: int, budget_s: float, threads: int) 
     re = np.zeros(1 << n)
     im = np.zeros(1 << n)
     re[0] = 1
-    # faithful slack (8 rows) and tuned slack (log2 threads)
     res = {}
-    for label, slack in (("tuned", max(3, int(np.log2(max(threads, 1))))), ("faithful", 3)):
+    for label, slack in (("faithful_slack3", 3), ("tuned_slack_log2_threads", max(3, int(np.log2(max(threads, 1)))))):
         t0 = time.perf_counter()
         O.grid_fused_execute(n, re, im, ops, slack=slack, threads=threads)
-        dt = time.perf_counter() - t0
-        res[label] = dt
-        if time.perf_counter() - t0 > budget_s:
-            break
+        res[label] = time.perf_counter() - t0
     best = min(res.values())
-    # every gate of the grid strategy reads and writes the whole state once
-    # (a row-local group still walks each row once per gate; rows of 2^(n-3)
-    # amplitudes do not stay in cache)
-    sweeps = len(ops)
     state_bytes = 2 * (1 << n) * 16
     return {
-        "value": round(sweeps * state_bytes / best / 1e9, 3),
-        "unit": "GB/s",
+        # gate-equivalent state traffic: one state read + write per gate (the
+        # grid strategy walks every row once per gate)
+        "value": round(len(ops) * state_bytes / best / 1e9, 3),
+        "unit": "GB/s (gate-equivalent: one state read and write per gate)",
         "cores": threads,
         "kind": "port",
-        "sample": f"GRID_PARALLEL_FUSED C/OpenMP restatement, random layered {n}q complex128, first {layers} "
-                  f"layers of the same generator ({len(ops)} gates, {sweeps} state sweeps), best of slack "
-                  f"settings {sorted(res)}; circuit build and allocation excluded",
-        "gate_layers_per_s": round(layers / best, 4),
+        "sample": f"GRID_PARALLEL_FUSED C/OpenMP restatement on the headline circuit: {n} qubits complex128, "
+                  f"the first {layers} of its 40 layers ({len(ops)} gates), best of the reference's default "
+                  f"grid (slack 3) and slack log2(threads); {threads} OpenMP threads; state allocation and "
+                  f"circuit build excluded",
+        "gate_layers_per_s": round(layers / best, 5),
         "seconds": {k: round(v, 3) for k, v in res.items()},
+        **cpu_info(),
     }
 
 
-def bench_batching(args) -> int:
+def bench_batching(args) -> dict:
     """4096 independent complex64 circuits, qubit counts log-uniform in
     [10, 22]: half Grover search (one seeded mark), half 20-layer random
     U3 + CZ brick circuits.  Timed: one hipGraph launch running every round
@@ -117,11 +128,10 @@ def bench_batching(args) -> int:
                      "kernel": "whole batch (every launch of the graph)"},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
-def bench_serving(args) -> int:
+def bench_serving(args) -> dict:
     """A seeded Poisson stream (40 % layered, 30 % QFT on random states,
     20 % Grover, 10 % amplitude estimation; half complex128, half complex64)
     through the serving engine, and the same stream one circuit at a time.
@@ -133,16 +143,22 @@ def bench_serving(args) -> int:
     t0 = time.perf_counter()
     jobs = serve.serving_jobs(spec)
     log(f"[bench] {len(jobs)} serving requests built in {time.perf_counter() - t0:.1f}s")
-    arrivals = serve.poisson_arrivals(len(jobs), spec.rate, spec.seed)
     sat = serve.bench_serving(jobs, None, log=log)
+    if spec.rate <= 0:
+        # a sustainable arrival rate: 70 % of the capacity just measured, so
+        # the latency percentiles describe service, not an ever-growing queue
+        spec.rate = round(0.7 * sat["engine"]["circuits_per_s"], 1)
+    arrivals = serve.poisson_arrivals(len(jobs), spec.rate, spec.seed)
     live = serve.bench_serving(jobs, arrivals, log=log)
     eng = sat["engine"]
     # the same mix restricted to 12-20 qubits: every request is launch-bound
     # alone, which is where batching across requests pays
-    small_spec = serve.StreamSpec(count=spec.count, rate=spec.rate, seed=spec.seed, lo=12, hi=20)
+    small_spec = serve.StreamSpec(count=spec.count, rate=args.rate, seed=spec.seed, lo=12, hi=20)
     small_jobs = serve.serving_jobs(small_spec)
     small = serve.bench_serving(small_jobs, None, log=log)
-    small_live = serve.bench_serving(small_jobs, serve.poisson_arrivals(len(small_jobs), spec.rate, spec.seed),
+    if small_spec.rate <= 0:
+        small_spec.rate = round(0.7 * small["engine"]["circuits_per_s"], 1)
+    small_live = serve.bench_serving(small_jobs, serve.poisson_arrivals(len(small_jobs), small_spec.rate, spec.seed),
                                      log=log)
     out = {
         "metric": "circuits/s through the serving engine (saturated), p99 latency at the stated rate",
@@ -168,17 +184,17 @@ def bench_serving(args) -> int:
             "one_circuit_per_launch_circuits_per_s": round(small["sequential"]["circuits_per_s"], 2),
             "speedup_vs_one_circuit_per_launch": round(small["speedup"], 3),
             "saturated_p99_ms": round(small["engine"]["p99_ms"], 3),
-            "at_rate": {"engine": {k: round(small_live["engine"][k], 3) for k in ("circuits_per_s", "p50_ms", "p99_ms")},
+            "at_rate": {"rate_per_s": small_spec.rate,
+                        "engine": {k: round(small_live["engine"][k], 3) for k in ("circuits_per_s", "p50_ms", "p99_ms")},
                         "one_circuit_per_launch": {k: round(small_live["sequential"][k], 3)
                                                    for k in ("circuits_per_s", "p50_ms", "p99_ms")}}},
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
-def bench_precision(args) -> int:
+def bench_precision(args) -> dict:
     """Value-encoding circuit (H on every qubit, exact dyadic phases for a
     seeded n-bit integer, inverse QFT, bit reversal) in complex64 BLOCK mode
     (5-qubit 32 x 32 blocks on FP32 MFMA, FP64 phase generation, compensated
@@ -276,8 +292,7 @@ def bench_precision(args) -> int:
                      "mfma_peak_tflops": 157.3},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 def read_traffic(name: str):
@@ -289,6 +304,22 @@ def read_traffic(name: str):
     for line in tfile.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split("=", 1)[1])
+    return None
+
+
+def read_value(name: str, key: str):
+    """`key = value` from a committed profile summary under profiles/ (None
+    when the file or the key is missing)."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        line = line.strip().strip("`")
+        if line.startswith(key) and "=" in line:
+            try:
+                return float(line.split("=", 1)[1].split()[0])
+            except ValueError:
+                return None
     return None
 
 
@@ -321,10 +352,11 @@ def main() -> int:
                     help="layered: compile every tile pass into its own kernel at plan time (BFLY_FLAG_JIT; "
                          "the compile is part of plan building, outside the timed region)")
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--cpu-qubits", type=int, default=26)
-    ap.add_argument("--cpu-layers", type=int, default=4)
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-qubits", type=int, default=0, help="CPU baseline qubits (0 = the headline's)")
+    ap.add_argument("--cpu-layers", type=int, default=1, help="CPU baseline: the first L layers of the circuit")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="layered: skip the precision / batching / serving summaries of the default run")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batching: compile every circuit with the default options instead of the tuned ones")
     ap.add_argument("--workload", choices=["layered", "batching", "serving", "precision"], default="layered",
@@ -333,12 +365,34 @@ def main() -> int:
                          "through the serving engine")
     ap.add_argument("--circuits", type=int, default=0, help="batching / serving: request count "
                     "(default 4096 / 200)")
-    ap.add_argument("--rate", type=float, default=200.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: Poisson arrival rate (requests/s; 0 = 70 %% of the measured capacity)")
     ap.add_argument("--big", action="store_true", help="serving: add one 33-qubit complex128 QFT")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
                     help="only copy one N-qubit complex128 state device to device (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes) and exit")
+    ap.add_argument("--scale-probe", type=int, default=0, metavar="N",
+                    help="only time the library's streaming kernel (bfly_scale: one in-place read and write of "
+                         "every amplitude) on an N-qubit complex128 state, the measured HBM ceiling, and exit")
     args = ap.parse_args()
+    if args.scale_probe:
+        from code_mojo_amd.state import GpuState
+        g = GpuState(args.scale_probe)
+        g.fill_random(1)
+        for _ in range(3):
+            g.scale(1.0)
+        g.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.scale(1.0)
+        g.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        nbytes = 2 * 2 * (1 << args.scale_probe) * 8
+        print(json.dumps({"scale_probe_qubits": args.scale_probe, "bytes_per_launch": nbytes,
+                          "ms_per_launch_wall": round(ms, 4), "read_write_gbs": round(nbytes / ms / 1e6, 1)}),
+              flush=True)
+        return 0
     if args.copy_probe:
         from code_mojo_amd.state import GpuState
         a, b = GpuState(args.copy_probe), GpuState(args.copy_probe)
@@ -363,11 +417,61 @@ def main() -> int:
         log("no GPU visible")
         return 1
     if args.workload == "batching":
-        return bench_batching(args)
+        print(json.dumps(bench_batching(args)), flush=True)
+        return 0
     if args.workload == "serving":
-        return bench_serving(args)
+        print(json.dumps(bench_serving(args)), flush=True)
+        return 0
     if args.workload == "precision":
-        return bench_precision(args)
+        print(json.dumps(bench_precision(args)), flush=True)
+        return 0
+    out = bench_layered(args, lib)
+    if not args.no_extras and args.qubits == 30 and args.dtype == "c128":
+        out["extras"] = extras(args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def extras(args) -> dict:
+    """The other BASELINE configurations, measured in the same default run
+    after the headline's timed region: the complex64 precision path (30-qubit
+    value encoding), batching (4096 circuits) and serving (the seeded stream
+    with the 33-qubit complex128 QFT, at 70 % of the measured capacity)."""
+    import copy
+    t0 = time.perf_counter()
+    res = {}
+    a = copy.copy(args)
+    a.qubits, a.steps, a.warmup = 30, 1, 1
+    p = bench_precision(a)
+    res["precision"] = {k: p[k] for k in ("metric", "value", "unit", "block_c64_ms", "fused_c128_ms", "plain_c128_ms",
+                                          "l2_vs_c128", "fidelity_vs_c128", "fidelity_vs_exact_c64",
+                                          "block_c128_ms", "block_c128_fidelity_vs_exact")}
+    res["precision"]["roofline"] = p["roofline"]
+    log(f"[bench] extras: precision done at {time.perf_counter() - t0:.1f}s")
+    b = copy.copy(args)
+    b.circuits, b.steps = 4096, 1
+    r = bench_batching(b)
+    res["batching"] = {k: r[k] for k in ("metric", "value", "unit", "one_circuit_per_launch_circuits_per_s",
+                                         "speedup_vs_one_circuit_per_launch", "min_fidelity_vs_c128",
+                                         "checked_vs_c128", "max_amp_err_batch_vs_alone", "roofline")}
+    log(f"[bench] extras: batching done at {time.perf_counter() - t0:.1f}s")
+    c = copy.copy(args)
+    c.circuits, c.rate, c.big = 200, 0.0, True
+    v = bench_serving(c)
+    res["serving"] = {k: v[k] for k in ("metric", "value", "unit", "one_circuit_per_launch_circuits_per_s",
+                                        "speedup_vs_one_circuit_per_launch", "saturated", "at_rate",
+                                        "small_requests_12_20q")}
+    res["serving"]["config"] = v["config"]
+    res["wall_s"] = round(time.perf_counter() - t0, 1)
+    log(f"[bench] extras: serving done at {res['wall_s']}s")
+    return res
+
+
+def bench_layered(args, lib) -> dict:
+    from code_mojo_amd import _lib, lower_circuit
+    from code_mojo_amd import workloads as W
+    from code_mojo_amd.state import GpuState, Plan
+
     n = args.qubits
     dtype = np.float64 if args.dtype == "c128" else np.float32
     b = 16 if args.dtype == "c128" else 8
@@ -415,7 +519,15 @@ def main() -> int:
     per_launch_ms = dom_ms / launches
     achieved = state_bytes / (per_launch_ms / 1e3) / 1e9
 
-    traffic = read_traffic(f"{dom}_{n}q_layered_{args.dtype}_traffic.txt")
+    # the profile summaries are named for the kernel the profiler reports:
+    # bfly_tile_pass for the generated tile-pass kernels, tile_pass_kernel for
+    # the interpreting one
+    kname = "bfly_tile_pass" if (args.jit == "on" and dom == "tile_pass_kernel") else dom
+    traffic = read_traffic(f"{kname}_{n}q_layered_{args.dtype}_traffic.txt")
+    fp64 = read_value(f"{kname}_{n}q_layered_{args.dtype}_pmc.md", "fp64_flops_per_launch")
+    # the in-place read + write streaming ceiling measured on MI355X (a copy
+    # kernel reading and writing 2 x 8 GiB in place), beside the 8 TB/s spec
+    copy_gbs = read_value("hbm_copy_ceiling.md", "copy_read_write_gbs")
 
     out = {
         "metric": "state GB/s vs HBM peak at 30q",
@@ -442,20 +554,23 @@ def main() -> int:
         "gate_layers_per_s": round(args.layers * 1e3 / ms_per_step, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": dom, "launches": launches, "mean_launch_ms": round(per_launch_ms, 4),
-                     "bytes_per_launch": state_bytes},
+                     "kernel": kname, "launches": launches, "mean_launch_ms": round(per_launch_ms, 4),
+                     "bytes_per_launch": state_bytes,
+                     "frac_of_measured_copy_ceiling": round(achieved / copy_gbs, 4) if copy_gbs else None,
+                     "fp64_tflops": round(fp64 / (per_launch_ms * 1e-3) / 1e12, 2) if fp64 else None,
+                     "fp64_frac_of_vector_peak": round(fp64 / (per_launch_ms * 1e-3) / 78.6e12, 4) if fp64 else None},
         "cpu_baseline": None,
     }
     plan.free()
     g.free()
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-        threads = min(threads, 16)
-        log(f"[bench] cpu baseline: {args.cpu_qubits}q x {args.cpu_layers} layers on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.cpu_layers, args.seed, args.cpu_budget, threads)
+        # every core this process may use (the GPU box grants its share through OMP_NUM_THREADS)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0)) or 1
+        cq = args.cpu_qubits or n
+        log(f"[bench] cpu baseline: {cq}q x {args.cpu_layers} layers on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(cq, args.cpu_layers, args.seed, threads)
         log("[bench] cpu baseline done")
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 if __name__ == "__main__":

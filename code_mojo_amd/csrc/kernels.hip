@@ -1861,11 +1861,49 @@ __global__ __launch_bounds__(256) void normalize_kernel(T* __restrict__ re, T* _
     }
 }
 
+// Streaming form measured fastest for an in-place read + write on MI355X:
+// 16-byte vectors, eight per thread in flight, non-temporal loads and stores
+// (the state is touched once per pass).
+template <typename T>
+struct SVec;
+template <>
+struct SVec<double> { typedef double type __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
+template <>
+struct SVec<float> { typedef float type __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
+constexpr int kStreamUnroll = 8;
+
 template <typename T>
 __global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ re, T* __restrict__ im, uint64_t size, double s) {
+    using V = typename SVec<T>::type;
+    constexpr int VN = SVec<T>::N;
     const T sc = static_cast<T>(s);
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < size; i += stride) {
+    const uint64_t nvec = size / VN;
+    V* vr = reinterpret_cast<V*>(re);
+    V* vi = reinterpret_cast<V*>(im);
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x * kStreamUnroll;
+    for (uint64_t v0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x * kStreamUnroll + threadIdx.x; v0 < nvec;
+         v0 += step) {
+        V a[kStreamUnroll], b[kStreamUnroll];
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+            const uint64_t v = v0 + static_cast<uint64_t>(u) * blockDim.x;
+            if (v < nvec) {
+                a[u] = __builtin_nontemporal_load(vr + v);
+                b[u] = __builtin_nontemporal_load(vi + v);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kStreamUnroll; ++u) {
+            const uint64_t v = v0 + static_cast<uint64_t>(u) * blockDim.x;
+            if (v < nvec) {
+                __builtin_nontemporal_store(a[u] * sc, vr + v);
+                __builtin_nontemporal_store(b[u] * sc, vi + v);
+            }
+        }
+    }
+    // the last size % VN elements (states smaller than one vector)
+    if (blockIdx.x == 0 && threadIdx.x < size - nvec * VN) {
+        const uint64_t i = nvec * VN + threadIdx.x;
         re[i] *= sc;
         im[i] *= sc;
     }
@@ -2566,13 +2604,21 @@ hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint
     return hipGetLastError();
 }
 
+// Blocks for the streaming kernels: one vector per thread per unrolled step,
+// at least one block, at most 32768 (measured: the rate is flat from 1024).
+unsigned stream_grid(uint64_t nvec) {
+    const uint64_t per_block = 256ull * kStreamUnroll;
+    const uint64_t b = (nvec + per_block - 1) / per_block;
+    return static_cast<unsigned>(b < 1 ? 1 : (b > 32768 ? 32768 : b));
+}
+
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream) {
     if (dtype == BFLY_C64)
-        hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<float*>(re),
-                           static_cast<float*>(im), size, scale);
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(stream_grid(size / 4)), dim3(256), 0, stream,
+                           static_cast<float*>(re), static_cast<float*>(im), size, scale);
     else
-        hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<double*>(re),
-                           static_cast<double*>(im), size, scale);
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(stream_grid(size / 2)), dim3(256), 0, stream,
+                           static_cast<double*>(re), static_cast<double*>(im), size, scale);
     return hipGetLastError();
 }
 

@@ -49,3 +49,17 @@ def test_copy_onto_itself_is_refused(gpu_lib):
     g = GpuState(8)
     with pytest.raises(BflyError):
         g.copy_from(g)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [1, 2, 3, 12, 21])
+def test_scale_streaming_kernel(gpu_lib, dtype, n):
+    # 16-byte vectors plus the scalar tail of states smaller than one vector
+    g = GpuState(n, dtype)
+    st = W.normal_state(n, 40 + n, dtype)
+    g.upload(st)
+    g.scale(-0.375)             # exact in both precisions: results must equal the host product
+    out = g.download()
+    g.free()
+    assert np.array_equal(out.re, (st.re * dtype(-0.375)).astype(dtype))
+    assert np.array_equal(out.im, (st.im * dtype(-0.375)).astype(dtype))
