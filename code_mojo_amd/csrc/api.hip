@@ -1158,6 +1158,7 @@ constexpr int64_t kMinstdMul = 48271;
 struct MeasureSetup {
     std::vector<int32_t> free_q;
     uint64_t forced_mask = 0, forced_val = 0, tmask = 0;
+    bool conflict = false;   // a qubit forced to both values: every probability is zero
     std::vector<double> P;
 };
 
@@ -1183,6 +1184,59 @@ void measure_spread(MeasureSetup& ms, int f, const std::vector<int32_t>& uq, con
     }
 }
 
+// Probabilities of the 2^f outcomes of the qubits fpos[0..f) (outcome bit k =
+// qubit fpos[k], f <= kOutcomeDirectMaxBits) among the amplitudes whose bits
+// in fixed_mask read fixed_val, summed over every other qubit: FP64
+// compensated, deterministic.  Up to kOutcomeMaxBits outcomes in LDS bins,
+// more with one thread per outcome and rest chunk.
+int outcome_probs_device(bfly_state* s, const int32_t* fpos, int f, uint64_t fixed_mask, uint64_t fixed_val,
+                         std::vector<double>& out) {
+    const uint64_t all = (1ull << s->n) - 1;
+    uint64_t fmask = 0;
+    for (int k = 0; k < f; ++k) fmask |= 1ull << fpos[k];
+    out.assign(size_t(1) << f, 0.0);
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    double* buf = nullptr;
+    hipError_t e = hipSuccess;
+    if (f > kOutcomeMaxBits) {
+        OutcomeDirect dp{};
+        dp.f = f;
+        for (int k = 0; k < f; ++k) dp.fpos[k] = fpos[k];
+        dp.fval = fixed_val;
+        dp.rmask = all & ~(fixed_mask | fmask);
+        dp.nrest = 1ull << __builtin_popcountll(dp.rmask);
+        const int R = outcome_direct_chunks(dp);
+        const size_t parts = R > 1 ? static_cast<size_t>(R) * out.size() : 0;
+        HIP_TRY(hipMalloc(&buf, (out.size() + parts) * sizeof(double)), "hipMalloc(measurement)");
+        e = launch_outcome_direct(s->dtype, s->re, s->im, dp, R, buf + out.size(), buf, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out.data(), buf, out.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    } else {
+        OutcomeSpec sp{};
+        sp.f = f;
+        for (int k = 0; k < f; ++k) sp.fpos[k] = fpos[k];
+        sp.fval = fixed_val;
+        sp.rmask = all & ~(fixed_mask | fmask);
+        const uint64_t rest_count = 1ull << __builtin_popcountll(sp.rmask);
+        const int nb = outcome_blocks(sp, rest_count);
+        sp.rest_per_block = rest_count / static_cast<uint64_t>(nb);
+        const size_t nout = out.size();
+        HIP_TRY(hipMalloc(&buf, (static_cast<size_t>(nb) * nout + nout) * sizeof(double)), "hipMalloc(measurement)");
+        e = launch_outcome_probs(s->dtype, s->re, s->im, sp, nb, buf, buf + static_cast<size_t>(nb) * nout, s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out.data(), buf + static_cast<size_t>(nb) * nout, nout * sizeof(double),
+                               hipMemcpyDeviceToHost, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
+    return BFLY_OK;
+}
+
+// Targets, forced values and, when the free list has at most
+// kOutcomeDirectMaxBits entries, the 2^f outcome probabilities in ms.P.
+// Wider free lists leave ms.P empty (measure_state then searches the outcome
+// level by level).
 int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, int m, MeasureSetup& ms) {
     bool conflict = false;
     for (int k = 0; k < m; ++k) {
@@ -1199,59 +1253,20 @@ int measure_setup(bfly_state* s, const int32_t* targets, const int32_t* values, 
             ms.forced_val |= static_cast<uint64_t>(v) << t;
         }
     }
+    ms.conflict = conflict;
     const int f = static_cast<int>(ms.free_q.size());
-    if (f > kOutcomeDirectMaxBits)
-        return fail(BFLY_EUNSUPPORTED, "more than " + std::to_string(kOutcomeDirectMaxBits) + " free measured targets");
+    if (f > 64) return fail(BFLY_EINVAL, "more than 64 free measured targets");
+    if (f > kOutcomeDirectMaxBits) return BFLY_OK;
     std::vector<int32_t> uq;
     uint64_t umask = 0;
     for (int32_t q : ms.free_q)
         if (!((ms.forced_mask >> q) & 1) && !((umask >> q) & 1)) { uq.push_back(q); umask |= 1ull << q; }
     ms.P.assign(size_t(1) << f, 0.0);
     if (conflict) return BFLY_OK;  // no amplitude matches the forced values
-    const uint64_t all = (1ull << s->n) - 1;
-    std::vector<double> Pu(size_t(1) << uq.size());
-    if (static_cast<int>(uq.size()) > kOutcomeMaxBits) {
-        // many outcomes: one thread per outcome
-        OutcomeDirect dp{};
-        dp.f = static_cast<int32_t>(uq.size());
-        for (size_t j = 0; j < uq.size(); ++j) dp.fpos[j] = uq[j];
-        dp.fval = ms.forced_val;
-        dp.rmask = all & ~(ms.forced_mask | umask);
-        dp.nrest = 1ull << __builtin_popcountll(dp.rmask);
-        const int R = outcome_direct_chunks(dp);
-        HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
-        double* buf = nullptr;
-        const size_t parts = R > 1 ? static_cast<size_t>(R) * Pu.size() : 0;
-        HIP_TRY(hipMalloc(&buf, (Pu.size() + parts) * sizeof(double)), "hipMalloc(measurement)");
-        hipError_t e = launch_outcome_direct(s->dtype, s->re, s->im, dp, R, buf + Pu.size(), buf, s->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(Pu.data(), buf, Pu.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-        (void)hipFree(buf);
-        if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
-        measure_spread(ms, f, uq, Pu);
-        return BFLY_OK;
-    }
-    OutcomeSpec sp{};
-    sp.f = static_cast<int32_t>(uq.size());
-    for (size_t j = 0; j < uq.size(); ++j) sp.fpos[j] = uq[j];
-    sp.fval = ms.forced_val;
-    sp.rmask = all & ~(ms.forced_mask | umask);
-    const uint64_t rest_count = 1ull << __builtin_popcountll(sp.rmask);
-    const int nb = outcome_blocks(sp, rest_count);
-    sp.rest_per_block = rest_count / static_cast<uint64_t>(nb);
-    const size_t nout = size_t(1) << sp.f;
-    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
-    double* buf = nullptr;
-    HIP_TRY(hipMalloc(&buf, (static_cast<size_t>(nb) * nout + nout) * sizeof(double)), "hipMalloc(measurement)");
-    hipError_t e = launch_outcome_probs(s->dtype, s->re, s->im, sp, nb, buf, buf + static_cast<size_t>(nb) * nout,
-                                        s->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(Pu.data(), buf + static_cast<size_t>(nb) * nout, nout * sizeof(double),
-                           hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(buf);
-    if (e != hipSuccess) return hip_fail(e, "measurement probabilities");
+    std::vector<double> Pu;
+    if (const int rc = outcome_probs_device(s, uq.data(), static_cast<int>(uq.size()), ms.forced_mask,
+                                            ms.forced_val, Pu))
+        return rc;
     measure_spread(ms, f, uq, Pu);
     return BFLY_OK;
 }
@@ -1284,6 +1299,80 @@ int measure_choose(const std::vector<double>& P, int64_t seed, uint64_t* chosen_
     return BFLY_OK;
 }
 
+// More than kOutcomeDirectMaxBits free targets: the reference's choice
+// without its 2^f outcome sums (state.mojo:475-533 allocates them).  The
+// outcomes that can have a nonzero probability are in one-to-one,
+// order-keeping correspondence with the values of the distinct unforced free
+// qubits read as a number whose most significant bit is the qubit whose last
+// place in the free list is highest (outcome bit k is the k-th free target,
+// so that place is its top bit).  Level by level, the next (at most 24) most
+// significant of those qubits get their bucket sums among the amplitudes
+// that agree with the bits fixed so far; the running sum continues from the
+// buckets before, and the bucket where it reaches r fixes those qubits.  The
+// total is the first level's sum of bucket sums; the chosen probability the
+// last level's bucket.  The running sums, the total and r are FP64 for both
+// precisions: a float32 running sum over 2^25 or more outcomes stagnates
+// (single outcome probabilities fall below half its ulp), so the reference's
+// float32 rule has no stable answer there.  The choice equals the
+// reference's complex128 one except where r lies within rounding of a
+// boundary of the running sum (its outcome sums are naive, these FP64
+// compensated).
+int measure_choose_wide(bfly_state* s, const MeasureSetup& ms, int64_t seed, uint64_t* tval_out, double* chosen_prob) {
+    if (ms.conflict) return fail(BFLY_EINVAL, "Measurement probability is zero");
+    const int f = static_cast<int>(ms.free_q.size());
+    std::vector<int32_t> order;   // distinct unforced free qubits, most significant first
+    uint64_t seen = 0;
+    for (int k = f - 1; k >= 0; --k) {
+        const int32_t q = ms.free_q[static_cast<size_t>(k)];
+        if (((ms.forced_mask >> q) & 1) || ((seen >> q) & 1)) continue;
+        seen |= 1ull << q;
+        order.push_back(q);
+    }
+    uint64_t fixed_mask = ms.forced_mask, fixed_val = ms.forced_val;
+    double r = 0.0, acc = 0.0, cp = 0.0;
+    std::vector<double> P;
+    for (size_t done = 0, level = 0; done < order.size(); ++level) {
+        const int h = static_cast<int>(std::min<size_t>(kOutcomeDirectMaxBits, order.size() - done));
+        int32_t fpos[kOutcomeDirectMaxBits];
+        for (int k = 0; k < h; ++k) fpos[k] = order[done + static_cast<size_t>(h - 1 - k)];  // top bit: order[done]
+        if (const int rc = outcome_probs_device(s, fpos, h, fixed_mask, fixed_val, P)) return rc;
+        if (level == 0) {
+            double total = 0.0;
+            for (double p : P) total += p;
+            if (!(total > 0.0)) return fail(BFLY_EINVAL, "Measurement probability is zero");
+            int64_t rng = seed % kMinstdMod;
+            rng = (rng * kMinstdMul) % kMinstdMod;
+            r = static_cast<double>(rng) / static_cast<double>(kMinstdMod) * total;
+        }
+        // first bucket whose running sum reaches r; past the end (rounding of
+        // the deeper sums), the last bucket with a nonzero probability
+        size_t pick = P.size(), last_nz = P.size();
+        double before = acc, before_last = acc;
+        for (size_t j = 0; j < P.size(); ++j) {
+            if (P[j] > 0.0) { last_nz = j; before_last = acc; }
+            const double next = acc + P[j];
+            if (r <= next) { pick = j; before = acc; break; }
+            acc = next;
+        }
+        if (pick == P.size()) {
+            if (last_nz == P.size()) return fail(BFLY_EINVAL, "Measurement outcome probability is zero");
+            pick = last_nz;
+            before = before_last;
+        }
+        acc = before;
+        for (int k = 0; k < h; ++k) {
+            fixed_mask |= 1ull << fpos[k];
+            if ((pick >> k) & 1) fixed_val |= 1ull << fpos[k];
+        }
+        cp = P[pick];
+        done += static_cast<size_t>(h);
+    }
+    if (!(cp > 0.0)) return fail(BFLY_EINVAL, "Measurement outcome probability is zero");
+    *tval_out = fixed_val;
+    *chosen_prob = cp;
+    return BFLY_OK;
+}
+
 }  // namespace
 
 namespace bfly {
@@ -1296,20 +1385,32 @@ int measure_state(bfly_state* s, const int32_t* targets, const int32_t* values, 
     int rc = measure_setup(s, targets, values, m, ms);
     if (rc) return rc;
     if (seed < 0) seed = clock_seed();
-    uint64_t chosen = 0;
+    uint64_t chosen = 0, tval = ms.forced_val;
     double cp = 0.0, inv = 0.0;
-    if (s->dtype == BFLY_C128) {
-        rc = measure_choose<double>(ms.P, seed, &chosen, &cp, &inv);
+    if (static_cast<int>(ms.free_q.size()) > kOutcomeDirectMaxBits) {
+        if ((rc = measure_choose_wide(s, ms, seed, &tval, &cp))) return rc;
+        if (s->dtype == BFLY_C128) {
+            inv = 1.0 / std::sqrt(cp);
+        } else {   // the collapse scale in the state's precision, as the reference computes it
+            const float cpf = static_cast<float>(cp);
+            cp = cpf;
+            inv = 1.0f / static_cast<float>(std::sqrt(static_cast<double>(cpf)));
+        }
+        for (size_t k = 0; k < ms.free_q.size(); ++k)
+            if ((tval >> ms.free_q[k]) & 1) chosen |= 1ull << k;
     } else {
-        float cpf = 0.f, invf = 0.f;
-        rc = measure_choose<float>(ms.P, seed, &chosen, &cpf, &invf);
-        cp = cpf;
-        inv = invf;
+        if (s->dtype == BFLY_C128) {
+            rc = measure_choose<double>(ms.P, seed, &chosen, &cp, &inv);
+        } else {
+            float cpf = 0.f, invf = 0.f;
+            rc = measure_choose<float>(ms.P, seed, &chosen, &cpf, &invf);
+            cp = cpf;
+            inv = invf;
+        }
+        if (rc) return rc;
+        for (size_t k = 0; k < ms.free_q.size(); ++k)
+            if ((chosen >> k) & 1) tval |= 1ull << ms.free_q[k];
     }
-    if (rc) return rc;
-    uint64_t tval = ms.forced_val;
-    for (size_t k = 0; k < ms.free_q.size(); ++k)
-        if ((chosen >> k) & 1) tval |= 1ull << ms.free_q[k];
     HIP_TRY(launch_collapse(s->dtype, s->re, s->im, 1ull << s->n, ms.tmask, tval, inv, s->stream), "collapse launch");
     HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
     if (outcome) *outcome = chosen;
@@ -1335,6 +1436,8 @@ int bfly_sample(bfly_state* s, const int32_t* targets, int32_t m, int64_t seed, 
     if (m < 1 || !targets || (shots && !samples)) return fail(BFLY_EINVAL, "null argument");
     if (seed < 0 || seed % kMinstdMod == 0) return fail(BFLY_EINVAL, "sample seed must be >= 0 and not a multiple of 2147483647");
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    if (m > kOutcomeDirectMaxBits)
+        return fail(BFLY_EUNSUPPORTED, "more than " + std::to_string(kOutcomeDirectMaxBits) + " sampled targets");
     MeasureSetup ms;
     if ((rc = measure_setup(s, targets, nullptr, m, ms))) return rc;
     // running sums in the state's precision, searched per shot

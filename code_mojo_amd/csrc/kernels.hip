@@ -2316,6 +2316,119 @@ __global__ __launch_bounds__(kDiffThreads) void diffuse_block_kernel(T* __restri
     }
 }
 
+// DIFFUSE over t > kDiffuseMaxBits bits of a partial mask (apply_grover_diffuser
+// takes any target list).  The amplitudes of one block of the other bits are
+// enumerated in mask order: thread j starts at deposit(j, tmask) and steps by
+// deposit(256, tmask) with the masked add ((d | ~tmask) + step) & tmask.
+__device__ __forceinline__ uint64_t mask_add(uint64_t d, uint64_t step, uint64_t mask) {
+    return ((d | ~mask) + step) & mask;
+}
+
+// Many blocks (more than 2^14): one workgroup per block sums its 2^t
+// amplitudes (TwoSum per thread, the fixed merge of diff_block_merge), then
+// updates them; the second sweep re-reads the block.
+template <typename T>
+__global__ __launch_bounds__(kDiffChunkThreads) void diffuse_wide_block_kernel(T* __restrict__ re, T* __restrict__ im,
+                                                                              int t, uint64_t tmask, uint64_t others,
+                                                                              uint64_t nblocks, int variant) {
+    __shared__ double sh[kDiffWaves][4];
+    __shared__ double mean[2];
+    const uint64_t count = 1ull << t;
+    const uint64_t first = deposit(threadIdx.x, tmask);
+    const uint64_t step = deposit(kDiffChunkThreads, tmask);
+    for (uint64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const uint64_t base = deposit(blk, others);
+        double v[4] = {0, 0, 0, 0};
+        uint64_t d = first;
+        for (uint64_t e = threadIdx.x; e < count; e += kDiffChunkThreads) {
+            two_sum_add(v[0], v[1], static_cast<double>(re[base | d]));
+            two_sum_add(v[2], v[3], static_cast<double>(im[base | d]));
+            d = mask_add(d, step, tmask);
+        }
+        diff_block_merge(v, sh);
+        if (threadIdx.x == 0) {
+            const double inv = ldexp(1.0, -t);
+            mean[0] = 2.0 * ((v[0] + v[1]) * inv);
+            mean[1] = 2.0 * ((v[2] + v[3]) * inv);
+        }
+        __syncthreads();
+        const double ar = mean[0], ai = mean[1];
+        double unused[4];
+        d = first;
+        for (uint64_t e = threadIdx.x; e < count; e += kDiffChunkThreads) {
+            diff_update(re[base | d], im[base | d], ar, ai, variant, false, 0, 0, 0, false, unused);
+            d = mask_add(d, step, tmask);
+        }
+        __syncthreads();   // mean[] is rewritten for the next block
+    }
+}
+
+// Few blocks: block b's 2^t amplitudes split into nch = 2^lch chunks of 2^cb
+// (chunk c = the amplitudes whose top lch mask bits read c); pass 1 writes
+// one compensated pair per (block, chunk) unit, pass 2 merges the block's
+// nch pairs in a fixed order in every unit and updates the unit's chunk.
+template <typename T>
+__global__ __launch_bounds__(kDiffChunkThreads) void diffuse_wide_partial_kernel(
+    const T* __restrict__ re, const T* __restrict__ im, int cb, int lch, uint64_t tmask, uint64_t others,
+    uint64_t units, double* __restrict__ partial) {
+    __shared__ double sh[kDiffWaves][4];
+    const uint64_t len = 1ull << cb;
+    const uint64_t first = deposit(threadIdx.x, tmask);
+    const uint64_t step = deposit(kDiffChunkThreads, tmask);
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint64_t b = u >> lch, c = u & ((1ull << lch) - 1);
+        const uint64_t base = deposit(b, others) | deposit(c << cb, tmask);
+        double v[4] = {0, 0, 0, 0};
+        uint64_t d = first;
+        for (uint64_t e = threadIdx.x; e < len; e += kDiffChunkThreads) {
+            two_sum_add(v[0], v[1], static_cast<double>(re[base | d]));
+            two_sum_add(v[2], v[3], static_cast<double>(im[base | d]));
+            d = mask_add(d, step, tmask);
+        }
+        diff_store_chunk(v, partial, u, sh);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kDiffChunkThreads) void diffuse_wide_apply_kernel(
+    T* __restrict__ re, T* __restrict__ im, int t, int cb, int lch, uint64_t tmask, uint64_t others, uint64_t units,
+    const double* __restrict__ partial, int variant) {
+    __shared__ double sh[kDiffWaves][4];
+    __shared__ double mean[2];
+    const uint64_t len = 1ull << cb;
+    const uint64_t nch = 1ull << lch;
+    const uint64_t first = deposit(threadIdx.x, tmask);
+    const uint64_t step = deposit(kDiffChunkThreads, tmask);
+    for (uint64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const uint64_t b = u >> lch, c = u & (nch - 1);
+        {
+            const double2* pp = reinterpret_cast<const double2*>(partial) + 2 * (b << lch);
+            double m[4] = {0, 0, 0, 0};
+            for (uint64_t i = threadIdx.x; i < nch; i += kDiffChunkThreads) {
+                const double2 x = pp[2 * i], y = pp[2 * i + 1];
+                acc_merge(m[0], m[1], x.x, x.y);
+                acc_merge(m[2], m[3], y.x, y.y);
+            }
+            diff_block_merge(m, sh);
+            if (threadIdx.x == 0) {
+                const double inv = ldexp(1.0, -t);
+                mean[0] = 2.0 * ((m[0] + m[1]) * inv);
+                mean[1] = 2.0 * ((m[2] + m[3]) * inv);
+            }
+            __syncthreads();
+        }
+        const double ar = mean[0], ai = mean[1];
+        const uint64_t base = deposit(b, others) | deposit(c << cb, tmask);
+        double unused[4];
+        uint64_t d = first;
+        for (uint64_t e = threadIdx.x; e < len; e += kDiffChunkThreads) {
+            diff_update(re[base | d], im[base | d], ar, ai, variant, false, 0, 0, 0, false, unused);
+            d = mask_add(d, step, tmask);
+        }
+        __syncthreads();   // mean[] is rewritten by the next unit
+    }
+}
+
 constexpr int kMargThreads = 64;
 constexpr int kMargMaxBits = 6;
 
@@ -2881,8 +2994,47 @@ hipError_t launch_diffuse(int dtype, void* re, void* im, int n, uint64_t mask, i
         const OpItem it = diffuse_item(re, im, n, variant, scratch, chain, fmask, fvalue);
         return launch_diffuse_items(dtype, it, nullptr, 0, it.count, stream);
     }
-    if (t > kDiffuseMaxBits) return hipErrorInvalidValue;
     const uint64_t nblocks = 1ull << (n - t);
+    if (t > kDiffuseMaxBits) {
+        // wide partial mask: 2^lb blocks of 2^t amplitudes; chunks of at least
+        // 2^12, at most 2^10 per block and 2^14 units (one partial buffer)
+        const int lb = n - t;
+        int lch = 14 - lb;
+        if (lch > 10) lch = 10;
+        if (lch > t - 12) lch = t - 12;
+        const uint64_t others = all & ~mask;
+        if (lch <= 0) {
+            const unsigned grid = static_cast<unsigned>(nblocks < 65536 ? nblocks : 65536);
+            if (dtype == BFLY_C64)
+                hipLaunchKernelGGL(diffuse_wide_block_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                                   static_cast<float*>(re), static_cast<float*>(im), t, mask, others, nblocks,
+                                   variant);
+            else
+                hipLaunchKernelGGL(diffuse_wide_block_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                                   static_cast<double*>(re), static_cast<double*>(im), t, mask, others, nblocks,
+                                   variant);
+            return hipGetLastError();
+        }
+        const int cb = t - lch;
+        const uint64_t units = nblocks << lch;   // <= 2^14: 4 doubles each fit kPartialSpan
+        const unsigned grid = static_cast<unsigned>(units);
+        if (dtype == BFLY_C64) {
+            hipLaunchKernelGGL(diffuse_wide_partial_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                               static_cast<const float*>(re), static_cast<const float*>(im), cb, lch, mask, others,
+                               units, scratch);
+            hipLaunchKernelGGL(diffuse_wide_apply_kernel<float>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                               static_cast<float*>(re), static_cast<float*>(im), t, cb, lch, mask, others, units,
+                               static_cast<const double*>(scratch), variant);
+        } else {
+            hipLaunchKernelGGL(diffuse_wide_partial_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                               static_cast<const double*>(re), static_cast<const double*>(im), cb, lch, mask, others,
+                               units, scratch);
+            hipLaunchKernelGGL(diffuse_wide_apply_kernel<double>, dim3(grid), dim3(kDiffChunkThreads), 0, stream,
+                               static_cast<double*>(re), static_cast<double*>(im), t, cb, lch, mask, others, units,
+                               static_cast<const double*>(scratch), variant);
+        }
+        return hipGetLastError();
+    }
     const unsigned grid = static_cast<unsigned>(nblocks < 65536 ? nblocks : 65536);
     if (dtype == BFLY_C64)
         hipLaunchKernelGGL(diffuse_block_kernel<float>, dim3(grid), dim3(kDiffThreads), 0, stream,

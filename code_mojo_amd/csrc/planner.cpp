@@ -188,10 +188,6 @@ int validate_ops(int32_t n, const bfly_op* ops, uint64_t nops, uint64_t udata_le
             case BFLY_OP_DIFFUSE:
                 if (op.aux == 0 || (op.aux & ~all)) { err = "Target out of bounds"; return BFLY_EINVAL; }
                 if (op.arg_i != 0 && op.arg_i != 1) { err = "unknown diffuser variant"; return BFLY_EINVAL; }
-                if (op.aux != all && popc(op.aux) > kDiffuseMaxBits) {
-                    err = "diffusers over more than 12 qubits must cover every qubit";
-                    return BFLY_EUNSUPPORTED;
-                }
                 break;
             default:
                 err = "Unknown transformation type";

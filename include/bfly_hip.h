@@ -334,12 +334,16 @@ int bfly_scale(bfly_state* s, double scale);
    targets[k] with values[k] = -1 (free) or 0 / 1 (forced).  Outcome
    probabilities of the free targets (bit k of an outcome = the k-th free
    target) among the amplitudes matching the forced values are FP64
-   compensated sums on the device (at most 24 free targets); the
-   outcome is chosen on the host exactly as the reference does: one MINSTD
-   step rng = (seed mod 2147483647) * 48271 mod 2147483647,
-   r = rng / 2147483647 * total in the state's precision, the first outcome
-   whose running sum reaches r.  The state is collapsed onto it and divided by
-   sqrt(its probability).  seed < 0 takes the clock, as the reference does
+   compensated sums on the device; the outcome is chosen on the host as the
+   reference does: one MINSTD step rng = (seed mod 2147483647) * 48271 mod
+   2147483647, r = rng / 2147483647 * total in the state's precision, the
+   first outcome whose running sum reaches r.  The state is collapsed onto it
+   and divided by sqrt(its probability).  Up to 24 free targets the 2^f
+   outcome sums are formed; with more (up to 64) the outcome is found level by
+   level over the distinct free qubits (at most 2^24 bucket sums per level,
+   one pass over the state each) with FP64 running sums and r for both
+   precisions, since a float32 running sum over 2^25 or more outcomes
+   stagnates.  seed < 0 takes the clock, as the reference does
    without a configured measure_seed.  `outcome` / `prob` (optional) receive
    the chosen outcome and its (unnormalised) probability.  Synchronous. */
 int bfly_measure(bfly_state* s, const int32_t* targets, const int32_t* values, int32_t m, int64_t seed,
@@ -347,7 +351,7 @@ int bfly_measure(bfly_state* s, const int32_t* targets, const int32_t* values, i
 /* `shots` outcomes of `targets` (all free) without collapse: shot j uses the
    (j+1)-th state of the same MINSTD sequence (shot 0 is the outcome
    bfly_measure would choose with `seed`).  seed >= 0, not a multiple of
-   2147483647.  Synchronous; the state is unchanged. */
+   2147483647, at most 24 targets.  Synchronous; the state is unchanged. */
 int bfly_sample(bfly_state* s, const int32_t* targets, int32_t m, int64_t seed, uint64_t shots, uint64_t* samples);
 
 /* ---- circuit ingestion: replaces parse_qasm3_string (utils/qasm3_import.mojo:52-123)

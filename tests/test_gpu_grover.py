@@ -66,6 +66,41 @@ def test_diffuse_masks_on_complex_states(gpu_lib, mask, variant):
         assert max(np.abs(re - re_o).max(), np.abs(im - im_o).max()) <= tol, dtype
 
 
+def _spread(n, t, seed):
+    rng = np.random.default_rng(seed)
+    bits = rng.choice(n, size=t, replace=False)
+    return int(sum(1 << int(b) for b in bits))
+
+
+# partial diffusers over more than 12 qubits (apply_grover_diffuser takes any
+# target list): few blocks split into chunks (two passes), and 2^14 or more
+# blocks of 2^t amplitudes (one workgroup per block)
+@pytest.mark.parametrize("n,mask", [(16, (1 << 14) - 1), (20, _spread(20, 15, 1)), (22, ((1 << 21) - 1) << 1),
+                                    (18, ((1 << 13) - 1) << 5), (27, _spread(27, 13, 2)), (27, (1 << 13) - 1)])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_wide_partial_diffusers(gpu_lib, n, mask, variant):
+    out = OpList(n)
+    out.diffuse(mask, variant)
+    out.phase_flip(mask, mask & 0x2A5A5A5)
+    out.diffuse(mask, variant)
+    ops, ud = out.finish()
+    print(f"[wide-diffuse] n={n} mask={mask:#x} variant={variant}", flush=True)
+    # a random state plus a common offset: the block means are of the order
+    # of the amplitudes, so a wrong or missing mean shows at every tolerance
+    st = W.normal_state(n, n + variant, np.float32 if n > 24 else np.float64)
+    off = 2.0 ** (-n / 2)
+    re0 = (st.re.astype(np.float64) + 3 * off).astype(st.re.dtype)
+    im0 = (st.im.astype(np.float64) - off).astype(st.im.dtype)
+    re_o, im_o = run_oracle(n, ops, ud, re0.astype(np.float64), im0.astype(np.float64))
+    scale = max(np.abs(re_o).max(), np.abs(im_o).max())
+    # complex128: the oracle bar; complex64: 1e-5 of the largest amplitude
+    dtypes = ((np.float32, 1e-5 * scale),) if n > 24 else ((np.float64, TOL_C128), (np.float32, 1e-5 * scale))
+    for dtype, tol in dtypes:
+        re, im, _ = run_gpu(n, ops, ud, re0.astype(dtype), im0.astype(dtype), dtype=dtype)
+        err = max(np.abs(re - re_o).max(), np.abs(im - im_o).max())
+        assert err <= tol, (dtype, err, tol)
+
+
 @pytest.mark.parametrize("seed", [5, 6])
 def test_flips_inside_tile_passes(gpu_lib, seed):
     n = 15

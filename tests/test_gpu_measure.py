@@ -122,8 +122,8 @@ def test_reference_errors_on_gpu(gpu_lib):
     one.upload_arrays(np.array([0.0, 1.0]), np.zeros(2))
     with pytest.raises(_lib.BflyError, match="outcome probability is zero"):
         one.measure([0], None, 0)       # seed 0 draws r = 0: outcome 0 with p = 0
-    with pytest.raises(_lib.BflyError, match="more than 24 free measured targets"):
-        GpuState(25).measure(list(range(25)), None, 3)
+    with pytest.raises(_lib.BflyError, match="more than 24 sampled targets"):
+        GpuState(25).sample(list(range(25)), 10, 3)
     g.free()
     one.free()
 
@@ -221,3 +221,61 @@ def test_sample_many_targets(gpu_lib):
     outcome, _ = g.measure(targets, None, seed)
     assert outcome == int(shots[0])
     g.free()
+
+
+WIDE = [
+    (26, list(range(26)), [-1] * 26, 20261204),                          # every qubit: 2^26 outcomes
+    # 25 distinct free qubits in falling order, one of them listed twice,
+    # and one forced qubit: 26 free entries
+    (26, list(range(25, 0, -1)) + [7, 0], [-1] * 26 + [1], 734019),
+]
+
+
+@pytest.mark.parametrize("n,targets,values,seed", WIDE)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_measure_more_than_24_free_targets(gpu_lib, n, targets, values, seed, dtype):
+    """More than 24 free targets (the reference allocates 2^f outcome sums,
+    state.mojo:475-501): the library searches the outcome level by level.
+    The draw is made in FP64 for both precisions (a float32 running sum over
+    2^25 or more outcomes stagnates: single outcome probabilities fall below
+    half its ulp).  complex128: the collapsed state within 1e-12 of the
+    oracle's (which keeps the 2^f sums); complex64: the outcome has the
+    probability returned, lies where r = rng / M * total falls in the FP64
+    running sums of the float32 state's outcome probabilities (within 1e-9 of
+    the total), and the collapse equals the oracle's collapse onto it."""
+    print(f"[measure-wide] n={n} free={sum(v < 0 for v in values)} seed={seed} {np.dtype(dtype).name}", flush=True)
+    st = W.normal_state(n, seed % 1000)
+    g = gpu_state(n, st, dtype)
+    outcome, prob = g.measure(targets, [None if v < 0 else bool(v) for v in values], seed)
+    got = g.download()
+    assert abs(g.norm2() - 1.0) < (1e-12 if dtype == np.float64 else 1e-5)
+    g.free()
+    free = [t for t, v in zip(targets, values) if v < 0]
+    forced = []
+    fi = 0
+    for t, v in zip(targets, values):
+        if v < 0:
+            forced.append((outcome >> fi) & 1)
+            fi += 1
+        else:
+            forced.append(v)
+    # the outcome is consistent: a repeated free qubit carries one value
+    for k, t in enumerate(free):
+        for k2, t2 in enumerate(free):
+            if t == t2:
+                assert ((outcome >> k) & 1) == ((outcome >> k2) & 1)
+    if dtype == np.float64:
+        re_o, im_o = oracle_measure(n, st, targets, values, seed)
+    else:
+        re32, im32 = st.re.astype(dtype), st.im.astype(dtype)
+        P = fp64_outcome_probs(n, re32, im32, targets, values)
+        assert P[outcome] > 0 and abs(prob - P[outcome]) <= 1e-6 * P[outcome], (prob, P[outcome])
+        cum = np.cumsum(P)
+        total = cum[-1]
+        r = ((seed % M) * 48271 % M) / M * total
+        lo = cum[outcome] - P[outcome]
+        assert lo - 1e-9 * total <= r <= cum[outcome] + 1e-9 * total, (outcome, r, lo, cum[outcome])
+        del P, cum
+        re_o, im_o = oracle_measure(n, st, targets, forced, seed, dtype)
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    assert max(np.abs(got.re - re_o).max(), np.abs(got.im - im_o).max()) <= tol, (outcome, prob)

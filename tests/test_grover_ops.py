@@ -201,3 +201,27 @@ def test_flip_and_diffuse_ops_inside_random_schedules(seed):
     for tb in (5, 7):
         plan = Plan(n, np.float64, allops, ud, _lib.opts(tile_bits=tb))
         assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,targets", [(16, list(range(14))), (20, [1, 3, 4, 7, 8, 9, 11, 12, 13, 14, 15, 16, 17, 18, 19]),
+                                       (6, [0, 2, 3, 5])])
+def test_partial_diffuser_of_any_width_lowers_and_plans(n, targets):
+    """GROVER_DIFFUSER takes any target list (algos/grover_classical.mojo:284-326):
+    diffusers over more than 12 of the qubits lower to one DIFFUSE op and the
+    planner keeps it as one diffuser step; at 6 qubits the oracle equals the
+    dense reflection of every block about its mean, per variant 1."""
+    qc = QuantumCircuit(n)
+    qc.add_classical("GROVER_DIFFUSER", targets)
+    ops, ud = lower_circuit(qc)
+    mask = sum(1 << t for t in targets)
+    assert len(ops) == 1 and ops[0]["kind"] == OP_DIFFUSE and int(ops[0]["aux"]) == mask and ops[0]["arg_i"] == 1
+    plan = Plan(n, np.float64, ops, ud, _lib.opts())
+    assert f"diffuse mask={mask} variant=1" in plan.dump()
+    if n <= 8:
+        t = len(targets)
+        J = np.full((1 << t, 1 << t), 2.0 / (1 << t))      # 2 |s><s| over the block
+        D = dense_on(n, mask, J)
+        psi = rand_state(n, 4)
+        # variant 1: re <- 2 mean_re - re, im <- -2 mean_im - im
+        want = (D @ psi.real - psi.real) + 1j * (-(D @ psi.imag) - psi.imag)
+        assert np.abs(run(n, ops, ud, psi) - want).max() < 1e-14
