@@ -359,7 +359,7 @@ def main() -> int:
                     help="layered: skip the precision / batching / serving summaries of the default run")
     ap.add_argument("--no-autotune", action="store_true",
                     help="batching: compile every circuit with the default options instead of the tuned ones")
-    ap.add_argument("--workload", choices=["layered", "batching", "serving", "precision"], default="layered",
+    ap.add_argument("--workload", choices=["layered", "batching", "serving", "precision", "unitary"], default="layered",
                     help="layered: the headline state-GB/s workload (default); batching: many independent "
                          "circuits in batched persistent launches; serving: a Poisson request stream "
                          "through the serving engine")
@@ -425,11 +425,93 @@ def main() -> int:
     if args.workload == "precision":
         print(json.dumps(bench_precision(args)), flush=True)
         return 0
+    if args.workload == "unitary":
+        print(json.dumps(bench_unitary(args)), flush=True)
+        return 0
     out = bench_layered(args, lib)
     if not args.no_extras and args.qubits == 30 and args.dtype == "c128":
         out["extras"] = extras(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def bench_unitary(args, widths=(4, 5, 6, 7)) -> dict:
+    """Dense m-qubit unitaries (UnitaryTransformation) on an n-qubit state:
+    one Haar-random 2^m x 2^m block on the middle qubits per launch.  m = 4..6
+    run on MFMA (unitary_mfma_kernel), wider ones on the FMA kernel
+    (unitary_kernel).  Per launch: HBM rate of the in-place read + write of
+    the state and the complex multiply-add rate (8 flops per complex MAC, 2^m
+    MACs per amplitude), from the library's HIP-event profile."""
+    from code_mojo_amd import _lib
+    from code_mojo_amd.lowering import OpList
+    from code_mojo_amd.state import GpuState, Plan
+
+    lib = _lib.load()
+    n = args.qubits if args.qubits <= 28 else 28
+    rng = np.random.default_rng(args.seed)
+    rows = {}
+    for dname, dtype, b in (("c128", np.float64, 16), ("c64", np.float32, 8)):
+        g = GpuState(n, dtype)
+        g.fill_random(args.seed)
+        for m in widths:
+            z = rng.normal(size=(1 << m, 1 << m)) + 1j * rng.normal(size=(1 << m, 1 << m))
+            q, r = np.linalg.qr(z)
+            out = OpList(n)
+            for _ in range(4):
+                out.unitary(q * (np.diag(r) / np.abs(np.diag(r))), n // 2 - m // 2, m, [])
+            ops, ud = out.finish()
+            plan = Plan(n, dtype, ops, ud, _lib.opts(flags=_lib.FLAG_PROFILE))
+            plan.execute(g)
+            g.synchronize()
+            lib.bfly_profile_reset()
+            for _ in range(max(args.steps, 1)):
+                plan.execute(g)
+            g.synchronize()
+            fam = profile_families(lib).get("unitary_kernel", (0.0, 0))
+            plan.free()
+            ms = fam[0] / max(fam[1], 1)
+            gbs = 2 * (1 << n) * b / (ms * 1e-3) / 1e9
+            tflops = (1 << n) * (1 << m) * 8 / (ms * 1e-3) / 1e12
+            kernel = "unitary_mfma_kernel" if 4 <= m <= 6 else "unitary_kernel"
+            rows[f"{dname}_m{m}"] = {"kernel": kernel, "ms_per_launch": round(ms, 4), "hbm_gbs": round(gbs, 1),
+                                     "frac_of_hbm_peak": round(gbs / 8000.0, 4), "tflops": round(tflops, 2),
+                                     "launches": fam[1]}
+            log(f"[bench] unitary {dname} m={m}: {ms:.3f} ms per launch, {gbs:.0f} GB/s, {tflops:.1f} TFLOP/s "
+                f"({kernel})")
+        g.free()
+    return {"metric": f"dense m-qubit unitary on a {n}-qubit state, per launch", "qubits": n, "results": rows,
+            "peaks": {"hbm_gbs": 8000.0, "fp64_tflops": 78.6, "fp32_matrix_tflops": 157.3}}
+
+
+def bench_host_boundary(args) -> dict:
+    """The headline circuit through bfly_execute_host, the one-shot form a
+    Mojo ExecutorFn binds: host re / im in (pageable numpy arrays), the state
+    allocated, uploaded over PCIe, run and downloaded, in one call.  The
+    PCIe-inclusive rate; never the headline value."""
+    from code_mojo_amd import _lib, lower_circuit
+    from code_mojo_amd import workloads as W
+    from code_mojo_amd.state import Plan
+
+    n = args.qubits
+    ops, ud = lower_circuit(W.random_layered(n, args.layers, args.seed))
+    opts = _lib.opts(flags=_lib.FLAG_JIT if args.jit == "on" else 0)
+    passes = Plan(n, np.float64, ops, ud, opts).stats()["passes"]   # compiles (or reuses) the kernels
+    re = np.zeros(1 << n)
+    im = np.zeros(1 << n)
+    re[:] = 0.0                       # touch every page before the timed call
+    im[:] = 0.0
+    re[0] = 1.0
+    lib = _lib.load()
+    t0 = time.perf_counter()
+    _lib.check(lib.bfly_execute_host(n, 0, re.ctypes.data, im.ctypes.data, re.size, _lib.ops_ptr(ops), ops.size,
+                                     _lib.udata_ptr(ud), ud.size // 2, opts))
+    dt = time.perf_counter() - t0
+    probe = float(np.sum(re[:1 << 20] ** 2 + im[:1 << 20] ** 2))
+    moved = passes * 2 * (1 << n) * 16
+    log(f"[bench] host boundary: {n}q execute_host {dt:.3f} s ({moved / dt / 1e9:.1f} GB/s state, PCIe included)")
+    return {"what": f"{n}-qubit complex128 layered circuit via bfly_execute_host (allocate, upload, run, download, free)",
+            "seconds": round(dt, 4), "state_gbps_pcie_inclusive": round(moved / dt / 1e9, 1),
+            "state_bytes_each_way": 2 * (1 << n) * 16 // 2, "probe_prob_first_2^20": probe}
 
 
 def extras(args) -> dict:
@@ -462,6 +544,12 @@ def extras(args) -> dict:
                                         "speedup_vs_one_circuit_per_launch", "saturated", "at_rate",
                                         "small_requests_12_20q")}
     res["serving"]["config"] = v["config"]
+    u = copy.copy(args)
+    u.qubits, u.steps = 28, 3
+    res["unitary"] = bench_unitary(u, widths=(4, 6))
+    log(f"[bench] extras: unitary done at {time.perf_counter() - t0:.1f}s")
+    res["host_boundary"] = bench_host_boundary(args)
+    log(f"[bench] extras: host boundary done at {time.perf_counter() - t0:.1f}s")
     res["wall_s"] = round(time.perf_counter() - t0, 1)
     log(f"[bench] extras: serving done at {res['wall_s']}s")
     return res

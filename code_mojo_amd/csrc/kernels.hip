@@ -16,8 +16,9 @@
 //                         per pass (generalises execute_grid_fused's row-local
 //                         groups, executors.mojo:1062-1093)
 //   (c) bitrev kernels    in place: 32x32 LDS tiles pair index (h,m,l) with
-//                         (rev l, rev m, rev h), both sides coalesced
-//                         (bit_reverse_state, state.mojo:131-204, is out of place)
+//                         (rev l, rev m, rev h), re and im together, both sides
+//                         coalesced (bit_reverse_state, state.mojo:131-204, is
+//                         out of place)
 //   (d) reduce kernels    FP64 TwoSum-compensated sums with a fixed reduction
 //                         tree (deterministic), for norms / probabilities /
 //                         inner products (grover_classical.mojo:8-31,109-131)
@@ -25,10 +26,12 @@
 //                         phase factors (BLOCK mode)
 //   flip / diffuse        phase flips and block-mean reflections
 //                         (grover_classical.mojo:262-326)
-//   unitary_kernel        dense 2^m blocks (transformations_unitary.mojo:22-139)
+//   unitary kernels       dense 2^m blocks (transformations_unitary.mojo:22-139):
+//                         m = 4..6 on MFMA (FP64 / FP32), others on FMA
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 
 #include "kernels.h"
@@ -1645,63 +1648,88 @@ __device__ __forceinline__ uint64_t rev_bits64(uint64_t x, int nbits) {
 }
 
 // Index i = (h: top B bits | m: middle M bits | l: low B bits); its reversal is
-// (rev l | rev m | rev h).  Block m (with m <= rev m) swaps tile m with tile
-// rev m through LDS, re first and then im (two W x W buffers, padded rows);
-// loads and stores are 16-byte vectors along l (W contiguous amplitudes: 512
-// bytes per row for complex128 at W = 64).  Pure data movement: bit-exact.
+// (rev l | rev m | rev h).  Tile m (with m <= rev m) is exchanged with tile
+// rev m through LDS, re and im together: complex-interleaved W x W tiles
+// (padded rows), 16-byte vectors along l, one load phase and one store phase
+// per tile pair.  Pure data movement: bit-exact.  The block order keeps the
+// pages in use few.  Tile m's rows lie
+// 2^(n-B) amplitudes apart and each page holds a run of consecutive m; the
+// partner tile rev m lies in the page run of m's low bits.  Consecutive
+// iterations u therefore step through m's middle bits first (u bits [0, c)
+// -> m bits [a, a + c), u bits [c, c + a) -> m bits [0, a), the rest above),
+// so both the blocks in flight and their partners share a few pages.  The
+// grid is persistent (blocks loop over u).
 template <typename T, int B>
-__global__ __launch_bounds__(256) void bitrev_tiled_kernel(T* __restrict__ re, T* __restrict__ im, int n) {
+__global__ __launch_bounds__(256) void bitrev_joint_kernel(T* __restrict__ re, T* __restrict__ im, int n) {
     constexpr int W = 1 << B;
-    constexpr int VN = 16 / static_cast<int>(sizeof(T));   // amplitudes per 16-byte vector
-    constexpr int RV = W / VN;                              // vectors per row
+    constexpr int VN = 16 / static_cast<int>(sizeof(T));
+    constexpr int RV = W / VN;
     using V = typename Vec<T>::type;
-    __shared__ T ta[W][W + 1], tb[W][W + 1];
+    struct C2 {
+        T r, i;
+    };
+    __shared__ C2 ta[W][W + 1], tb[W][W + 1];
     const int M = n - 2 * B;
     const int hs = n - B;
-    // blocks walk the middle indices (grid capped by the launcher)
-    for (uint64_t m = blockIdx.x; m < (1ull << M); m += gridDim.x) {
+    const int c = M < 8 ? M : 8;
+    const int a = (M - c) / 2;
+    const uint64_t cm = (1ull << c) - 1, am = (1ull << a) - 1;
+    for (uint64_t u = blockIdx.x; u < (1ull << M); u += gridDim.x) {
+        const uint64_t m = ((u & cm) << a) | ((u >> c) & am) | ((u >> (c + a)) << (a + c));
         const uint64_t rm = rev_bits64(m, M);
         if (m > rm) continue;   // uniform across the block
         const bool self = (m == rm);
-        for (int arr = 0; arr < 2; ++arr) {
-            T* p = arr == 0 ? re : im;
-            for (int e = threadIdx.x; e < W * RV; e += 256) {
-                const int h = e / RV, l0 = (e % RV) * VN;
-                const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
-                const V va = *reinterpret_cast<const V*>(p + ia);
-                const T* xa = reinterpret_cast<const T*>(&va);
-#pragma unroll
-                for (int j = 0; j < VN; ++j) ta[h][l0 + j] = xa[j];
-                if (!self) {
-                    const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
-                    const V vb = *reinterpret_cast<const V*>(p + ib);
-                    const T* xb = reinterpret_cast<const T*>(&vb);
-#pragma unroll
-                    for (int j = 0; j < VN; ++j) tb[h][l0 + j] = xb[j];
-                }
+        for (int e = threadIdx.x; e < W * RV; e += 256) {
+            const int h = e / RV, l0 = (e % RV) * VN;
+            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+            const V ra = *reinterpret_cast<const V*>(re + ia);
+            const V ia_ = *reinterpret_cast<const V*>(im + ia);
+            V rb, ib_;
+            if (!self) {
+                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                rb = *reinterpret_cast<const V*>(re + ib);
+                ib_ = *reinterpret_cast<const V*>(im + ib);
             }
-            __syncthreads();
-            for (int e = threadIdx.x; e < W * RV; e += 256) {
-                const int h = e / RV, l0 = (e % RV) * VN;
-                const int rh = static_cast<int>(rev_bits32(static_cast<uint32_t>(h), B));
-                V oa, ob;
-                T* ya = reinterpret_cast<T*>(&oa);
-                T* yb = reinterpret_cast<T*>(&ob);
+            const T* xr = reinterpret_cast<const T*>(&ra);
+            const T* xi = reinterpret_cast<const T*>(&ia_);
 #pragma unroll
-                for (int j = 0; j < VN; ++j) {
-                    const int rl = static_cast<int>(rev_bits32(static_cast<uint32_t>(l0 + j), B));
-                    ya[j] = self ? ta[rl][rh] : tb[rl][rh];
-                    yb[j] = ta[rl][rh];
-                }
-                const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
-                *reinterpret_cast<V*>(p + ia) = oa;
-                if (!self) {
-                    const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
-                    *reinterpret_cast<V*>(p + ib) = ob;
-                }
+            for (int j = 0; j < VN; ++j) ta[h][l0 + j] = C2{xr[j], xi[j]};
+            if (!self) {
+                const T* yr = reinterpret_cast<const T*>(&rb);
+                const T* yi = reinterpret_cast<const T*>(&ib_);
+#pragma unroll
+                for (int j = 0; j < VN; ++j) tb[h][l0 + j] = C2{yr[j], yi[j]};
             }
-            __syncthreads();   // the next array reuses the buffers
         }
+        __syncthreads();
+        for (int e = threadIdx.x; e < W * RV; e += 256) {
+            const int h = e / RV, l0 = (e % RV) * VN;
+            const int rh = static_cast<int>(rev_bits32(static_cast<uint32_t>(h), B));
+            V oar, oai, obr, obi;
+            T* par = reinterpret_cast<T*>(&oar);
+            T* pai = reinterpret_cast<T*>(&oai);
+            T* pbr = reinterpret_cast<T*>(&obr);
+            T* pbi = reinterpret_cast<T*>(&obi);
+#pragma unroll
+            for (int j = 0; j < VN; ++j) {
+                const int rl = static_cast<int>(rev_bits32(static_cast<uint32_t>(l0 + j), B));
+                const C2 x = self ? ta[rl][rh] : tb[rl][rh];
+                const C2 y = ta[rl][rh];
+                par[j] = x.r;
+                pai[j] = x.i;
+                pbr[j] = y.r;
+                pbi[j] = y.i;
+            }
+            const uint64_t ia = (static_cast<uint64_t>(h) << hs) | (m << B) | static_cast<uint64_t>(l0);
+            *reinterpret_cast<V*>(re + ia) = oar;
+            *reinterpret_cast<V*>(im + ia) = oai;
+            if (!self) {
+                const uint64_t ib = (static_cast<uint64_t>(h) << hs) | (rm << B) | static_cast<uint64_t>(l0);
+                *reinterpret_cast<V*>(re + ib) = obr;
+                *reinterpret_cast<V*>(im + ib) = obi;
+            }
+        }
+        __syncthreads();   // the next tile pair reuses the buffers
     }
 }
 
@@ -2474,23 +2502,19 @@ hipError_t launch_gate_t(T* re, T* im, const GateLaunch& G, hipStream_t st) {
     return hipGetLastError();
 }
 
-// One block per middle index up to 30 qubits (complex128); beyond, blocks
-// loop, which keeps blocks x threads within the 2^32 launch limit.
-constexpr uint64_t kBitrevMaxBlocks = 1ull << 20;
+// Persistent grid of 4096 blocks (W = 32: 34 KiB of LDS for complex128,
+// 17 KiB for complex64).  Measured at 30 qubits: complex128 7.26 ms, complex64
+// 4.02 ms, against 9.36 / 5.33 ms for re and im in separate phases with one
+// block per tile pair in index order.
+constexpr unsigned kBitrevBlocks = 4096;
 
 template <typename T>
 hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
     const uint64_t size = 1ull << n;
-    if (mask == 0 && n >= 12 && sizeof(T) == 4) {
-        const uint64_t blocks = std::min<uint64_t>(1ull << (n - 12), kBitrevMaxBlocks);   // 64 x 64 tiles
-                                                                                      // (complex64: 33 KiB of LDS)
-        hipLaunchKernelGGL((bitrev_tiled_kernel<T, 6>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
-                           n);
-    } else if (mask == 0 && n >= 10) {
-        const uint64_t blocks = std::min<uint64_t>(1ull << (n - 10), kBitrevMaxBlocks);   // 32 x 32 tiles
-                                          // (complex128 at 64 x 64 would hold 65 KiB of LDS: two blocks per CU, slower)
-        hipLaunchKernelGGL((bitrev_tiled_kernel<T, 5>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, re, im,
-                           n);
+    if (mask == 0 && n >= 10) {
+        const uint64_t tiles = 1ull << (n - 10);
+        const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(tiles, kBitrevBlocks));
+        hipLaunchKernelGGL((bitrev_joint_kernel<T, 5>), dim3(blocks), dim3(256), 0, st, re, im, n);
     } else {
         if (mask == 0) mask = size - 1;
         hipLaunchKernelGGL(bitrev_simple_kernel<T>, dim3(grid_for(size)), dim3(256), 0, st, re, im, size, mask, n);
@@ -2674,6 +2698,101 @@ hipError_t launch_swap(int dtype, void* re, void* im, int n, int a, int b, hipSt
     return hipGetLastError();
 }
 
+// Dense 2^M blocks, M = 4..6, on MFMA: every wave takes 16 groups (the N
+// dimension) at a time and computes, with A = [Ur, -Ui] (D x 2D, D = 2^M),
+//   [re'] = A [re; im]   and   [im'] = A [im; -re]
+// over the K = 2D inputs of each group.  A is staged in LDS once per
+// workgroup as At[k][row] (k-major); the lane's B operand (its group's inputs
+// k = 4 s + lane / 16) is loaded from HBM into registers, all 2D of them
+// before any output of the wave's groups is written, so the update is in
+// place.  complex128: v_mfma_f64_16x16x4f64 (FP64 products and sums; lane l
+// holds output rows l / 16 + 4 v of a 16-row block); complex64:
+// v_mfma_f32_16x16x4f32 (FP32 products and sums, the reference's FLOAT32
+// arithmetic; lane l holds rows 4 (l / 16) + v).  Groups past the end load
+// zeros and store nothing: an MFMA column depends on its own B column only.
+constexpr int kUMfmaThreads = 256;
+
+template <typename T, int M>
+__global__ __launch_bounds__(kUMfmaThreads) void unitary_mfma_kernel(T* __restrict__ re, T* __restrict__ im,
+                                                                     const UnitaryArgs A,
+                                                                     const double* __restrict__ u) {
+    constexpr int D = 1 << M;
+    constexpr int NS = D / 4;    // K steps per half (re part, im part)
+    constexpr int NR = D / 16;   // 16-row output blocks
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    T* At = reinterpret_cast<T*>(lds_raw);   // At[k * D + row], k < 2D
+    for (int i = threadIdx.x; i < 2 * D * D; i += kUMfmaThreads) {
+        const int k = i / D, row = i % D;
+        At[i] = k < D ? static_cast<T>(u[2 * (row * D + k)]) : static_cast<T>(-u[2 * (row * D + (k - D)) + 1]);
+    }
+    __syncthreads();
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    const int g = lane >> 4, c = lane & 15;
+    constexpr bool kF64 = sizeof(T) == 8;
+    typedef T acc_t __attribute__((ext_vector_type(4)));
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * (kUMfmaThreads / 4);
+    for (uint64_t g0 = (static_cast<uint64_t>(blockIdx.x) * (kUMfmaThreads / 64) + wave) * 16; g0 < A.ngroups;
+         g0 += step) {
+        const uint64_t grp = g0 + static_cast<uint64_t>(c);
+        const bool live = grp < A.ngroups;
+        const uint64_t base = live ? unitary_base(grp, A) : 0;
+        T xr[NS], xi[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const uint64_t idx = base + (static_cast<uint64_t>(4 * s + g) << A.target);
+            xr[s] = live ? re[idx] : T(0);
+            xi[s] = live ? im[idx] : T(0);
+        }
+        // one 16-row output block at a time keeps 16 accumulator values per
+        // lane live (beside the 2D inputs), not all D rows
+#pragma unroll 1
+        for (int r = 0; r < NR; ++r) {
+            acc_t acc_r = acc_t{T(0), T(0), T(0), T(0)};
+            acc_t acc_i = acc_t{T(0), T(0), T(0), T(0)};
+            const T* at = At + 16 * r + c;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const T a_re = at[(4 * s + g) * D];         // Ur[row][4 s + g]
+                const T a_im = at[(D + 4 * s + g) * D];     // -Ui[row][4 s + g]
+                if constexpr (kF64) {
+                    acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re, xr[s], acc_r, 0, 0, 0);
+                    acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im, xi[s], acc_r, 0, 0, 0);
+                    acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(a_re, xi[s], acc_i, 0, 0, 0);
+                    acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(a_im, -xr[s], acc_i, 0, 0, 0);
+                } else {
+                    acc_r = __builtin_amdgcn_mfma_f32_16x16x4f32(a_re, xr[s], acc_r, 0, 0, 0);
+                    acc_r = __builtin_amdgcn_mfma_f32_16x16x4f32(a_im, xi[s], acc_r, 0, 0, 0);
+                    acc_i = __builtin_amdgcn_mfma_f32_16x16x4f32(a_re, xi[s], acc_i, 0, 0, 0);
+                    acc_i = __builtin_amdgcn_mfma_f32_16x16x4f32(a_im, -xr[s], acc_i, 0, 0, 0);
+                }
+            }
+            // every input of the wave's groups is in registers already: the
+            // rows of this block can be stored before the next block runs
+            if (live) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = 16 * r + (kF64 ? g + 4 * v : 4 * g + v);
+                    const uint64_t idx = base + (static_cast<uint64_t>(row) << A.target);
+                    re[idx] = acc_r[v];
+                    im[idx] = acc_i[v];
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int M>
+hipError_t launch_unitary_mfma_t(T* re, T* im, const UnitaryArgs& A, const double* d_u, hipStream_t stream) {
+    constexpr int D = 1 << M;
+    const size_t lds = 2ull * D * D * sizeof(T);
+    const uint64_t waves = (A.ngroups + 15) / 16;
+    const uint64_t blocks = (waves + 3) / 4;
+    const unsigned grid = static_cast<unsigned>(blocks < 4096 ? blocks : 4096);
+    hipLaunchKernelGGL((unitary_mfma_kernel<T, M>), dim3(grid), dim3(kUMfmaThreads), lds, stream, re, im, A, d_u);
+    return hipGetLastError();
+}
+
 hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int m, uint64_t controls,
                           const double* d_u, hipStream_t stream) {
     UnitaryArgs A{};
@@ -2684,6 +2803,20 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
     for (int b = 0; b < n; ++b)
         if ((bits >> b) & 1) A.pos[A.npos++] = b;
     A.ngroups = 1ull << (n - A.npos);
+    if (m >= 4 && m <= 6) {
+        if (dtype == BFLY_C64) {
+            float* r = static_cast<float*>(re);
+            float* i = static_cast<float*>(im);
+            return m == 4 ? launch_unitary_mfma_t<float, 4>(r, i, A, d_u, stream)
+                 : m == 5 ? launch_unitary_mfma_t<float, 5>(r, i, A, d_u, stream)
+                          : launch_unitary_mfma_t<float, 6>(r, i, A, d_u, stream);
+        }
+        double* r = static_cast<double*>(re);
+        double* i = static_cast<double*>(im);
+        return m == 4 ? launch_unitary_mfma_t<double, 4>(r, i, A, d_u, stream)
+             : m == 5 ? launch_unitary_mfma_t<double, 5>(r, i, A, d_u, stream)
+                      : launch_unitary_mfma_t<double, 6>(r, i, A, d_u, stream);
+    }
     const int dim = 1 << m;
     const int G = dim >= 256 ? 1 : 256 / dim;
     const size_t lds = 2ull * G * dim * (dtype == BFLY_C64 ? sizeof(float) : sizeof(double));
