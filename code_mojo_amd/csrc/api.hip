@@ -844,6 +844,13 @@ hipError_t launch_step(const Step& st, bfly_state* s, bfly_plan* p, hipStream_t 
 
 namespace {
 
+// Largest batch captured into one graph: graphs of up to about 1,700
+// launches (the 4096-circuit batching workload) run in every test and
+// benchmark, and nothing larger has been run.  Larger batches run without a
+// graph.
+constexpr size_t kMaxGraphLaunches = 2048;
+
+
 hipError_t enqueue_batch(bfly_batch* b) {
     for (const auto& L : b->launches) {
         hipError_t e;
@@ -1003,6 +1010,14 @@ int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* co
     // as the batch, nothing but kernels is captured).  The direct launches are
     // the same kernels in the same order.
     const bool profiled = std::getenv("ROCP_TOOL_LIBRARIES") != nullptr;
+    if ((flags & BFLY_BATCH_GRAPH) && b->launches.size() > kMaxGraphLaunches) {
+        const size_t nl = b->launches.size();
+        free_batch(b);
+        return fail(BFLY_EUNSUPPORTED, "a batch graph of " + std::to_string(nl) + " launches (more than " +
+                                           std::to_string(kMaxGraphLaunches) +
+                                           "): split the circuits into smaller batches or create it without "
+                                           "BFLY_BATCH_GRAPH");
+    }
     if ((flags & BFLY_BATCH_GRAPH) && !profiled) {
         e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
         if (e == hipSuccess) {

@@ -217,7 +217,8 @@ int bfly_execute_host(int32_t num_qubits, int32_t dtype, void* re, void* im, uin
 typedef struct bfly_batch bfly_batch;
 #define BFLY_BATCH_GRAPH 1    /* capture every round into one hipGraph, replayed by bfly_batch_launch
                                  (not under a rocprofiler-sdk tool, e.g. rocprofv3: there the same
-                                 kernels launch directly on the batch's stream) */
+                                 kernels launch directly on the batch's stream); at most 2048
+                                 launches, BFLY_EUNSUPPORTED beyond */
 int bfly_batch_create(bfly_batch** out, bfly_state* const* states, bfly_plan* const* plans, int32_t count,
                       int32_t flags);
 /* Enqueue all rounds (ordered after pending work on every state's stream;

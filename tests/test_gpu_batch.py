@@ -88,3 +88,20 @@ def test_graph_replays_match_eager(gpu_lib):
             assert np.array_equal(a.re, c.re) and np.array_equal(a.im, c.im), rep
     be.free()
     bg.free()
+
+
+def test_graph_of_more_than_2048_launches_is_refused(gpu_lib):
+    """A batch graph is limited to 2048 launches (BFLY_EUNSUPPORTED beyond,
+    before any capture); the same batch without a graph is accepted."""
+    n = 10
+    ops, ud = lower_circuit(W.random_layered(n, 240, 4))
+    plan = Plan(n, np.float32, ops, ud, _lib.opts(tile_bits=8, max_gates_per_pass=1))
+    assert plan.stats()["launches"] > 2048
+    g = GpuState(n, np.float32)
+    with pytest.raises(_lib.BflyError, match="launches"):
+        Batch([g], [plan], graph=True)
+    b = Batch([g], [plan], graph=False)
+    assert b.stats()["launches"] > 2048
+    b.free()
+    plan.free()
+    g.free()
