@@ -448,7 +448,7 @@ bool jit_supported(const HostPlan& hp, const Step& st) {
     if (st.kind != S_TILE) return false;
     const TilePass& P = st.tile;
     const size_t elem = hp.dtype == BFLY_C64 ? 4 : 8;
-    if (2 * elem * (size_t(1) << P.k) > 64 * 1024) return false;   // static LDS limit
+    if (2 * elem * (size_t(1) << P.k) > 128 * 1024) return false;   // LDS of one workgroup (gfx950: 160 KiB)
     return P.slots == 3 || P.slots == 4;
 }
 
