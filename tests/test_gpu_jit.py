@@ -41,7 +41,8 @@ def oracle(n, ops, ud, st):
 
 
 @pytest.mark.parametrize("n,tile_bits,threads", [(5, 4, 0), (9, 6, 0), (12, 8, 64), (14, 0, 0), (16, 12, 0),
-                                                 (16, 12, 256), (17, 10, 0), (17, 10, 64)])
+                                                 (16, 12, 256), (17, 10, 0), (17, 10, 64), (18, 13, 0),
+                                                 (21, 13, 0)])
 def test_random_ops_vs_oracle(gpu_lib, n, tile_bits, threads):
     rng = np.random.default_rng(300 + n)
     ops, ud = random_ops(rng, n, 70)
