@@ -347,6 +347,8 @@ def main() -> int:
     ap.add_argument("--dtype", choices=["c128", "c64"], default="c128")
     ap.add_argument("--mode", choices=["fused", "unfused"], default="fused")
     ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--low-bits", type=int, default=0,
+                    help="layered: low qubits kept in every tile (contiguous runs of 2^L amplitudes; 0 = default)")
     ap.add_argument("--max-gates", type=int, default=0, help="cap on gates per tile pass (0 = planner default)")
     ap.add_argument("--threads", type=int, default=0,
                     help="tile-pass workgroup size (0 = default; 2^(k-4) selects 16-amplitude register sweeps)")
@@ -571,8 +573,8 @@ def bench_layered(args, lib) -> dict:
     ops, ud = lower_circuit(qc)
     mode = _lib.MODE_FUSED if args.mode == "fused" else _lib.MODE_UNFUSED
     flags = _lib.FLAG_PROFILE | (_lib.FLAG_JIT if args.jit == "on" else 0)
-    plan = Plan(n, dtype, ops, ud, _lib.opts(mode=mode, tile_bits=args.tile_bits, max_gates_per_pass=args.max_gates,
-                                         threads=args.threads, flags=flags))
+    plan = Plan(n, dtype, ops, ud, _lib.opts(mode=mode, tile_bits=args.tile_bits, low_bits=args.low_bits,
+                                         max_gates_per_pass=args.max_gates, threads=args.threads, flags=flags))
     plan_s = time.perf_counter() - t0
     st = plan.stats()
     log(f"[bench] lowered {len(ops)} gates -> {st['passes']} passes ({st['launches']} launches) "
@@ -634,7 +636,7 @@ def bench_layered(args, lib) -> dict:
         "data": "synthetic (seeded random layered circuit, |0..0> start)",
         "config": {"workload": "random_layered_u3_czbrick_cx", "qubits": n, "layers": args.layers,
                    "state": "complex128" if args.dtype == "c128" else "complex64", "seed": args.seed,
-                   "mode": args.mode, "tile_bits": args.tile_bits, "max_gates_per_pass": args.max_gates,
+                   "mode": args.mode, "tile_bits": args.tile_bits, "low_bits": args.low_bits, "max_gates_per_pass": args.max_gates,
                    "threads": args.threads,
                    "gates": int(len(ops)), "passes_per_circuit": st["passes"],
                    "launches_per_circuit": st["launches"], "jit": args.jit == "on",
