@@ -87,3 +87,40 @@ def test_28q_fused_vs_unfused_and_precisions(gpu_lib):
     lo128.free()
     fused.free()
     assert fid >= 1 - 1e-6, fid
+
+
+def test_30q_headline_generated_equals_interpreter_and_round_trip(gpu_lib):
+    """The headline configuration itself (30 qubits, complex128, 40 layers,
+    16 GiB per state), checked through size-independent properties on the
+    device: the generated tile kernels against the interpreting kernel (two
+    code paths; L2 distance <= 1e-12, so every amplitude agrees to 1e-12),
+    the norm, and the circuit followed by its inverse returning |0..0>
+    (<0|U^-1 U|0> = 1 to 1e-12)."""
+    n = 30
+    qc = W.random_layered(n, 40, 2026)
+    ops, ud = lower_circuit(qc)
+    print("[layered-full] 30q x 40 layers: generated vs interpreting kernels", flush=True)
+    jit = run(n, ops, ud, flags=_lib.FLAG_JIT)
+    interp = run(n, ops, ud)
+    d2 = jit.diff_norm2(interp)
+    assert math.sqrt(d2) <= 1e-12, d2
+    assert abs(jit.norm2() - 1.0) <= 1e-12
+    interp.free()
+    # the inverse: ops in reverse order, each gate as its adjoint 2x2 (the
+    # reference's inverse() refuses custom gates such as U3)
+    inv_ops = ops[::-1].copy()
+    for rec in inv_ops:
+        assert rec["kind"] == 0
+        m = rec["mat"][0::2] + 1j * rec["mat"][1::2]
+        adj = m.reshape(2, 2).conj().T.reshape(-1)
+        rec["mat"][0::2], rec["mat"][1::2] = adj.real, adj.imag
+        rec["gate"], rec["theta"] = 8, 0.0          # CUSTOM
+    p = Plan(n, np.float64, inv_ops, ud, _lib.opts(flags=_lib.FLAG_JIT))
+    p.execute(jit)
+    jit.synchronize()
+    p.free()
+    zero = GpuState(n)                       # |0..0>
+    ip = zero.inner(jit)
+    zero.free()
+    jit.free()
+    assert abs(ip - 1.0) <= 1e-12, ip
