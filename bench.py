@@ -355,6 +355,9 @@ def main() -> int:
     ap.add_argument("--jit", choices=["on", "off"], default="on",
                     help="layered: compile every tile pass into its own kernel at plan time (BFLY_FLAG_JIT; "
                          "the compile is part of plan building, outside the timed region)")
+    ap.add_argument("--first-fit", action="store_true",
+                    help="layered: tile passes take their qubits in program order (BFLY_FLAG_FIRST_FIT) "
+                         "instead of the planner's search for the set that carries the most gates")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-qubits", type=int, default=0, help="CPU baseline qubits (0 = the headline's)")
     ap.add_argument("--cpu-layers", type=int, default=1, help="CPU baseline: the first L layers of the circuit")
@@ -573,6 +576,7 @@ def bench_layered(args, lib) -> dict:
     ops, ud = lower_circuit(qc)
     mode = _lib.MODE_FUSED if args.mode == "fused" else _lib.MODE_UNFUSED
     flags = _lib.FLAG_PROFILE | (_lib.FLAG_JIT if args.jit == "on" else 0)
+    flags |= _lib.FLAG_FIRST_FIT if args.first_fit else 0
     plan = Plan(n, dtype, ops, ud, _lib.opts(mode=mode, tile_bits=args.tile_bits, low_bits=args.low_bits,
                                          max_gates_per_pass=args.max_gates, threads=args.threads, flags=flags))
     plan_s = time.perf_counter() - t0
@@ -637,7 +641,7 @@ def bench_layered(args, lib) -> dict:
         "config": {"workload": "random_layered_u3_czbrick_cx", "qubits": n, "layers": args.layers,
                    "state": "complex128" if args.dtype == "c128" else "complex64", "seed": args.seed,
                    "mode": args.mode, "tile_bits": args.tile_bits, "low_bits": args.low_bits, "max_gates_per_pass": args.max_gates,
-                   "threads": args.threads,
+                   "threads": args.threads, "tile_qubit_choice": "first fit" if args.first_fit else "search",
                    "gates": int(len(ops)), "passes_per_circuit": st["passes"],
                    "launches_per_circuit": st["launches"], "jit": args.jit == "on",
                    "plan_build_s": round(plan_s, 2)},

@@ -20,7 +20,7 @@ BFLY_OK, BFLY_EINVAL, BFLY_ENOMEM, BFLY_EHIP, BFLY_EUNSUPPORTED, BFLY_ENUMERIC =
 C128, C64 = 0, 1
 MODE_FUSED, MODE_UNFUSED, MODE_BLOCK, MODE_AUTO = 0, 1, 2, 3
 TUNE_ALL = (1 << MODE_FUSED) | (1 << MODE_BLOCK) | (1 << MODE_AUTO)
-FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS, FLAG_JIT = 1, 2, 4, 8
+FLAG_SYNC, FLAG_PROFILE, FLAG_GENERIC_PAIRS, FLAG_JIT, FLAG_FIRST_FIT = 1, 2, 4, 8, 16
 BATCH_GRAPH = 1
 
 ERROR_NAMES = {BFLY_EINVAL: "EINVAL", BFLY_ENOMEM: "ENOMEM", BFLY_EHIP: "EHIP",

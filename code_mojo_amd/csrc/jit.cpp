@@ -463,7 +463,9 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     const int NT = tile_threads_h(K, S);
     const unsigned NG = TS / G;
     const int WPE = tile_waves_per_eu_h(elem, K, S);
-    const bool prefetch = WPE <= 2;
+    // BFLY_JIT_PREFETCH=0/1 overrides the choice (measurement)
+    bool prefetch = WPE <= 2;
+    if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
 
     JitSource js;
@@ -531,10 +533,20 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("  }\n");
     }
     o("  __syncthreads();\n");
-    o("  const unsigned long long next = tile + gridDim.x;\n  unsigned long long nbase = 0;\n");
-    o("  if (next < %lluull) {\n    nbase = %s;\n", ntiles, tile_base("next").c_str());
-    if (prefetch) load("nbase");
-    o("  }\n");
+    o("  const unsigned long long next = tile + gridDim.x;\n");
+    if (prefetch) {
+        // The next tile's loads are issued before the sweeps and stay in
+        // flight across them.  They are unconditional (the last tile reloads
+        // its own amplitudes, which are never used): a conditional load makes
+        // the compiler copy the loaded registers where the branches join,
+        // which waits for the loads before the first sweep.
+        o("  const unsigned long long nc = next < %lluull ? next : tile;\n", ntiles);
+        o("  const unsigned long long nbase = %s;\n", tile_base("nc").c_str());
+        load("nbase");
+    } else {
+        o("  unsigned long long nbase = 0;\n");
+        o("  if (next < %lluull) nbase = %s;\n", ntiles, tile_base("next").c_str());
+    }
 
     // sweeps
     for (int s = 0; s < P.nsweeps; ++s) {

@@ -1545,6 +1545,107 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
     std::vector<PendingGate> placed;
     PhaseCarry carry;  // phases waiting for a later tile pass
     constexpr size_t kScanWindow = 8192;  // gates examined past the first deferral
+    // per op: qubits acted on non-diagonally (a stand-alone op: all of its
+    // qubits) and all qubits it touches; stand-alone ops defer everything
+    // on their qubits
+    std::vector<uint64_t> op_nd(nops), op_all(nops);
+    std::vector<uint8_t> op_place(nops);
+    for (uint64_t i = 0; i < nops; ++i) {
+        const bfly_op& op = ops[i];
+        op_place[i] = placeable(op) ? 1 : 0;
+        if (!op_place[i]) {
+            op_nd[i] = op_all[i] = op_qubits(op, n);
+        } else {
+            op_all[i] = op.kind == BFLY_OP_PHASE_FLIP ? op.aux : (1ull << op.target) | op.controls;
+            op_nd[i] = cls[i].diag ? 0 : (1ull << op.target);
+        }
+    }
+    // One scan of the remaining ops from `head` in program order: an op joins
+    // the pass when it commutes with every op deferred before it and its
+    // non-diagonal target is a tile qubit.  With `grow` a target outside the
+    // tile joins it while the tile has fewer than k qubits (first fit);
+    // otherwise the tile is fixed.  Returns the number of ops placed and the
+    // final tile; with `record` fills `placed`, `rest` and the scan end.
+    size_t scan_end = head;
+    auto scan = [&](uint64_t tile, bool grow, bool record, uint64_t* tile_out) -> size_t {
+        uint64_t d_nd = 0, d_all = 0;  // qubits of deferred gates: non-diagonal targets / all qubits
+        size_t nplaced = 0;
+        if (record) {
+            placed.clear();
+            rest.clear();
+        }
+        size_t scanned_after_defer = 0;
+        bool deferred_any = false;
+        size_t idx = head;
+        for (; idx < remaining.size(); ++idx) {
+            const uint32_t oi = remaining[idx];
+            if (deferred_any && ++scanned_after_defer > kScanWindow) break;
+            if ((d_nd & all) == all) break;  // every qubit blocked: nothing more can move
+            const uint64_t qnd = op_nd[oi], qall = op_all[oi];
+            bool ok = op_place[oi] && !(qnd & d_all) && !(qall & d_nd) && static_cast<int>(nplaced) < max_gates;
+            if (ok && qnd && !(qnd & tile)) {
+                if (grow && popc(tile) < k) tile |= qnd;
+                else ok = false;
+            }
+            if (!ok) {
+                d_nd |= qnd; d_all |= qall;
+                if (record) rest.push_back(oi);
+                deferred_any = true;
+                continue;
+            }
+            ++nplaced;
+            if (record) placed.push_back(PendingGate{&ops[oi], cls[oi]});
+        }
+        if (record) scan_end = idx;
+        if (tile_out) *tile_out = tile;
+        return nplaced;
+    };
+    // Tile qubits of a pass.  First fit takes the targets in program order.
+    // The search (default) also builds the set one qubit at a time, each time
+    // adding the qubit that lets the most ops into the pass, keeps the better
+    // of the two, then swaps single qubits while that places more ops.  The
+    // reference's grid strategies fix the row-local qubits to the low bits
+    // (executors.mojo:213-228,949-955); the set here is free, so choosing it
+    // well cuts the number of HBM passes (30-qubit layered circuit, 2^12
+    // tiles: 55 passes first fit, 38 after the greedy build, 30 after the
+    // swaps).
+    const bool search = !(opts.flags & BFLY_FLAG_FIRST_FIT) && k < n;
+    constexpr int kMaxSwaps = 64;   // bounds the planner's time on long circuits
+    auto choose_tile = [&]() -> uint64_t {
+        uint64_t t_ff = 0;
+        scan(low_mask, true, false, &t_ff);
+        for (int b = 0; popc(t_ff) < k && b < n; ++b) t_ff |= 1ull << b;
+        const size_t c_ff = scan(t_ff, false, false, nullptr);
+        uint64_t t_gr = low_mask;
+        size_t c_gr = 0;
+        while (popc(t_gr) < k) {
+            int bq = -1;
+            size_t bc = 0;
+            for (int q = 0; q < n; ++q) {
+                if ((t_gr >> q) & 1) continue;
+                const size_t c = scan(t_gr | (1ull << q), false, false, nullptr);
+                if (bq < 0 || c > bc) { bq = q; bc = c; }
+            }
+            t_gr |= 1ull << bq;
+            c_gr = bc;
+        }
+        uint64_t t = c_gr > c_ff ? t_gr : t_ff;
+        size_t cur = std::max(c_gr, c_ff);
+        for (int swaps = 0; swaps < kMaxSwaps; ++swaps) {
+            bool improved = false;
+            for (int a = 0; a < n && !improved; ++a) {
+                if (!((t >> a) & 1) || ((low_mask >> a) & 1)) continue;
+                for (int b = 0; b < n && !improved; ++b) {
+                    if ((t >> b) & 1) continue;
+                    const uint64_t t2 = (t & ~(1ull << a)) | (1ull << b);
+                    const size_t c = scan(t2, false, false, nullptr);
+                    if (c > cur) { cur = c; t = t2; improved = true; }
+                }
+            }
+            if (!improved) break;
+        }
+        return t;
+    };
     while (head < remaining.size()) {
         const bfly_op& front = ops[remaining[head]];
         if (!placeable(front)) {
@@ -1553,40 +1654,13 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
             continue;
         }
         uint64_t tile = low_mask;
-        uint64_t d_nd = 0, d_all = 0;  // qubits of deferred gates: non-diagonal targets / all qubits
-        placed.clear();
-        rest.clear();
-        size_t scanned_after_defer = 0;
-        bool deferred_any = false;
-        size_t idx = head;
-        for (; idx < remaining.size(); ++idx) {
-            const uint32_t oi = remaining[idx];
-            const bfly_op& op = ops[oi];
-            if (deferred_any && ++scanned_after_defer > kScanWindow) break;
-            if ((d_nd & all) == all) break;  // every qubit blocked: nothing more can move
-            if (!placeable(op)) {
-                const uint64_t q = op_qubits(op, n);
-                d_nd |= q; d_all |= q;
-                rest.push_back(oi);
-                deferred_any = true;
-                continue;
-            }
-            const Classified& c = cls[oi];
-            const uint64_t qall = op.kind == BFLY_OP_PHASE_FLIP ? op.aux : (1ull << op.target) | op.controls;
-            const uint64_t qnd = c.diag ? 0 : (1ull << op.target);
-            bool ok = !(qnd & d_all) && !(qall & d_nd) && static_cast<int>(placed.size()) < max_gates;
-            if (ok && qnd && !(qnd & tile)) {
-                if (popc(tile) < k) tile |= qnd;
-                else ok = false;
-            }
-            if (!ok) {
-                d_nd |= qnd; d_all |= qall;
-                rest.push_back(oi);
-                deferred_any = true;
-                continue;
-            }
-            placed.push_back(PendingGate{&op, c});
+        if (search) {
+            tile = choose_tile();
+            scan(tile, false, true, &tile);
+        } else {
+            scan(low_mask, true, true, &tile);
         }
+        const size_t idx = scan_end;
         if (placed.empty()) {
             err = "internal planner error: a pass placed no gate";
             return BFLY_EINVAL;

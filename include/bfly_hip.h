@@ -121,6 +121,9 @@ typedef struct bfly_exec_opts {
                                  (hipRTC, at bfly_plan_create; cached per process by pass structure);
                                  same amplitudes as the interpreting tile kernel.  Worth it for plans
                                  executed many times: compiling takes about 0.3 s per new pass */
+#define BFLY_FLAG_FIRST_FIT 16 /* fused mode: each tile pass takes its qubits in program order (first
+                                  fit) instead of searching for the set that carries the most gates
+                                  (faster planning, usually more passes) */
 
 typedef struct bfly_state bfly_state;   /* opaque: device-resident SoA state */
 typedef struct bfly_plan  bfly_plan;    /* opaque: compiled pass schedule    */
