@@ -110,9 +110,11 @@ struct Flip {
 };
 
 struct IrOp {
-    enum Kind { ROT, PHASE, NEG, TEXT } kind = TEXT;
+    enum Kind { ROT, PHASE, NEG, SCALE, TEXT } kind = TEXT;
     int slot = -1;
     int ct = -1, cr = -1, ci = -1;
+    double zr = 1.0, zi = 0.0;     // ROT: pre-phase, PHASE: phase, SCALE: factor (FP64 values)
+    bool hoisted = false;          // ROT: pre-phase applied by the sweep's member factors
     bool form1 = false;
     unsigned members = 0;          // NEG
     std::string cond;              // NEG: condition ("" = always)
@@ -144,13 +146,15 @@ void lower_instr(std::vector<IrOp>& ir, const TileInstr& in, int S, bool f32, Cs
         std::snprintf(b, sizeof b, "((jb & %uu) == %uu)", in.lmask, in.lval);
         cond = and_cond(cond, b);
     }
-    auto rot = [&](int slot, int ct, int cr, int ci, bool form1) {
+    auto rot = [&](int slot, int ct, int cr, int ci, bool form1, double zr, double zi) {
         IrOp op;
         op.kind = IrOp::ROT;
         op.slot = slot;
         op.ct = ct;
         op.cr = cr;
         op.ci = ci;
+        op.zr = zr;
+        op.zi = zi;
         op.form1 = form1;
         if (!cond.empty()) {
             // a conditional rotation stays as written
@@ -181,12 +185,12 @@ void lower_instr(std::vector<IrOp>& ir, const TileInstr& in, int S, bool f32, Cs
     if (S == 3 && kind == T_ROT3) {
         for (int a = 0; a < 3; ++a) {
             const int ct = C(3 * a), cr = C(3 * a + 1), ci = C(3 * a + 2);
-            rot(a, ct, cr, ci, false);
+            rot(a, ct, cr, ci, false, md[3 * a + 1], md[3 * a + 2]);
         }
     } else if (kind == T_ROT) {
         const bool form1 = f32 ? mf[1] != 0.0f : md[1] != 0.0;
         const int ct = C(0), cr = C(2), ci = C(3);
-        rot(in.slot, ct, cr, ci, form1);
+        rot(in.slot, ct, cr, ci, form1, md[2], md[3]);
     } else if (S == 3 && kind == T_ROTS) {
         for (int a = 0; a < S; ++a) {
             if (!((in.cslots >> a) & 1u)) continue;
@@ -195,7 +199,7 @@ void lower_instr(std::vector<IrOp>& ir, const TileInstr& in, int S, bool f32, Cs
             const bool form1 = f32 ? std::signbit(hf) : std::signbit(hd);
             const int ct = cst(std::fabs(hd), std::fabs(hf));
             const int cr = C(3 * a + 1), ci = C(3 * a + 2);
-            rot(a, ct, cr, ci, form1);
+            rot(a, ct, cr, ci, form1, md[3 * a + 1], md[3 * a + 2]);
         }
     } else if (kind == T_PH) {
         const int cr = C(0), ci = C(1);
@@ -205,11 +209,20 @@ void lower_instr(std::vector<IrOp>& ir, const TileInstr& in, int S, bool f32, Cs
             op.slot = in.slot;
             op.cr = cr;
             op.ci = ci;
+            op.zr = md[0];
+            op.zi = md[1];
             ir.push_back(op);
         } else {
             std::snprintf(b, sizeof b, "    slot_phase<T, %d, %d>(ar, ai, C[%d], C[%d]);\n", G, int(in.slot), cr, ci);
             text(b, 0);
         }
+    } else if (kind == T_SCALE && cond.empty()) {
+        // a scalar: the sweep folds it into its member factors
+        IrOp op;
+        op.kind = IrOp::SCALE;
+        op.zr = md[0];
+        op.zi = md[1];
+        ir.push_back(op);
     } else if (kind == T_SCALE) {
         const int cr = C(0), ci = C(1);
         const bool real = f32 ? mf[1] == 0.0f : md[1] == 0.0;
@@ -370,10 +383,154 @@ std::string flip_expr(const std::vector<Flip>& fl, int m) {
     return e;
 }
 
-void emit_ops(Out& o, const std::vector<IrOp>& ir, int S) {
+// Member factors: every diagonal action of a sweep that commutes with all
+// the ops still ahead of it (pre-phases of rotations on slots nothing earlier
+// acts on non-diagonally, phases, unconditional sign flips, scalars) is
+// applied first, as one complex factor per register member, computed in FP64
+// on the host.  A three-rotation sweep then costs 7 complex products and
+// 12 four-FMA pair rotations instead of 12 pre-phased pair rotations (about
+// a fifth fewer FP64 instructions).  Returns the per-member factors; ops
+// folded into them are marked (ROT: pre-phase and pre flips) or dropped.
+struct Cxd {
+    double r, i;
+};
+Cxd cxmul(Cxd a, Cxd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+
+bool flips_unconditional(const std::vector<Flip>& fl) {
+    for (const Flip& f : fl)
+        if (!f.cond.empty()) return false;
+    return true;
+}
+
+std::vector<Cxd> hoist_diagonals(std::vector<IrOp>& ir, int S) {
+    const int G = 1 << S;
+    std::vector<Cxd> F(static_cast<size_t>(G), Cxd{1.0, 0.0});
+    // BFLY_JIT_HOIST=0 keeps every diagonal action in place (measurement)
+    if (const char* h = std::getenv("BFLY_JIT_HOIST"))
+        if (h[0] == '0') return F;
+    std::vector<IrOp> out;
+    unsigned nd_before = 0;   // slots acted on non-diagonally by the ops kept so far
+    auto commutes_with_kept = [&](unsigned M) {
+        for (const IrOp& y : out)
+            if (!neg_commutes(M, y, G)) return false;
+        return true;
+    };
+    auto flips_commute = [&](const std::vector<Flip>& fl) {
+        for (const Flip& f : fl)
+            if (!commutes_with_kept(f.members)) return false;
+        return true;
+    };
+    auto apply_flips = [&](const std::vector<Flip>& fl) {
+        for (const Flip& f : fl)
+            for (int m = 0; m < G; ++m)
+                if ((f.members >> m) & 1u) F[static_cast<size_t>(m)] = Cxd{-F[static_cast<size_t>(m)].r, -F[static_cast<size_t>(m)].i};
+    };
+    auto apply_phase = [&](int slot, double zr, double zi) {
+        for (int m = 0; m < G; ++m)
+            if ((m >> slot) & 1) F[static_cast<size_t>(m)] = cxmul(F[static_cast<size_t>(m)], Cxd{zr, zi});
+    };
+    for (IrOp& x : ir) {
+        const unsigned sb = x.slot >= 0 ? 1u << x.slot : 0u;
+        if (x.kind == IrOp::ROT) {
+            if (!(nd_before & sb) && flips_unconditional(x.pre) && flips_commute(x.pre)) {
+                apply_phase(x.slot, x.zr, x.zi);
+                apply_flips(x.pre);
+                x.pre.clear();
+                x.hoisted = true;
+            }
+            nd_before |= sb;
+            out.push_back(x);
+        } else if (x.kind == IrOp::PHASE) {
+            if (!(nd_before & sb) && flips_unconditional(x.pre) && flips_unconditional(x.post) &&
+                flips_commute(x.pre) && flips_commute(x.post)) {
+                apply_flips(x.pre);
+                apply_phase(x.slot, x.zr, x.zi);
+                apply_flips(x.post);
+                continue;
+            }
+            out.push_back(x);
+        } else if (x.kind == IrOp::NEG) {
+            if (x.cond.empty() && commutes_with_kept(x.members)) {
+                apply_flips({Flip{"", x.members}});
+                continue;
+            }
+            out.push_back(x);
+        } else if (x.kind == IrOp::SCALE) {
+            for (Cxd& f : F) f = cxmul(f, Cxd{x.zr, x.zi});
+            continue;
+        } else {
+            nd_before |= x.nd;
+            out.push_back(x);
+        }
+    }
+    ir.swap(out);
+    return F;
+}
+
+template <typename Cst>
+void emit_ops(Out& o, std::vector<IrOp>& ir, int S, Cst&& cst) {
     const int G = 1 << S;
     int var = 0;
+    const std::vector<Cxd> F = hoist_diagonals(ir, S);
+    for (int m = 0; m < G; ++m) {
+        const Cxd f = F[static_cast<size_t>(m)];
+        if (f.r == 1.0 && f.i == 0.0) continue;
+        if (f.r == -1.0 && f.i == 0.0) {
+            o("    ar[%d] = -ar[%d]; ai[%d] = -ai[%d];\n", m, m, m, m);
+        } else if (f.i == 0.0) {
+            const int c = cst(f.r, static_cast<float>(f.r));
+            o("    ar[%d] *= C[%d]; ai[%d] *= C[%d];\n", m, c, m, c);
+        } else {
+            const int cr = cst(f.r, static_cast<float>(f.r)), ci = cst(f.i, static_cast<float>(f.i));
+            o("    cmul(ar[%d], ai[%d], C[%d], C[%d]);\n", m, m, cr, ci);
+        }
+    }
     for (const IrOp& x : ir) {
+        if (x.kind == IrOp::ROT && x.hoisted) {
+            // pre-phase already applied.  An unconditional post flip of v is
+            // z = -1 with t -> -t; a lane / tile conditional one negates v
+            // after the rotation (one sign-bit XOR per double).
+            const int lo_mask = (1 << x.slot) - 1;
+            o("    {\n");
+            bool have_pos = false, have_neg = false;
+            const int vp = var++, vn = var++;
+            std::map<std::string, int> masks;
+            for (int p = 0; p < G / 2; ++p) {
+                const int m0 = ((p & ~lo_mask) << 1) | (p & lo_mask);
+                const int m1 = m0 | (1 << x.slot);
+                const std::string fe = flip_expr(x.post, m1);
+                const bool flip = fe == "true";
+                if (flip && !have_neg) { o("      const T t%d = -C[%d];\n", vn, x.ct); have_neg = true; }
+                if (!flip && !have_pos) { o("      const T t%d = C[%d];\n", vp, x.ct); have_pos = true; }
+                o("      rot_pair%dz<%d>(ar[%d], ai[%d], ar[%d], ai[%d], t%d);\n", x.form1 ? 1 : 0, flip ? -1 : 1, m0, m0,
+                  m1, m1, flip ? vn : vp);
+                if (!fe.empty() && !flip) {
+                    auto it = masks.find(fe);
+                    int mv;
+                    if (it == masks.end()) {
+                        mv = var++;
+                        masks.emplace(fe, mv);
+                        o("      const unsigned sg%d = %s ? 0x80000000u : 0u;\n", mv, fe.c_str());
+                    } else {
+                        mv = it->second;
+                    }
+                    o("      ar[%d] = xsign(ar[%d], sg%d); ai[%d] = xsign(ai[%d], sg%d);\n", m1, m1, mv, m1, m1, mv);
+                }
+            }
+            o("    }\n");
+            continue;
+        }
+        if (x.kind == IrOp::SCALE) {
+            // not folded into the member factors (measurement switch)
+            const int cr = cst(x.zr, static_cast<float>(x.zr));
+            if (x.zi == 0.0) {
+                o("    {\n#pragma unroll\n      for (int m = 0; m < %d; ++m) { ar[m] *= C[%d]; ai[m] *= C[%d]; }\n    }\n", G, cr, cr);
+            } else {
+                const int ci = cst(x.zi, static_cast<float>(x.zi));
+                o("    scale_all<T, %d>(ar, ai, C[%d], C[%d]);\n", G, cr, ci);
+            }
+            continue;
+        }
         if (x.kind == IrOp::TEXT) {
             o("%s", x.text.c_str());
         } else if (x.kind == IrOp::NEG) {
@@ -600,7 +757,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         for (int q = 0; q < sw.ninstr; ++q)
             lower_instr(ir, hp.instrs[static_cast<size_t>(sw.first + q)], S, f32, cst);
         fold_signs(ir, S);
-        emit_ops(o, ir, S);
+        emit_ops(o, ir, S, cst);
         for (int m = 0; m < G; ++m)
             o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs ^ %uu)) = e; }\n",
               m, m, moff[m]);

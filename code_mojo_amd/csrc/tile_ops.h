@@ -165,6 +165,39 @@ __device__ __forceinline__ void rot_pair1(T& ur_, T& ui_, T& vr_, T& vi_, T t, T
     vi_ = fmat(t, vi, ui);
 }
 
+// x with its sign bit XORed with bit 31 of s (s = 0 or 0x80000000u): a
+// per-lane sign flip in one integer instruction.
+__device__ __forceinline__ double xsign(double x, unsigned s) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ (static_cast<unsigned long long>(s) << 32));
+}
+__device__ __forceinline__ float xsign(float x, unsigned s) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) ^ s);
+}
+
+// The same rotations with the pre-phase z = SZ (+1 or -1, a compile-time
+// sign): the generated kernels apply a sweep's pre-phases earlier, as one
+// factor per register member, and a post flip of v leaves z = -1.  Four FMAs
+// per pair; the sign folds into the FMA operands.
+template <int SZ, typename T>
+__device__ __forceinline__ void rot_pair0z(T& ur_, T& ui_, T& vr_, T& vi_, T t) {
+    const T ur = ur_, ui = ui_;
+    const T vr = SZ > 0 ? vr_ : -vr_, vi = SZ > 0 ? vi_ : -vi_;
+    const T nt = -t;
+    ur_ = fmat(nt, vr, ur);
+    ui_ = fmat(nt, vi, ui);
+    vr_ = fmat(t, ur, vr);
+    vi_ = fmat(t, ui, vi);
+}
+template <int SZ, typename T>
+__device__ __forceinline__ void rot_pair1z(T& ur_, T& ui_, T& vr_, T& vi_, T t) {
+    const T ur = ur_, ui = ui_;
+    const T vr = SZ > 0 ? vr_ : -vr_, vi = SZ > 0 ? vi_ : -vi_;
+    ur_ = fmat(t, ur, -vr);
+    ui_ = fmat(t, ui, -vi);
+    vr_ = fmat(t, vr, ur);
+    vi_ = fmat(t, vi, ui);
+}
+
 template <typename T, int G, int SLOT>
 __device__ __forceinline__ void slot_rot(T* ar, T* ai, T t, bool form1, T zr, T zi) {
     constexpr int lo_mask = (1 << SLOT) - 1;
