@@ -657,6 +657,30 @@ def bench_layered(args, lib) -> dict:
                      "fp64_frac_of_vector_peak": round(fp64 / (per_launch_ms * 1e-3) / 78.6e12, 4) if fp64 else None},
         "cpu_baseline": None,
     }
+    if args.mode == "fused" and not args.first_fit and not args.no_extras:
+        # The same circuit scheduled with first-fit tile qubits (program
+        # order), timed on the same state after the timed region: the
+        # searched schedule moves fewer bytes in fewer, heavier passes, so
+        # state GB/s falls while the circuit runs faster.
+        ff = Plan(n, dtype, ops, ud, _lib.opts(mode=mode, tile_bits=args.tile_bits, low_bits=args.low_bits,
+                                               max_gates_per_pass=args.max_gates, threads=args.threads,
+                                               flags=flags | _lib.FLAG_FIRST_FIT))
+        ff.execute(g)
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ff.execute(g)
+        g.synchronize()
+        ff_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        ff_passes = ff.stats()["passes"]
+        out["first_fit_schedule"] = {
+            "what": "same circuit, tile qubits taken in program order (BFLY_FLAG_FIRST_FIT)",
+            "passes_per_circuit": ff_passes, "ms_per_circuit": round(ff_ms, 3),
+            "state_gbs": round(ff_passes * state_bytes / (ff_ms / 1e3) / 1e9, 2),
+            "gate_layers_per_s": round(args.layers * 1e3 / ff_ms, 3),
+            "searched_speedup": round(ff_ms / ms_per_step, 3)}
+        log(f"[bench] first-fit schedule: {ff_passes} passes, {ff_ms:.1f} ms per circuit")
+        ff.free()
     plan.free()
     g.free()
     if not args.no_cpu_baseline:

@@ -620,8 +620,10 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     const int NT = tile_threads_h(K, S);
     const unsigned NG = TS / G;
     const int WPE = tile_waves_per_eu_h(elem, K, S);
-    // BFLY_JIT_PREFETCH=0/1 overrides the choice (measurement)
-    bool prefetch = WPE <= 2;
+    // The next tile's loads are issued before the sweeps (30-qubit layered
+    // circuit, 4-slot sweeps: complex128 335.6 vs 343.7 ms, complex64 175.3
+    // vs 183.5 ms); BFLY_JIT_PREFETCH=0 turns it off (measurement).
+    bool prefetch = true;
     if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
 
@@ -637,21 +639,27 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     };
 
     const int B = f32 ? 5 : 4;
-    const unsigned BM = (1u << B) - 1;
     const int E = 2 * elem;   // bytes of a complex element in LDS
-    auto swz = [&](unsigned j) { return j ^ ((j >> B) & BM) ^ ((j >> (2 * B)) & BM); };
+    // LDS tile: complex-interleaved elements (re, im adjacent: one 16-byte
+    // access per complex128 amplitude, 8 bytes per complex64 one), element j
+    // at slot pad(j) = j + (j >> B) + (j >> 2B): one spare slot after every
+    // 2^B and every 2^2B elements, with B = 4 (16 lanes of 16-byte accesses
+    // per LDS cycle) or 5 (32 lanes of 8 bytes).  Lanes whose index bits have
+    // distinct residues mod B fall on distinct banks.  For indices with
+    // disjoint bits the map is additive (no carries: pad(a | b) = pad(a) +
+    // pad(b)), so a sweep's member addresses are one base register plus
+    // immediate offsets; an XOR swizzle needs one XOR per member, which the
+    // compiler hoists out of the tile loop for every sweep at once (208
+    // VGPRs for 13 sweeps of 16 members: spills).
+    auto pad = [&](unsigned j) { return j + (j >> B) + (j >> (2 * B)); };
+    const unsigned TSP = pad(TS - 1) + 1;   // slots of the padded tile
     o("typedef %s T;\ntypedef %s C2;\n", T, f32 ? "float2" : "double2");
-    o("__device__ __forceinline__ unsigned swzc(unsigned j) { return j ^ ((j >> %d) & %uu) ^ ((j >> %d) & %uu); }\n", B,
-      BM, 2 * B, BM);
+    o("__device__ __forceinline__ unsigned padc(unsigned j) { return j + (j >> %d) + (j >> %d); }\n", B, 2 * B);
     o("extern \"C\" __global__ __launch_bounds__(%d, %d) void bfly_tile_pass(T* __restrict__ re, T* __restrict__ im, "
       "const T* __restrict__ Cp) {\n",
       NT, WPE);
     o("  using namespace bfly;\n");
-    // LDS tile: complex-interleaved elements (re, im adjacent: one 16-byte
-    // access per complex128 amplitude, 8 bytes per complex64 one), element j
-    // at slot swz(j) = j ^ ((j >> B) & M) ^ ((j >> 2B) & M) with B = 4 (16
-    // lanes of 16-byte accesses per LDS cycle) or 5 (32 lanes of 8 bytes).
-    o("  __shared__ __attribute__((aligned(16))) C2 s_c[%u];\n", TS);
+    o("  __shared__ __attribute__((aligned(16))) C2 s_c[%u];\n", TSP);
     o("  const unsigned tid = threadIdx.x;\n");
     o("  unsigned long long tile = blockIdx.x;\n  if (tile >= %lluull) return;\n", ntiles);
     // Stage-in / stage-out: each lane moves VN consecutive amplitudes per
@@ -678,15 +686,15 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     };
     o("  unsigned long long base = %s;\n", tile_base("tile").c_str());
     load("base");
-    // swz is linear: the slot of tv + c NT VN + v is swz(tv) ^ swz(c NT VN + v)
-    o("  const unsigned stv = swzc(tv) * %du;\n", E);
+    // the slot of tv + c NT VN + v (disjoint bits) is pad(tv) + pad(c NT VN + v)
+    o("  const unsigned stv = padc(tv) * %du;\n", E);
     o("  for (;;) {\n");
     for (unsigned c = 0; c < PV; ++c) {
         if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
         o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
         for (int v = 0; v < VN; ++v)
-            o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv ^ %uu)) = e; }\n",
-              v, v, swz(c * NT * VN + static_cast<unsigned>(v)) * E);
+            o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = e; }\n",
+              v, v, pad(c * NT * VN + static_cast<unsigned>(v)) * E);
         o("  }\n");
     }
     o("  __syncthreads();\n");
@@ -717,7 +725,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         // sweep keeps the compiler from hoisting every constant of the pass out
         // of the tile loop (which spills scalar registers into vector lanes)
         // thread -> group: the first B thread bits go to non-slot local bits
-        // of distinct residues mod B (distinct LDS bank groups under swz), the
+        // of distinct residues mod B (distinct LDS bank groups under pad), the
         // rest to the remaining non-slot bits in ascending order
         int32_t tpos[kMaxTileBits];
         {
@@ -743,14 +751,14 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         }
         o("  {\n    unsigned zs = 0u;\n    asm volatile(\"\" : \"+s\"(zs));\n    const T* __restrict__ C = Cp + zs;\n");
         o("    const unsigned jb = static_cast<unsigned>(%s);\n", deposit_expr("tid", tpos, 0, K - S).c_str());
-        o("    const unsigned jbs = swzc(jb) * %du;\n", E);
+        o("    const unsigned jbs = padc(jb) * %du;\n", E);
         o("    T ar[%d], ai[%d];\n", G, G);
         unsigned moff[1 << kSlots];
         for (int m = 0; m < G; ++m) {
             unsigned d = 0;
             for (int a = 0; a < S; ++a) d |= ((static_cast<unsigned>(m) >> a) & 1u) << sw.s[a];
-            moff[m] = swz(d) * E;
-            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs ^ %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
+            moff[m] = pad(d) * E;
+            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
               moff[m], m, m);
         }
         std::vector<IrOp> ir;
@@ -759,7 +767,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         fold_signs(ir, S);
         emit_ops(o, ir, S, cst);
         for (int m = 0; m < G; ++m)
-            o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs ^ %uu)) = e; }\n",
+            o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs + %uu)) = e; }\n",
               m, m, moff[m]);
         o("  }\n  __syncthreads();\n");
     }
@@ -770,8 +778,8 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
         o("  { V vr, vi; T* a = reinterpret_cast<T*>(&vr); T* b = reinterpret_cast<T*>(&vi);\n");
         for (int v = 0; v < VN; ++v)
-            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (stv ^ %uu)); a[%d] = e.x; b[%d] = e.y; }\n",
-              swz(c * NT * VN + static_cast<unsigned>(v)) * E, v, v);
+            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (stv + %uu)); a[%d] = e.x; b[%d] = e.y; }\n",
+              pad(c * NT * VN + static_cast<unsigned>(v)) * E, v, v);
         o("    *reinterpret_cast<V*>(re + base + toff + %lluull) = vr; *reinterpret_cast<V*>(im + base + toff + %lluull) = vi; }\n",
           u, u);
     }
