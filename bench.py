@@ -621,6 +621,10 @@ def bench_layered(args, lib) -> dict:
     kname = "bfly_tile_pass" if (args.jit == "on" and dom == "tile_pass_kernel") else dom
     traffic = read_traffic(f"{kname}_{n}q_layered_{args.dtype}_traffic.txt")
     fp64 = read_value(f"{kname}_{n}q_layered_{args.dtype}_pmc.md", "fp64_flops_per_launch")
+    valu = read_value(f"{kname}_{n}q_layered_{args.dtype}_pmc.md", "valu_insts_per_launch")
+    # VALU issue time of one launch: 4 cycles per wave64 instruction on one of
+    # the 1024 SIMDs (256 CUs x 4) at the 2.4 GHz peak engine clock
+    valu_issue_ms = valu * 4 / (1024 * 2.4e9) * 1e3 if valu else None
     # the in-place read + write streaming ceiling measured on MI355X (a copy
     # kernel reading and writing 2 x 8 GiB in place), beside the 8 TB/s spec
     copy_gbs = read_value("hbm_copy_ceiling.md", "copy_read_write_gbs")
@@ -654,7 +658,8 @@ def bench_layered(args, lib) -> dict:
                      "bytes_per_launch": state_bytes,
                      "frac_of_measured_copy_ceiling": round(achieved / copy_gbs, 4) if copy_gbs else None,
                      "fp64_tflops": round(fp64 / (per_launch_ms * 1e-3) / 1e12, 2) if fp64 else None,
-                     "fp64_frac_of_vector_peak": round(fp64 / (per_launch_ms * 1e-3) / 78.6e12, 4) if fp64 else None},
+                     "fp64_frac_of_vector_peak": round(fp64 / (per_launch_ms * 1e-3) / 78.6e12, 4) if fp64 else None,
+                     "valu_issue_frac": round(valu_issue_ms / per_launch_ms, 4) if valu_issue_ms else None},
         "cpu_baseline": None,
     }
     if args.mode == "fused" and not args.first_fit and not args.no_extras:
