@@ -1,0 +1,143 @@
+"""GPU parity of the generated tile-pass kernels' sweep forms, against the
+oracle at the drop-in's bars (complex128 max |amplitude error| <= 1e-12;
+complex64 against the complex128 oracle: L2 <= 2e-5 and fidelity >= 1 - 1e-6):
+
+- 8-amplitude (3-slot) and 16-amplitude (4-slot) register groups at tile
+  sizes 2^10..2^13, on the padded LDS layout;
+- member factors: rotations' pre-phases, phases, sign flips and scale
+  factors applied once per register member, and lane- or tile-conditional
+  flips (a CZ with a lane or outer qubit) as sign XORs.  Circuits rich in
+  X / Y (quarter-turn rotations, the cotangent form) and in CZ / controlled
+  phases between random pairs exercise every folding case;
+- the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place)
+  and BFLY_JIT_PREFETCH=0 (no next-tile loads during the sweeps) give the
+  same amplitudes within the complex128 bar."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from code_mojo_amd import _lib, lower_circuit
+from code_mojo_amd import gates as G
+from code_mojo_amd import workloads as W
+from code_mojo_amd.lowering import OpList
+from code_mojo_amd.state import GpuState, Plan, QuantumState
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_C128 = 1e-12
+
+
+def run(n, ops, ud, st, dtype=np.float64, **kw):
+    g = GpuState(n, dtype)
+    g.upload(QuantumState(n, dtype, st.re.astype(dtype), st.im.astype(dtype)))
+    p = Plan(n, dtype, ops, ud, _lib.opts(flags=_lib.FLAG_JIT, **kw))
+    info = p.jit_info()
+    p.execute(g)
+    out = g.download()
+    p.free()
+    g.free()
+    return out.re.astype(np.float64) + 1j * out.im.astype(np.float64), info
+
+
+def oracle(n, ops, ud, st):
+    re, im = st.re.astype(np.float64).copy(), st.im.astype(np.float64).copy()
+    O.execute(n, re, im, ops, ud)
+    return re + 1j * im
+
+
+def flips_and_quarter_turns(n, count, seed):
+    """X, Y, H, U3 and RX near quarter turns on random qubits, interleaved with
+    CZ, CP and multi-controlled Z between random qubits (the sign and phase
+    records that fold into member factors or lane-conditional XORs)."""
+    rng = np.random.default_rng(seed)
+    ops = OpList(n)
+    for _ in range(count):
+        r = rng.random()
+        t = int(rng.integers(n))
+        others = [q for q in range(n) if q != t]
+        if r < 0.45:
+            k = ["X", "Y", "H", "U3", "RX"][int(rng.integers(5))]
+            if k == "U3":
+                info = G.Custom_Gate(G.U3(float(rng.uniform(0, 6.3)), float(rng.uniform(0, 6.3)),
+                                          float(rng.uniform(0, 6.3))))
+            elif k == "RX":
+                info = G.GateInfo.of(G.GateKind.RX, math.pi * (1 + 1e-9 * rng.normal()))
+            else:
+                info = G.GateInfo.of(G.GateKind.names.index(k))
+            ops.gate(t, [], info.kind, info.gate, info.arg or 0.0)
+        elif r < 0.75:
+            c = int(rng.choice(others))
+            info = G.GateInfo.of(G.GateKind.Z)
+            ops.gate(t, [c], info.kind, info.gate, 0.0)
+        elif r < 0.9:
+            c = int(rng.choice(others))
+            info = G.GateInfo.of(G.GateKind.P, float(rng.uniform(-math.pi, math.pi)))
+            ops.gate(t, [c], info.kind, info.gate, info.arg)
+        else:
+            cs = [int(x) for x in rng.choice(others, size=min(2, len(others)), replace=False)]
+            info = G.GateInfo.of(G.GateKind.Z)
+            ops.gate(t, cs, info.kind, info.gate, 0.0)
+    return ops.finish()
+
+
+# (qubits, tile bits, threads): threads 2^(k-3) selects 3-slot sweeps, 2^(k-4) 4-slot
+SWEEP_CASES = [(16, 10, 128), (16, 10, 64), (17, 11, 256), (17, 11, 128), (18, 12, 512), (18, 12, 256),
+               (19, 13, 1024), (19, 13, 512)]
+
+
+@pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
+def test_layered_sweep_forms_vs_oracle(gpu_lib, n, tile_bits, threads):
+    qc = W.random_layered(n, 10, seed=n + threads)
+    ops, ud = lower_circuit(qc)
+    st = W.normal_state(n, n)
+    out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert info["kernels"] > 0
+    err = np.abs(out - oracle(n, ops, ud, st)).max()
+    assert err <= TOL_C128, (n, tile_bits, threads, err)
+
+
+@pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
+def test_flip_heavy_circuits_vs_oracle(gpu_lib, n, tile_bits, threads):
+    ops, ud = flips_and_quarter_turns(n, 400, seed=7 * n + threads)
+    st = W.normal_state(n, 2 * n)
+    out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert info["kernels"] > 0
+    err = np.abs(out - oracle(n, ops, ud, st)).max()
+    assert err <= TOL_C128, (n, tile_bits, threads, err)
+
+
+@pytest.mark.parametrize("tile_bits,threads", [(13, 1024), (13, 512), (12, 256)])
+def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, tile_bits, threads):
+    n = 19
+    ops, ud = flips_and_quarter_turns(n, 300, seed=tile_bits + threads)
+    st = W.normal_state(n, 5)
+    out, _ = run(n, ops, ud, st, dtype=np.float32, tile_bits=tile_bits, threads=threads)
+    ref = oracle(n, ops, ud, st)
+    l2 = float(np.linalg.norm(out - ref))
+    fid = float(abs(np.vdot(ref, out)) ** 2 / (np.vdot(ref, ref).real * np.vdot(out, out).real))
+    assert l2 <= 2e-5 and fid >= 1 - 1e-6, (tile_bits, threads, l2, fid)
+
+
+@pytest.mark.parametrize("switch", ["BFLY_JIT_HOIST", "BFLY_JIT_PREFETCH"])
+def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switch):
+    n = 18
+    qc = W.random_layered(n, 12, seed=11)
+    ops, ud = lower_circuit(qc)
+    st = W.normal_state(n, 3)
+    ref = oracle(n, ops, ud, st)
+    base, _ = run(n, ops, ud, st)
+    old = os.environ.get(switch)
+    os.environ[switch] = "0"
+    try:
+        off, info = run(n, ops, ud, st)
+    finally:
+        if old is None:
+            del os.environ[switch]
+        else:
+            os.environ[switch] = old
+    assert info["compiled"] > 0   # the switched sources differ, so they compile anew
+    assert np.abs(off - ref).max() <= TOL_C128
+    assert np.abs(base - ref).max() <= TOL_C128
