@@ -50,7 +50,12 @@ def test_gate_cap_respected(max_gates):
     # phase), plus the pass's scale instructions
     for block in text.split("tile ")[1:]:
         assert block.count("instr ") <= 3 * max_gates + 2
-    assert plan.stats()["passes"] >= -(-len(ops) // max_gates)
+    # adjacent gates with the same target and controls merge into one (runs
+    # of at most 64), so the cap counts runs
+    runs = sum(1 for i, o in enumerate(ops)
+               if not (i and o["kind"] == 0 and ops[i - 1]["kind"] == 0 and o["target"] == ops[i - 1]["target"]
+                       and o["controls"] == ops[i - 1]["controls"]))
+    assert plan.stats()["passes"] >= -(-runs // max_gates)
     psi = W.normal_state(n, 3).to_complex()
     assert np.abs(run_plan(text, psi, ud) - oracle_run(n, ops, ud, psi)).max() < 1e-12
 
@@ -236,3 +241,34 @@ def test_tile_search_random_circuits_match_oracle(seed):
     for tb in (5, 6, 7):
         plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
         assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
+
+
+def test_runs_on_one_target_merge_into_one_gate():
+    """Adjacent gates with the same target and controls multiply into one 2x2
+    gate (runs of at most 64): amplitude estimation's 1569 controlled gates on
+    qubit 0 become a few dozen tile instructions, and long runs of every gate
+    kind (200 gates, controlled and not) replay to the oracle's amplitudes."""
+    from code_mojo_amd import serve
+    job = serve.ae_job(5, np.float64)
+    text = Plan(job.n, np.float64, job.ops, job.udata, _lib.opts()).dump()
+    assert text.count("instr ") < 120, text.count("instr ")
+    n = 8
+    rng = np.random.default_rng(21)
+    ops = OpList(n)
+    kinds = ["H", "P", "X", "Y", "Z", "RX", "RY", "RZ"]
+    for target, ctrls in ((0, []), (3, [5]), (7, [1, 2]), (2, [])):
+        for _ in range(200):
+            k = G.GateKind.names.index(kinds[int(rng.integers(len(kinds)))])
+            info = G.GateInfo.of(k, float(rng.uniform(-3, 3)) if k in (1, 5, 6, 7) else None)
+            ops.gate(target, ctrls, info.kind, info.gate, info.arg or 0.0)
+    ops, ud = ops.finish()
+    psi = W.normal_state(n, 4).to_complex()
+    ref = oracle_run(n, ops, ud, psi)
+    for tb in (0, 5):
+        plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
+        assert plan.dump().count("instr ") < 80
+        assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
+    # the reference's per-gate arithmetic (generic pairs) keeps every gate
+    plan = Plan(n, np.float64, ops, ud, _lib.opts(flags=_lib.FLAG_GENERIC_PAIRS))
+    assert plan.dump().count("instr ") >= 700   # (consecutive sign flips still XOR-merge)
+    assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
