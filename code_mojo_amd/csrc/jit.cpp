@@ -620,16 +620,24 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     const int NT = tile_threads_h(K, S);
     const unsigned NG = TS / G;
     const int WPE = tile_waves_per_eu_h(elem, K, S);
-    // The next tile's loads are issued before the sweeps (30-qubit layered
-    // circuit, 4-slot sweeps: complex128 335.6 vs 343.7 ms, complex64 175.3
-    // vs 183.5 ms); BFLY_JIT_PREFETCH=0 turns it off (measurement).
-    bool prefetch = true;
-    if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = pf[0] == '1';
+    // One block per tile: the hardware dispatcher starts a new block as soon
+    // as one finishes, so the stage-in loads of some blocks overlap the
+    // sweeps of others on every CU.  Measured on the 30-qubit layered circuit
+    // against persistent blocks that walk the tiles with the next tile's
+    // loads issued before their sweeps: complex128 297 vs 333 ms, complex64
+    // 157 vs 176 ms (first-fit schedule 389 vs 435 ms).  BFLY_JIT_GRID=
+    // persistent builds the persistent form (BFLY_JIT_PREFETCH=0: without
+    // the next tile's loads) for measurement.
+    bool persistent = false;
+    if (const char* g = std::getenv("BFLY_JIT_GRID")) persistent = g[0] == 'p';
+    bool prefetch = persistent;
+    if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = persistent && pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
 
     JitSource js;
     js.threads = NT;
     js.ntiles = ntiles;
+    js.persistent = persistent;
     Out o;
     auto cst = [&](double dv, float fv) {
         const size_t i = f32 ? js.cf.size() : js.cd.size();
@@ -688,7 +696,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     load("base");
     // the slot of tv + c NT VN + v (disjoint bits) is pad(tv) + pad(c NT VN + v)
     o("  const unsigned stv = padc(tv) * %du;\n", E);
-    o("  for (;;) {\n");
+    if (persistent) o("  for (;;) {\n");
     for (unsigned c = 0; c < PV; ++c) {
         if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
         o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
@@ -698,8 +706,10 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("  }\n");
     }
     o("  __syncthreads();\n");
-    o("  const unsigned long long next = tile + gridDim.x;\n");
-    if (prefetch) {
+    if (!persistent) {
+        // one tile per block: nothing to load ahead
+    } else if (prefetch) {
+        o("  const unsigned long long next = tile + gridDim.x;\n");
         // The next tile's loads are issued before the sweeps and stay in
         // flight across them.  They are unconditional (the last tile reloads
         // its own amplitudes, which are never used): a conditional load makes
@@ -709,6 +719,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("  const unsigned long long nbase = %s;\n", tile_base("nc").c_str());
         load("nbase");
     } else {
+        o("  const unsigned long long next = tile + gridDim.x;\n");
         o("  unsigned long long nbase = 0;\n");
         o("  if (next < %lluull) nbase = %s;\n", ntiles, tile_base("next").c_str());
     }
@@ -783,9 +794,12 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("    *reinterpret_cast<V*>(re + base + toff + %lluull) = vr; *reinterpret_cast<V*>(im + base + toff + %lluull) = vi; }\n",
           u, u);
     }
-    o("  if (next >= %lluull) break;\n  __syncthreads();\n  tile = next;\n  base = nbase;\n", ntiles);
-    if (!prefetch) load("base");
-    o("  }\n}\n");
+    if (persistent) {
+        o("  if (next >= %lluull) break;\n  __syncthreads();\n  tile = next;\n  base = nbase;\n", ntiles);
+        if (!prefetch) load("base");
+        o("  }\n");
+    }
+    o("}\n");
     js.text = std::move(o.s);
     return js;
 }
@@ -848,6 +862,7 @@ int jit_build(JitPlan& jp, const HostPlan& hp, std::string& err) {
         JitPass pass;
         pass.threads = srcs.back().threads;
         pass.ntiles = srcs.back().ntiles;
+        pass.persistent = srcs.back().persistent;
         // constants: 16-byte aligned offsets into one array per plan
         const size_t elem = hp.dtype == BFLY_C64 ? 4 : 8;
         while ((jp.consts.size() % 16) != 0) jp.consts.push_back(0);
@@ -949,7 +964,9 @@ int jit_load(JitPlan& jp, int device, std::string& err) {
             return BFLY_EHIP;
         }
         const unsigned long long cap = static_cast<unsigned long long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
-        ps.grid = static_cast<unsigned>(ps.ntiles < cap ? ps.ntiles : cap);
+        // one block per tile (at most 2^30 tiles: 40 qubits, 2^10 tiles), or
+        // a persistent grid of the blocks the device holds at once
+        ps.grid = static_cast<unsigned>(ps.persistent ? (ps.ntiles < cap ? ps.ntiles : cap) : ps.ntiles);
         ps.fn = fn;
     }
     jp.device = device;

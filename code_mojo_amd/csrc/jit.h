@@ -21,6 +21,7 @@ struct JitSource {
     std::vector<float> cf;
     int threads = 0;
     unsigned long long ntiles = 0;
+    bool persistent = false;       // blocks walk the tiles (else one block per tile)
 };
 
 struct JitPass {
@@ -28,7 +29,8 @@ struct JitPass {
     size_t const_off = 0;          // byte offset of the pass's parameters in JitPlan::consts
     int threads = 0;
     unsigned long long ntiles = 0;
-    unsigned grid = 0;             // persistent grid on the loaded device
+    bool persistent = false;
+    unsigned grid = 0;             // blocks per launch on the loaded device
     hipFunction_t fn = nullptr;
 };
 

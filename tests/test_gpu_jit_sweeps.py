@@ -10,8 +10,9 @@ complex64 against the complex128 oracle: L2 <= 2e-5 and fidelity >= 1 - 1e-6):
   X / Y (quarter-turn rotations, the cotangent form) and in CZ / controlled
   phases between random pairs exercise every folding case;
 - the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place)
-  and BFLY_JIT_PREFETCH=0 (no next-tile loads during the sweeps) give the
-  same amplitudes within the complex128 bar."""
+  and BFLY_JIT_GRID=persistent (blocks walk the tiles, with or without the
+  next tile's loads during the sweeps: BFLY_JIT_PREFETCH) give the same
+  amplitudes within the complex128 bar."""
 import math
 import os
 
@@ -121,23 +122,25 @@ def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, tile_bits, threads)
     assert l2 <= 2e-5 and fid >= 1 - 1e-6, (tile_bits, threads, l2, fid)
 
 
-@pytest.mark.parametrize("switch", ["BFLY_JIT_HOIST", "BFLY_JIT_PREFETCH"])
-def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switch):
+@pytest.mark.parametrize("switches", [{"BFLY_JIT_HOIST": "0"}, {"BFLY_JIT_GRID": "persistent"},
+                                      {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "0"}])
+def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switches):
     n = 18
     qc = W.random_layered(n, 12, seed=11)
     ops, ud = lower_circuit(qc)
     st = W.normal_state(n, 3)
     ref = oracle(n, ops, ud, st)
     base, _ = run(n, ops, ud, st)
-    old = os.environ.get(switch)
-    os.environ[switch] = "0"
+    old = {k: os.environ.get(k) for k in switches}
+    os.environ.update(switches)
     try:
         off, info = run(n, ops, ud, st)
     finally:
-        if old is None:
-            del os.environ[switch]
-        else:
-            os.environ[switch] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     assert info["compiled"] > 0   # the switched sources differ, so they compile anew
     assert np.abs(off - ref).max() <= TOL_C128
     assert np.abs(base - ref).max() <= TOL_C128
