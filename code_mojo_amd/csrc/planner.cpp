@@ -14,6 +14,7 @@
 // Gates are taken out of program order only when they commute with every
 // gate left behind, so each pass is exact, not approximate.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -1644,25 +1645,24 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
             op_nd[i] = cls[i].diag ? 0 : (1ull << op.target);
         }
     }
-    // One scan of the remaining ops from `head` in program order: an op joins
-    // the pass when it commutes with every op deferred before it and its
-    // non-diagonal target is a tile qubit.  With `grow` a target outside the
-    // tile joins it while the tile has fewer than k qubits (first fit);
-    // otherwise the tile is fixed.  Returns the number of ops placed and the
-    // final tile; with `record` fills `placed`, `rest` and the scan end.
-    size_t scan_end = head;
-    auto scan = [&](uint64_t tile, bool grow, bool record, uint64_t* tile_out) -> size_t {
+    // One scan of the ops rem[from..] in program order: an op joins the pass
+    // when it commutes with every op deferred before it and its non-diagonal
+    // target is a tile qubit.  With `grow` a target outside the tile joins
+    // it while the tile has fewer than k qubits (first fit); otherwise the
+    // tile is fixed.  Returns the number of ops placed and the final tile;
+    // fills the placed ops, the deferred ops and the scan end when asked.
+    auto scan_in = [&](const std::vector<uint32_t>& rem, size_t from, uint64_t tile, bool grow,
+                       std::vector<PendingGate>* placed_out, std::vector<uint32_t>* rest_out, size_t* end_out,
+                       uint64_t* tile_out) -> size_t {
         uint64_t d_nd = 0, d_all = 0;  // qubits of deferred gates: non-diagonal targets / all qubits
         size_t nplaced = 0;
-        if (record) {
-            placed.clear();
-            rest.clear();
-        }
+        if (placed_out) placed_out->clear();
+        if (rest_out) rest_out->clear();
         size_t scanned_after_defer = 0;
         bool deferred_any = false;
-        size_t idx = head;
-        for (; idx < remaining.size(); ++idx) {
-            const uint32_t oi = remaining[idx];
+        size_t idx = from;
+        for (; idx < rem.size(); ++idx) {
+            const uint32_t oi = rem[idx];
             if (deferred_any && ++scanned_after_defer > kScanWindow) break;
             if ((d_nd & all) == all) break;  // every qubit blocked: nothing more can move
             const uint64_t qnd = op_nd[oi], qall = op_all[oi];
@@ -1673,16 +1673,21 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
             }
             if (!ok) {
                 d_nd |= qnd; d_all |= qall;
-                if (record) rest.push_back(oi);
+                if (rest_out) rest_out->push_back(oi);
                 deferred_any = true;
                 continue;
             }
             ++nplaced;
-            if (record) placed.push_back(PendingGate{&ops[oi], cls[oi]});
+            if (placed_out) placed_out->push_back(PendingGate{&ops[oi], cls[oi]});
         }
-        if (record) scan_end = idx;
+        if (end_out) *end_out = idx;
         if (tile_out) *tile_out = tile;
         return nplaced;
+    };
+    size_t scan_end = head;
+    auto scan = [&](uint64_t tile, bool grow, bool record, uint64_t* tile_out) -> size_t {
+        return scan_in(remaining, head, tile, grow, record ? &placed : nullptr, record ? &rest : nullptr,
+                       record ? &scan_end : nullptr, tile_out);
     };
     // Tile qubits of a pass.  First fit takes the targets in program order.
     // The search (default) also builds the set one qubit at a time, each time
@@ -1695,26 +1700,7 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
     // swaps).
     const bool search = !(opts.flags & BFLY_FLAG_FIRST_FIT) && k < n;
     constexpr int kMaxSwaps = 64;   // bounds the planner's time on long circuits
-    auto choose_tile = [&]() -> uint64_t {
-        uint64_t t_ff = 0;
-        scan(low_mask, true, false, &t_ff);
-        for (int b = 0; popc(t_ff) < k && b < n; ++b) t_ff |= 1ull << b;
-        const size_t c_ff = scan(t_ff, false, false, nullptr);
-        uint64_t t_gr = low_mask;
-        size_t c_gr = 0;
-        while (popc(t_gr) < k) {
-            int bq = -1;
-            size_t bc = 0;
-            for (int q = 0; q < n; ++q) {
-                if ((t_gr >> q) & 1) continue;
-                const size_t c = scan(t_gr | (1ull << q), false, false, nullptr);
-                if (bq < 0 || c > bc) { bq = q; bc = c; }
-            }
-            t_gr |= 1ull << bq;
-            c_gr = bc;
-        }
-        uint64_t t = c_gr > c_ff ? t_gr : t_ff;
-        size_t cur = std::max(c_gr, c_ff);
+    auto refine = [&](const std::vector<uint32_t>& rem, size_t from, uint64_t t, size_t cur) {
         for (int swaps = 0; swaps < kMaxSwaps; ++swaps) {
             bool improved = false;
             for (int a = 0; a < n && !improved; ++a) {
@@ -1722,7 +1708,7 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
                 for (int b = 0; b < n && !improved; ++b) {
                     if ((t >> b) & 1) continue;
                     const uint64_t t2 = (t & ~(1ull << a)) | (1ull << b);
-                    const size_t c = scan(t2, false, false, nullptr);
+                    const size_t c = scan_in(rem, from, t2, false, nullptr, nullptr, nullptr, nullptr);
                     if (c > cur) { cur = c; t = t2; improved = true; }
                 }
             }
@@ -1730,6 +1716,123 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
         }
         return t;
     };
+    // The search's choice; with `alts`, also the first-fit and greedy sets,
+    // each improved by swaps (the candidates of the beam below).
+    auto choose_tile_in = [&](const std::vector<uint32_t>& rem, size_t from, std::vector<uint64_t>* alts) {
+        uint64_t t_ff = 0;
+        scan_in(rem, from, low_mask, true, nullptr, nullptr, nullptr, &t_ff);
+        for (int b = 0; popc(t_ff) < k && b < n; ++b) t_ff |= 1ull << b;
+        const size_t c_ff = scan_in(rem, from, t_ff, false, nullptr, nullptr, nullptr, nullptr);
+        uint64_t t_gr = low_mask;
+        size_t c_gr = 0;
+        while (popc(t_gr) < k) {
+            int bq = -1;
+            size_t bc = 0;
+            for (int q = 0; q < n; ++q) {
+                if ((t_gr >> q) & 1) continue;
+                const size_t c = scan_in(rem, from, t_gr | (1ull << q), false, nullptr, nullptr, nullptr, nullptr);
+                if (bq < 0 || c > bc) { bq = q; bc = c; }
+            }
+            t_gr |= 1ull << bq;
+            c_gr = bc;
+        }
+        const uint64_t best = refine(rem, from, c_gr > c_ff ? t_gr : t_ff, std::max(c_gr, c_ff));
+        if (alts) {
+            alts->clear();
+            auto add = [&](uint64_t t) {
+                if (std::find(alts->begin(), alts->end(), t) == alts->end()) alts->push_back(t);
+            };
+            add(best);
+            add(t_ff);
+            add(t_gr);
+            add(refine(rem, from, t_ff, c_ff));
+            add(refine(rem, from, t_gr, c_gr));
+        }
+        return best;
+    };
+    auto choose_tile = [&]() -> uint64_t { return choose_tile_in(remaining, head, nullptr); };
+    // Beam over whole schedules (BFLY_PLAN_BEAM=1): the kBeam partial
+    // schedules with the fewest ops left are extended by each candidate set
+    // of their next pass, and the first schedule to place every op gives the
+    // tile sets of all passes (kept only when it needs fewer passes than the
+    // greedy schedule).  30-qubit layered circuit, 2^12 tiles: 28 passes
+    // against 30 in 0.06 s of planning, but the same 306 ms per circuit on
+    // the generated kernels (the searched passes are bound by their
+    // arithmetic, which fewer passes only regroup), so it is off by default.
+    std::vector<uint64_t> beam_tiles;
+    size_t beam_pos = 0;
+    constexpr int kBeam = 3;
+    constexpr uint64_t kBeamMaxOps = 20000;
+    bool beam_on = false;
+    if (const char* b = std::getenv("BFLY_PLAN_BEAM")) beam_on = b[0] == '1';
+    if (search && beam_on && nops <= kBeamMaxOps) {
+        struct Partial {
+            std::vector<uint32_t> rem;
+            std::vector<uint64_t> tiles;
+        };
+        auto first_placeable = [&](const std::vector<uint32_t>& rem) {
+            size_t f = 0;
+            while (f < rem.size() && !op_place[rem[f]]) ++f;
+            return f;
+        };
+        std::vector<uint64_t> alts;
+        std::vector<uint32_t> rest_ids;
+        // the greedy schedule in the same model: the beam keeps it only when
+        // it needs no more passes
+        std::vector<uint64_t> greedy_tiles;
+        {
+            std::vector<uint32_t> rem = remaining;
+            for (;;) {
+                const size_t from = first_placeable(rem);
+                if (from == rem.size()) break;
+                const uint64_t t = choose_tile_in(rem, from, nullptr);
+                size_t end = from;
+                if (scan_in(rem, from, t, false, nullptr, &rest_ids, &end, nullptr) == 0) break;
+                greedy_tiles.push_back(t);
+                std::vector<uint32_t> nx = rest_ids;
+                nx.insert(nx.end(), rem.begin() + static_cast<long>(end), rem.end());
+                rem.swap(nx);
+            }
+        }
+        std::vector<Partial> beam(1);
+        beam[0].rem = remaining;
+        for (;;) {
+            const Partial* done = nullptr;
+            for (const Partial& ps : beam)
+                if (first_placeable(ps.rem) == ps.rem.size()) { done = &ps; break; }
+            if (done) {
+                beam_tiles = done->tiles;
+                break;
+            }
+            if (beam[0].tiles.size() >= greedy_tiles.size()) break;   // cannot beat the greedy schedule
+            std::vector<Partial> next;
+            for (const Partial& ps : beam) {
+                const size_t from = first_placeable(ps.rem);
+                choose_tile_in(ps.rem, from, &alts);
+                for (const uint64_t t : alts) {
+                    size_t end = from;
+                    if (scan_in(ps.rem, from, t, false, nullptr, &rest_ids, &end, nullptr) == 0) continue;
+                    Partial nx;
+                    nx.rem = rest_ids;
+                    nx.rem.insert(nx.rem.end(), ps.rem.begin() + static_cast<long>(end), ps.rem.end());
+                    nx.tiles = ps.tiles;
+                    nx.tiles.push_back(t);
+                    next.push_back(std::move(nx));
+                }
+            }
+            if (next.empty()) break;   // cannot happen: the search's own choice always places an op
+            std::stable_sort(next.begin(), next.end(),
+                             [](const Partial& a, const Partial& b) { return a.rem.size() < b.rem.size(); });
+            beam.clear();
+            for (Partial& ps : next) {
+                bool dup = false;
+                for (const Partial& q : beam) dup = dup || q.rem == ps.rem;
+                if (!dup) beam.push_back(std::move(ps));
+                if (static_cast<int>(beam.size()) == kBeam) break;
+            }
+        }
+        if (beam_tiles.empty() || beam_tiles.size() > greedy_tiles.size()) beam_tiles = greedy_tiles;
+    }
     while (head < remaining.size()) {
         const bfly_op& front = ops[remaining[head]];
         if (!placeable(front)) {
@@ -1739,8 +1842,14 @@ int build_plan_steps(HostPlan& plan, int32_t n, int32_t dtype, const bfly_op* op
         }
         uint64_t tile = low_mask;
         if (search) {
-            tile = choose_tile();
+            // the beam's set for this pass (the search's own choice when the
+            // beam has none left or its set places nothing here)
+            tile = beam_pos < beam_tiles.size() ? beam_tiles[beam_pos++] : choose_tile();
             scan(tile, false, true, &tile);
+            if (placed.empty()) {
+                tile = choose_tile();
+                scan(tile, false, true, &tile);
+            }
         } else {
             scan(low_mask, true, true, &tile);
         }

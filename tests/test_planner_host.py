@@ -272,3 +272,29 @@ def test_runs_on_one_target_merge_into_one_gate():
     plan = Plan(n, np.float64, ops, ud, _lib.opts(flags=_lib.FLAG_GENERIC_PAIRS))
     assert plan.dump().count("instr ") >= 700   # (consecutive sign flips still XOR-merge)
     assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
+
+
+def test_beam_schedules_match_oracle_and_need_no_more_passes(monkeypatch):
+    """The beam over whole schedules (BFLY_PLAN_BEAM=1) replays to the oracle's
+    amplitudes and never needs more passes than the greedy search (the
+    30-qubit headline circuit: 28 against 30)."""
+    ops, ud = lower_circuit(W.random_layered(30, 40, 2026))
+    greedy = Plan(30, np.float64, ops, ud, _lib.opts()).stats()["passes"]
+    monkeypatch.setenv("BFLY_PLAN_BEAM", "1")
+    beam = Plan(30, np.float64, ops, ud, _lib.opts()).stats()["passes"]
+    assert beam <= 28 and beam <= greedy, (beam, greedy)
+    n = 14
+    rng = np.random.default_rng(8)
+    for qc in (W.random_layered(n, 16, seed=6), None):
+        if qc is None:
+            ops, ud = random_ops(rng, n, 200)
+        else:
+            ops, ud = lower_circuit(qc)
+        psi = W.normal_state(n, 7).to_complex()
+        ref = oracle_run(n, ops, ud, psi)
+        for tb in (6, 8, 10):
+            monkeypatch.setenv("BFLY_PLAN_BEAM", "1")
+            plan = Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb))
+            assert np.abs(run_plan(plan.dump(), psi, ud) - ref).max() < 1e-12
+            monkeypatch.setenv("BFLY_PLAN_BEAM", "0")
+            assert plan.stats()["passes"] <= Plan(n, np.float64, ops, ud, _lib.opts(tile_bits=tb)).stats()["passes"]
