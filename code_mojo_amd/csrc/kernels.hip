@@ -2811,7 +2811,13 @@ hipError_t launch_unitary_mfma_t(T* re, T* im, const UnitaryArgs& A, const doubl
     const size_t lds = 2ull * D * D * sizeof(T);
     const uint64_t waves = (A.ngroups + 15) / 16;
     const uint64_t blocks = (waves + 3) / 4;
-    const unsigned grid = static_cast<unsigned>(blocks < 4096 ? blocks : 4096);
+    // Every block stages the matrix in LDS: 4-qubit blocks (4 KiB in
+    // complex128) take one block per 4 waves of groups; 5 and 6 qubits keep
+    // a persistent grid that loads it once per block.  Measured at 28
+    // qubits (complex128 / complex64): m = 4 1.49 / 0.95 ms full against
+    // 1.79 / 1.23 ms persistent; m = 5 1.90 / 1.37 against 1.75 / 1.26;
+    // m = 6 3.45 / 2.40 against 2.40 / 1.45.
+    const unsigned grid = M <= 4 ? full_grid(blocks, 4096) : static_cast<unsigned>(blocks < 4096 ? blocks : 4096);
     hipLaunchKernelGGL((unitary_mfma_kernel<T, M>), dim3(grid), dim3(kUMfmaThreads), lds, stream, re, im, A, d_u);
     return hipGetLastError();
 }
