@@ -630,6 +630,11 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // the next tile's loads) for measurement.
     bool persistent = false;
     if (const char* g = std::getenv("BFLY_JIT_GRID")) persistent = g[0] == 'p';
+    // a dispatch covers at most 2^32 - 1 work-items per dimension: beyond
+    // that (2^33 amplitudes in 2^9-amplitude tiles, 2^36 in 2^12) the
+    // blocks walk the tiles
+    if ((1ull << (P.n - P.k)) * static_cast<unsigned long long>(tile_threads_h(P.k, P.slots)) > 0xffffffffull)
+        persistent = true;
     bool prefetch = persistent;
     if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = persistent && pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
@@ -964,8 +969,9 @@ int jit_load(JitPlan& jp, int device, std::string& err) {
             return BFLY_EHIP;
         }
         const unsigned long long cap = static_cast<unsigned long long>(per_cu > 0 ? per_cu : 1) * (cus > 0 ? cus : 1);
-        // one block per tile (at most 2^30 tiles: 40 qubits, 2^10 tiles), or
-        // a persistent grid of the blocks the device holds at once
+        // one block per tile (fewer than 2^32 work-items: the generator
+        // builds the persistent form beyond that), or a persistent grid of
+        // the blocks the device holds at once
         ps.grid = static_cast<unsigned>(ps.persistent ? (ps.ntiles < cap ? ps.ntiles : cap) : ps.ntiles);
         ps.fn = fn;
     }

@@ -545,9 +545,13 @@ bool persistent_grids() {
     }();
     return p;
 }
-inline unsigned full_grid(uint64_t units, int cap) {
+// `threads` per block: a dispatch covers at most 2^32 - 1 work-items per
+// dimension, so larger grids (from 2^33 amplitudes) are capped and the
+// kernels' grid-stride loops cover the rest.
+inline unsigned full_grid(uint64_t units, int cap, unsigned threads) {
     if (persistent_grids()) return static_cast<unsigned>(units < static_cast<uint64_t>(cap) ? units : cap);
-    return static_cast<unsigned>(units < 0x7fffffffull ? units : 0x7fffffffull);
+    const uint64_t max_blocks = 0xffffffffull / threads;
+    return static_cast<unsigned>(units < max_blocks ? units : max_blocks);
 }
 
 template <typename T, int K, int S>
@@ -560,7 +564,7 @@ hipError_t launch_tile_k(T* re, T* im, const TilePass& P, const TileSweep* sw, c
     if (e != hipSuccess) return e;
     if (!re) return hipSuccess;  // preparation only
     const uint64_t ntiles = 1ull << (P.n - P.k);
-    const unsigned grid = full_grid(ntiles, grid_cap);
+    const unsigned grid = full_grid(ntiles, grid_cap, NT);
     hipLaunchKernelGGL((tile_pass_kernel<T, K, S>), dim3(grid), dim3(NT), lds, st, re, im, P, sw, in);
     return hipGetLastError();
 }
@@ -574,7 +578,7 @@ hipError_t launch_batch_k(const BatchItem* items, int nitems, uint64_t total_til
     hipError_t e = persistent_grid(tile_batch_kernel<T, K, S>, NT, lds, grid_cache, &grid_cap);
     if (e != hipSuccess) return e;
     if (!items) return hipSuccess;  // preparation only
-    const unsigned grid = full_grid(total_tiles, grid_cap);
+    const unsigned grid = full_grid(total_tiles, grid_cap, NT);
     hipLaunchKernelGGL((tile_batch_kernel<T, K, S>), dim3(grid), dim3(NT), lds, st, items, nitems, total_tiles);
     return hipGetLastError();
 }
@@ -2512,16 +2516,19 @@ __global__ __launch_bounds__(kMargThreads) void marginal_kernel(const T* __restr
     }
 }
 
-inline unsigned grid_for(uint64_t work, unsigned per_block = 256, unsigned cap = 16384) {
+// Blocks of 256 threads for a grid-stride kernel: one per `per_block` units
+// of work (the full grid of the streaming launches), or at most `cap` when
+// one is given.
+inline unsigned grid_for(uint64_t work, unsigned per_block = 256, unsigned cap = 0) {
     uint64_t g = (work + per_block - 1) / per_block;
     if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return static_cast<unsigned>(g);
+    if (cap) return static_cast<unsigned>(g > cap ? cap : g);
+    return full_grid(g, 16384, 256);
 }
 
 template <typename T>
 hipError_t launch_gate_t(T* re, T* im, const GateLaunch& G, hipStream_t st) {
-    hipLaunchKernelGGL(gate_kernel<T>, dim3(full_grid((G.count + 255) / 256, 16384)), dim3(256), 0, st, re, im, G);
+    hipLaunchKernelGGL(gate_kernel<T>, dim3(full_grid((G.count + 255) / 256, 16384, 256)), dim3(256), 0, st, re, im, G);
     return hipGetLastError();
 }
 
@@ -2536,7 +2543,7 @@ hipError_t launch_bitrev_t(T* re, T* im, int n, uint64_t mask, hipStream_t st) {
     const uint64_t size = 1ull << n;
     if (mask == 0 && n >= 10) {
         const uint64_t tiles = 1ull << (n - 10);
-        const unsigned blocks = full_grid(tiles, kBitrevBlocks);
+        const unsigned blocks = full_grid(tiles, kBitrevBlocks, 256);
         hipLaunchKernelGGL((bitrev_joint_kernel<T, 5>), dim3(blocks), dim3(256), 0, st, re, im, n);
     } else {
         if (mask == 0) mask = size - 1;
@@ -2817,7 +2824,8 @@ hipError_t launch_unitary_mfma_t(T* re, T* im, const UnitaryArgs& A, const doubl
     // qubits (complex128 / complex64): m = 4 1.49 / 0.95 ms full against
     // 1.79 / 1.23 ms persistent; m = 5 1.90 / 1.37 against 1.75 / 1.26;
     // m = 6 3.45 / 2.40 against 2.40 / 1.45.
-    const unsigned grid = M <= 4 ? full_grid(blocks, 4096) : static_cast<unsigned>(blocks < 4096 ? blocks : 4096);
+    const unsigned grid =
+        M <= 4 ? full_grid(blocks, 4096, kUMfmaThreads) : static_cast<unsigned>(blocks < 4096 ? blocks : 4096);
     hipLaunchKernelGGL((unitary_mfma_kernel<T, M>), dim3(grid), dim3(kUMfmaThreads), lds, stream, re, im, A, d_u);
     return hipGetLastError();
 }
@@ -2884,7 +2892,7 @@ hipError_t launch_random_fill(int dtype, void* re, void* im, uint64_t size, uint
 unsigned stream_grid(uint64_t nvec) {
     const uint64_t per_block = 256ull * kStreamUnroll;
     const uint64_t b = (nvec + per_block - 1) / per_block;
-    return full_grid(b < 1 ? 1 : b, 32768);
+    return full_grid(b < 1 ? 1 : b, 32768, 256);
 }
 
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream) {

@@ -88,6 +88,8 @@ def test_33_qubit_complex128_qft_through_the_engine(gpu_lib):
         try:
             rid = eng.submit(job)
         except _lib.BflyError as e:
+            if e.code != _lib.BFLY_ENOMEM:
+                raise
             pytest.skip(f"a 33-qubit complex128 state does not fit: {e}")
         eng.wait(rid, 300)
         st = eng.status(rid)[0]
