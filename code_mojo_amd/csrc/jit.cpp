@@ -666,6 +666,92 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // VGPRs for 13 sweeps of 16 members: spills).
     auto pad = [&](unsigned j) { return j + (j >> B) + (j >> (2 * B)); };
     const unsigned TSP = pad(TS - 1) + 1;   // slots of the padded tile
+    const int VN = f32 ? 4 : 2;   // amplitudes per 16-byte stage-in access
+
+    // thread -> group of a sweep: the first B thread bits go to non-slot local
+    // bits of distinct residues mod B (distinct LDS bank groups under pad), the
+    // rest to the remaining non-slot bits in ascending order
+    auto thread_pos = [&](const TileSweep& sw, int32_t* tpos) {
+        bool is_slot[kMaxTileBits] = {};
+        for (int a = 0; a < S; ++a) is_slot[sw.s[a]] = true;
+        bool used[kMaxTileBits] = {};
+        int nt = 0;
+        // thread bit i < B goes to the lowest free non-slot bit congruent to
+        // i mod B: its slot bit is then i, and the lanes of an LDS pass
+        // cover one block of 2^B consecutive slots in order
+        for (int i = 0; i < B && i < K - S; ++i) {
+            int pick = -1;
+            for (int pbit = i; pbit < K && pick < 0; pbit += B)
+                if (!is_slot[pbit] && !used[pbit]) pick = pbit;
+            if (pick < 0)
+                for (int pbit = 0; pbit < K && pick < 0; ++pbit)
+                    if (!is_slot[pbit] && !used[pbit]) pick = pbit;
+            used[pick] = true;
+            tpos[nt++] = pick;
+        }
+        for (int pbit = 0; pbit < K; ++pbit)
+            if (!is_slot[pbit] && !used[pbit]) tpos[nt++] = pbit;
+    };
+    // Contiguous amplitudes (log2) that one wave-wide access covers: the local
+    // bits its 64 lanes and its vector elements span, counted up from local
+    // bit 0 while local bit b is global bit b.
+    auto contiguous_bits = [&](const bool* covered) {
+        int r = 0;
+        while (r < K && covered[r] && P.qpos[r] == r) ++r;
+        return r;
+    };
+    auto staged_bits = [&]() {
+        bool cov[kMaxTileBits] = {};
+        for (int b = 0; b < K && b < 6 + (f32 ? 2 : 1); ++b) cov[b] = true;
+        return contiguous_bits(cov);
+    };
+    auto sweep_bits = [&](const TileSweep& sw) {
+        int32_t tpos[kMaxTileBits];
+        thread_pos(sw, tpos);
+        bool cov[kMaxTileBits] = {};
+        for (int i = 0; i < 6 && i < K - S; ++i) cov[tpos[i]] = true;
+        bool is_slot[kMaxTileBits] = {};
+        for (int a = 0; a < S; ++a) is_slot[sw.s[a]] = true;
+        if (is_slot[0]) {
+            cov[0] = true;
+            if (f32 && is_slot[1] && P.qpos[0] == 0) cov[1] = true;
+        }
+        return contiguous_bits(cov);
+    };
+    // Direct stage-in / stage-out: the first sweep loads its members from
+    // HBM in its own register layout and the last sweep stores them back, so
+    // a pass of s sweeps makes s - 1 LDS round trips instead of s + 1 (a
+    // staged pass writes the loaded tile to LDS and reads it back for its
+    // first sweep, and the same again before its stores).  A sweep's layout
+    // is taken when its wave-wide accesses cover runs of HBM at least as long
+    // as the staged layout's (1 KiB when local bits 0-6 are global bits 0-6),
+    // and only in passes of 6 or more sweeps: the saved round trips pay in
+    // passes bound by their sweeps, while in light, HBM-bound passes the
+    // staged layout's access order streams slightly better.  Measured on the
+    // 30-qubit layered circuit (ms per circuit, searched / first-fit tile
+    // qubits): every pass staged 302 / 416, direct where the runs allow 295 /
+    // 422, and in passes of >= 6 sweeps 294.5 / 414.5; every first and last
+    // sweep direct whatever its runs 309 / 456.  Needs one group of members
+    // per thread.  Measurement switches: BFLY_JIT_DIRECT=0 stages every pass,
+    // =1 takes the first and last sweep's layouts whatever their runs, =in /
+    // =out only one side; BFLY_JIT_DIRECT_MIN_SWEEPS sets the pass size (1:
+    // every pass).
+    bool direct_in = false, direct_out = false;
+    int min_sweeps = 6;
+    if (const char* ms = std::getenv("BFLY_JIT_DIRECT_MIN_SWEEPS")) min_sweeps = std::atoi(ms);
+    if (!persistent && NT == static_cast<int>(NG) && P.nsweeps > 0 && P.nsweeps >= min_sweeps) {
+        const TileSweep& s0 = hp.sweeps[static_cast<size_t>(P.first_sweep)];
+        const TileSweep& s1 = hp.sweeps[static_cast<size_t>(P.first_sweep + P.nsweeps - 1)];
+        const int need = staged_bits();
+        direct_in = sweep_bits(s0) >= need;
+        direct_out = sweep_bits(s1) >= need;
+        if (const char* d = std::getenv("BFLY_JIT_DIRECT")) {
+            if (d[0] == '0') direct_in = direct_out = false;
+            else if (d[0] == '1') direct_in = direct_out = true;
+            else if (d[0] == 'i') direct_out = false;
+            else if (d[0] == 'o') direct_in = false;
+        }
+    }
     o("typedef %s T;\ntypedef %s C2;\n", T, f32 ? "float2" : "double2");
     o("__device__ __forceinline__ unsigned padc(unsigned j) { return j + (j >> %d) + (j >> %d); }\n", B, 2 * B);
     o("extern \"C\" __global__ __launch_bounds__(%d, %d) void bfly_tile_pass(T* __restrict__ re, T* __restrict__ im, "
@@ -679,7 +765,6 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // 16-byte HBM access (re and im); thread tid holds local indices
     // (tid + c NT) VN + v, whose global offsets split into deposit(tid VN) and
     // the constant deposit(c NT VN + v) (disjoint bits).
-    const int VN = f32 ? 4 : 2;
     const unsigned NV = TS / static_cast<unsigned>(VN);
     const unsigned PV = (NV + static_cast<unsigned>(NT) - 1) / static_cast<unsigned>(NT);
     o("typedef %s V;\n", f32 ? "float4" : "double2");
@@ -698,19 +783,21 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         }
     };
     o("  unsigned long long base = %s;\n", tile_base("tile").c_str());
-    load("base");
     // the slot of tv + c NT VN + v (disjoint bits) is pad(tv) + pad(c NT VN + v)
     o("  const unsigned stv = padc(tv) * %du;\n", E);
-    if (persistent) o("  for (;;) {\n");
-    for (unsigned c = 0; c < PV; ++c) {
-        if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
-        o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
-        for (int v = 0; v < VN; ++v)
-            o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = e; }\n",
-              v, v, pad(c * NT * VN + static_cast<unsigned>(v)) * E);
-        o("  }\n");
+    if (!direct_in) {
+        load("base");
+        if (persistent) o("  for (;;) {\n");
+        for (unsigned c = 0; c < PV; ++c) {
+            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+            o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
+            for (int v = 0; v < VN; ++v)
+                o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = e; }\n",
+                  v, v, pad(c * NT * VN + static_cast<unsigned>(v)) * E);
+            o("  }\n");
+        }
+        o("  __syncthreads();\n");
     }
-    o("  __syncthreads();\n");
     if (!persistent) {
         // one tile per block: nothing to load ahead
     } else if (prefetch) {
@@ -732,72 +819,111 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // sweeps
     for (int s = 0; s < P.nsweeps; ++s) {
         const TileSweep& sw = hp.sweeps[static_cast<size_t>(P.first_sweep + s)];
-        int srt[kSlots];
-        for (int a = 0; a < S; ++a) srt[a] = sw.s[a];
-        std::sort(srt, srt + S);
         o("  // sweep %d: slots %d %d %d %d\n", s, sw.s[0], sw.s[1], S > 2 ? sw.s[2] : -1, S > 3 ? sw.s[3] : -1);
         if (NT > static_cast<int>(NG)) o("  if (tid < %uu)\n", NG);
+        int32_t tpos[kMaxTileBits];
+        thread_pos(sw, tpos);
         // the sweep's parameters load inside the sweep: an opaque pointer per
         // sweep keeps the compiler from hoisting every constant of the pass out
         // of the tile loop (which spills scalar registers into vector lanes)
-        // thread -> group: the first B thread bits go to non-slot local bits
-        // of distinct residues mod B (distinct LDS bank groups under pad), the
-        // rest to the remaining non-slot bits in ascending order
-        int32_t tpos[kMaxTileBits];
-        {
-            bool is_slot[kMaxTileBits] = {};
-            for (int a = 0; a < S; ++a) is_slot[srt[a]] = true;
-            bool used[kMaxTileBits] = {};
-            int nt = 0;
-            // thread bit i < B goes to the lowest free non-slot bit congruent to
-            // i mod B: its slot bit is then i, and the lanes of an LDS pass
-            // cover one block of 2^B consecutive slots in order
-            for (int i = 0; i < B && i < K - S; ++i) {
-                int pick = -1;
-                for (int pbit = i; pbit < K && pick < 0; pbit += B)
-                    if (!is_slot[pbit] && !used[pbit]) pick = pbit;
-                if (pick < 0)
-                    for (int pbit = 0; pbit < K && pick < 0; ++pbit)
-                        if (!is_slot[pbit] && !used[pbit]) pick = pbit;
-                used[pick] = true;
-                tpos[nt++] = pick;
-            }
-            for (int pbit = 0; pbit < K; ++pbit)
-                if (!is_slot[pbit] && !used[pbit]) tpos[nt++] = pbit;
-        }
         o("  {\n    unsigned zs = 0u;\n    asm volatile(\"\" : \"+s\"(zs));\n    const T* __restrict__ C = Cp + zs;\n");
         o("    const unsigned jb = static_cast<unsigned>(%s);\n", deposit_expr("tid", tpos, 0, K - S).c_str());
         o("    const unsigned jbs = padc(jb) * %du;\n", E);
         o("    T ar[%d], ai[%d];\n", G, G);
         unsigned moff[1 << kSlots];
+        unsigned mloc[1 << kSlots];
         for (int m = 0; m < G; ++m) {
             unsigned d = 0;
             for (int a = 0; a < S; ++a) d |= ((static_cast<unsigned>(m) >> a) & 1u) << sw.s[a];
+            mloc[m] = d;
             moff[m] = pad(d) * E;
-            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
-              moff[m], m, m);
+        }
+        const bool from_hbm = direct_in && s == 0, to_hbm = direct_out && s == P.nsweeps - 1;
+        // Direct member accesses: member m of the thread's group is local
+        // index jb | d(m) (disjoint bits), at global offset base + deposit(jb)
+        // + deposit(d(m)).  Members differing only in the slots on local bits
+        // 0 (and 1) sit at consecutive global addresses when those local bits
+        // are global bits 0 (and 1): one vector access carries them.
+        int vw = 1, vslot[2] = {-1, -1};
+        if (from_hbm || to_hbm) {
+            o("    const unsigned long long gj = %s;\n", deposit_expr("jb", P.qpos, 0, K).c_str());
+            for (int a = 0; a < S; ++a) {
+                if (sw.s[a] == 0 && P.qpos[0] == 0) vslot[0] = a;
+                if (sw.s[a] == 1 && P.qpos[1] == 1) vslot[1] = a;
+            }
+            if (vslot[0] >= 0) vw = (f32 && vslot[1] >= 0) ? 4 : 2;
+        }
+        const char* VT = vw == 1 ? T : (f32 ? (vw == 4 ? "float4" : "float2") : "double2");
+        static const char* const kComp[4] = {"x", "y", "z", "w"};
+        auto vmember = [&](int m, int e) {   // member of element e of the vector at member m
+            int r = m;
+            if (e & 1) r |= 1 << vslot[0];
+            if (e & 2) r |= 1 << vslot[1];
+            return r;
+        };
+        auto vhead = [&](int m) {
+            return !(vw >= 2 && ((m >> vslot[0]) & 1)) && !(vw == 4 && ((m >> vslot[1]) & 1));
+        };
+        if (from_hbm) {
+            for (int m = 0; m < G; ++m) {
+                if (!vhead(m)) continue;
+                const unsigned long long u = deposit_const(mloc[m], P.qpos, K);
+                if (vw == 1) {
+                    o("    ar[%d] = re[base + gj + %lluull]; ai[%d] = im[base + gj + %lluull];\n", m, u, m, u);
+                    continue;
+                }
+                o("    { const %s vr = *reinterpret_cast<const %s*>(re + base + gj + %lluull);"
+                  " const %s vi = *reinterpret_cast<const %s*>(im + base + gj + %lluull);\n",
+                  VT, VT, u, VT, VT, u);
+                for (int e = 0; e < vw; ++e)
+                    o("      ar[%d] = vr.%s; ai[%d] = vi.%s;\n", vmember(m, e), kComp[e], vmember(m, e), kComp[e]);
+                o("    }\n");
+            }
+        } else {
+            for (int m = 0; m < G; ++m)
+                o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
+                  moff[m], m, m);
         }
         std::vector<IrOp> ir;
         for (int q = 0; q < sw.ninstr; ++q)
             lower_instr(ir, hp.instrs[static_cast<size_t>(sw.first + q)], S, f32, cst);
         fold_signs(ir, S);
         emit_ops(o, ir, S, cst);
-        for (int m = 0; m < G; ++m)
-            o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs + %uu)) = e; }\n",
-              m, m, moff[m]);
-        o("  }\n  __syncthreads();\n");
+        if (to_hbm) {
+            for (int m = 0; m < G; ++m) {
+                if (!vhead(m)) continue;
+                const unsigned long long u = deposit_const(mloc[m], P.qpos, K);
+                if (vw == 1) {
+                    o("    re[base + gj + %lluull] = ar[%d]; im[base + gj + %lluull] = ai[%d];\n", u, m, u, m);
+                    continue;
+                }
+                o("    { %s vr, vi;\n", VT);
+                for (int e = 0; e < vw; ++e)
+                    o("      vr.%s = ar[%d]; vi.%s = ai[%d];\n", kComp[e], vmember(m, e), kComp[e], vmember(m, e));
+                o("      *reinterpret_cast<%s*>(re + base + gj + %lluull) = vr; *reinterpret_cast<%s*>(im + base + gj + %lluull) = vi; }\n",
+                  VT, u, VT, u);
+            }
+            o("  }\n");
+        } else {
+            for (int m = 0; m < G; ++m)
+                o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs + %uu)) = e; }\n",
+                  m, m, moff[m]);
+            o("  }\n  __syncthreads();\n");
+        }
     }
 
     // stage out
-    for (unsigned c = 0; c < PV; ++c) {
-        const unsigned long long u = deposit_const(static_cast<unsigned long long>(c) * NT * VN, P.qpos, K);
-        if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
-        o("  { V vr, vi; T* a = reinterpret_cast<T*>(&vr); T* b = reinterpret_cast<T*>(&vi);\n");
-        for (int v = 0; v < VN; ++v)
-            o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (stv + %uu)); a[%d] = e.x; b[%d] = e.y; }\n",
-              pad(c * NT * VN + static_cast<unsigned>(v)) * E, v, v);
-        o("    *reinterpret_cast<V*>(re + base + toff + %lluull) = vr; *reinterpret_cast<V*>(im + base + toff + %lluull) = vi; }\n",
-          u, u);
+    if (!direct_out) {
+        for (unsigned c = 0; c < PV; ++c) {
+            const unsigned long long u = deposit_const(static_cast<unsigned long long>(c) * NT * VN, P.qpos, K);
+            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+            o("  { V vr, vi; T* a = reinterpret_cast<T*>(&vr); T* b = reinterpret_cast<T*>(&vi);\n");
+            for (int v = 0; v < VN; ++v)
+                o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (stv + %uu)); a[%d] = e.x; b[%d] = e.y; }\n",
+                  pad(c * NT * VN + static_cast<unsigned>(v)) * E, v, v);
+            o("    *reinterpret_cast<V*>(re + base + toff + %lluull) = vr; *reinterpret_cast<V*>(im + base + toff + %lluull) = vi; }\n",
+              u, u);
+        }
     }
     if (persistent) {
         o("  if (next >= %lluull) break;\n  __syncthreads();\n  tile = next;\n  base = nbase;\n", ntiles);

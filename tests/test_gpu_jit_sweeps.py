@@ -9,10 +9,14 @@ complex64 against the complex128 oracle: L2 <= 2e-5 and fidelity >= 1 - 1e-6):
   flips (a CZ with a lane or outer qubit) as sign XORs.  Circuits rich in
   X / Y (quarter-turn rotations, the cotangent form) and in CZ / controlled
   phases between random pairs exercise every folding case;
-- the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place)
-  and BFLY_JIT_GRID=persistent (blocks walk the tiles, with or without the
-  next tile's loads during the sweeps: BFLY_JIT_PREFETCH) give the same
-  amplitudes within the complex128 bar."""
+- direct stage-in / stage-out (the first sweep loads its members from HBM
+  in its own layout, the last stores them back: scalar, 2- and 4-wide vector
+  accesses), forced on every pass of every sweep form;
+- the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place),
+  BFLY_JIT_GRID=persistent (blocks walk the tiles, with or without the next
+  tile's loads during the sweeps: BFLY_JIT_PREFETCH) and BFLY_JIT_DIRECT
+  (every pass staged through LDS, or direct on one or both sides) give the
+  same amplitudes within the complex128 bar."""
 import math
 import os
 
@@ -110,8 +114,27 @@ def test_flip_heavy_circuits_vs_oracle(gpu_lib, n, tile_bits, threads):
     assert err <= TOL_C128, (n, tile_bits, threads, err)
 
 
+ALL_DIRECT = {"BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"}
+
+
+@pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
+def test_direct_stage_in_and_out_vs_oracle(gpu_lib, monkeypatch, n, tile_bits, threads):
+    for k, v in ALL_DIRECT.items():
+        monkeypatch.setenv(k, v)
+    ops, ud = flips_and_quarter_turns(n, 300, seed=3 * n + threads)
+    st = W.normal_state(n, 4 * n)
+    out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert info["kernels"] > 0
+    err = np.abs(out - oracle(n, ops, ud, st)).max()
+    assert err <= TOL_C128, (n, tile_bits, threads, err)
+
+
+@pytest.mark.parametrize("direct", [False, True])
 @pytest.mark.parametrize("tile_bits,threads", [(13, 1024), (13, 512), (12, 256)])
-def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, tile_bits, threads):
+def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, monkeypatch, tile_bits, threads, direct):
+    if direct:
+        for k, v in ALL_DIRECT.items():
+            monkeypatch.setenv(k, v)
     n = 19
     ops, ud = flips_and_quarter_turns(n, 300, seed=tile_bits + threads)
     st = W.normal_state(n, 5)
@@ -123,7 +146,10 @@ def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, tile_bits, threads)
 
 
 @pytest.mark.parametrize("switches", [{"BFLY_JIT_HOIST": "0"}, {"BFLY_JIT_GRID": "persistent"},
-                                      {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "0"}])
+                                      {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "0"},
+                                      {"BFLY_JIT_DIRECT": "0"}, ALL_DIRECT,
+                                      {"BFLY_JIT_DIRECT": "in", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
+                                      {"BFLY_JIT_DIRECT": "out", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"}])
 def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switches):
     n = 18
     qc = W.random_layered(n, 12, seed=11)

@@ -58,3 +58,44 @@ def test_plans_without_the_flag_report_no_kernels():
     assert p.jit_info() == {"kernels": 0, "compiled": 0}
     pu = Plan(n, np.float64, ops, ud, _lib.opts(mode=_lib.MODE_UNFUSED, flags=_lib.FLAG_JIT))
     assert pu.jit_info()["kernels"] == 0
+
+
+def _dump_sources(tmp_path, n, dtype, env, monkeypatch, **kw):
+    monkeypatch.setenv("BFLY_JIT_DUMP", str(tmp_path))
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    ops, ud = lower_circuit(W.random_layered(n, 40, 2026))
+    p = _plan(n, dtype, ops, ud, **kw)
+    srcs = [(tmp_path / f"pass{i}.hip").read_text() for i in range(p.jit_info()["kernels"])]
+    for f in tmp_path.glob("*.hip"):
+        f.unlink()
+    return srcs
+
+
+def _direct_sides(src):
+    # a staged pass loads the tile into pr / pi and stores it from LDS through
+    # the stage-in offsets; a direct side reads or writes the first / last
+    # sweep's members at base + gj
+    return ("pr[0] = *" not in src,
+            "= vr; *reinterpret_cast<V*>(im + base + toff" not in src)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_direct_stage_in_and_out_only_in_passes_of_six_sweeps(tmp_path, monkeypatch, dtype):
+    n = 20
+    srcs = _dump_sources(tmp_path, n, dtype, {}, monkeypatch)
+    direct = 0
+    for s in srcs:
+        d_in, d_out = _direct_sides(s)
+        sweeps = s.count("// sweep ")
+        if d_in or d_out:
+            assert sweeps >= 6
+            direct += 1
+        if d_in:
+            assert "gj =" in s.split("// sweep 1:")[0]
+    assert direct > 0
+    off = _dump_sources(tmp_path, n, dtype, {"BFLY_JIT_DIRECT": "0"}, monkeypatch)
+    assert len(off) == len(srcs) and not any(any(_direct_sides(s)) for s in off)
+    every = _dump_sources(tmp_path, n, dtype, {"BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
+                          monkeypatch)
+    assert all(all(_direct_sides(s)) for s in every)
