@@ -651,19 +651,27 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         return static_cast<int>(i);
     };
 
-    const int B = f32 ? 5 : 4;
+    int B = 5;
+    if (const char* pb = std::getenv("BFLY_JIT_PAD_BITS")) B = std::atoi(pb) == 4 ? 4 : 5;
     const int E = 2 * elem;   // bytes of a complex element in LDS
     // LDS tile: complex-interleaved elements (re, im adjacent: one 16-byte
     // access per complex128 amplitude, 8 bytes per complex64 one), element j
     // at slot pad(j) = j + (j >> B) + (j >> 2B): one spare slot after every
-    // 2^B and every 2^2B elements, with B = 4 (16 lanes of 16-byte accesses
-    // per LDS cycle) or 5 (32 lanes of 8 bytes).  Lanes whose index bits have
-    // distinct residues mod B fall on distinct banks.  For indices with
+    // 2^B and every 2^2B elements.  Local bit p then moves an element by
+    // 2^(p mod B) slots modulo 2^B, so lanes whose index bits have distinct
+    // residues mod B fall on distinct banks.  B = 5 for both widths: measured
+    // on gfx950 with 64-lane 16-byte accesses, one of ds_read_b128 /
+    // ds_write_b128 serves 32 lanes of 8 bytes per cycle over 128 banks and
+    // the other 8 lanes of 16 bytes over 32 banks; with B = 5 and thread bits
+    // 0-4 on local bits of residues 0-4 both are conflict-free (B = 4: 2.0
+    // bank-conflict cycles per access on average in the tile passes of the
+    // 30-qubit layered circuit, SQ_LDS_BANK_CONFLICT).  For indices with
     // disjoint bits the map is additive (no carries: pad(a | b) = pad(a) +
     // pad(b)), so a sweep's member addresses are one base register plus
     // immediate offsets; an XOR swizzle needs one XOR per member, which the
     // compiler hoists out of the tile loop for every sweep at once (208
-    // VGPRs for 13 sweeps of 16 members: spills).
+    // VGPRs for 13 sweeps of 16 members: spills).  BFLY_JIT_PAD_BITS=4 keeps
+    // the 16-slot rows for measurement.
     auto pad = [&](unsigned j) { return j + (j >> B) + (j >> (2 * B)); };
     const unsigned TSP = pad(TS - 1) + 1;   // slots of the padded tile
     const int VN = f32 ? 4 : 2;   // amplitudes per 16-byte stage-in access
