@@ -261,6 +261,7 @@ int bfly_state_destroy(bfly_state* s) {
     if (s->re) (void)hipFree(s->re);
     if (s->im) (void)hipFree(s->im);
     if (s->scratch) (void)hipFree(s->scratch);
+    if (s->tiles) (void)hipFree(s->tiles);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     s->re = s->im = nullptr;
     delete s;
@@ -563,6 +564,24 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
     HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
     if ((rc = upload_plan(p, s->device))) return rc;
     ProfScope prof((p->hp.opts.flags & BFLY_FLAG_PROFILE) != 0, s->stream);
+    if (p->jit && p->jit->device == s->device && p->jit->nqueue > 0) {
+        // the state's tile counters, grown when a plan needs more (the old
+        // ones are freed once the state's queued work is done), then cleared
+        // on its stream ahead of the passes
+        if (s->tiles_cap < p->jit->nqueue) {
+            const int cap = std::max(p->jit->nqueue, 2 * s->tiles_cap);
+            if (s->tiles) {
+                HIP_TRY(hipStreamSynchronize(s->stream), "hipStreamSynchronize(tile counters)");
+                (void)hipFree(s->tiles);
+                s->tiles = nullptr;
+                s->tiles_cap = 0;
+            }
+            HIP_TRY(hipMalloc(&s->tiles, sizeof(unsigned long long) * static_cast<size_t>(cap)), "hipMalloc(tile counters)");
+            s->tiles_cap = cap;
+        }
+        const hipError_t e = launch_zero_words(s->tiles, p->jit->nqueue, s->stream);
+        if (e != hipSuccess) return fail(BFLY_EHIP, std::string("tile counters: ") + hipGetErrorString(e));
+    }
     for (size_t si = 0; si < p->hp.steps.size(); ++si) {
         const Step& st = p->hp.steps[si];
         hipError_t e = hipSuccess;
@@ -572,7 +591,7 @@ int bfly_plan_execute(bfly_plan* p, bfly_state* s) {
                 if (p->jit && p->jit->device == s->device && p->jit->step_pass[si] >= 0) {
                     const JitPass& jp = p->jit->passes[static_cast<size_t>(p->jit->step_pass[si])];
                     e = jit_launch(jp, s->re, s->im, static_cast<const char*>(p->jit->d_consts) + jp.const_off,
-                                   s->stream);
+                                   jp.queue ? s->tiles + jp.qidx : nullptr, s->stream);
                 } else {
                     e = launch_tile_pass(s->dtype, s->re, s->im, st.tile, p->d_sweeps, p->d_instrs, s->stream);
                 }

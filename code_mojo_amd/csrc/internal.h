@@ -19,6 +19,10 @@ struct bfly_state {
     void* im = nullptr;
     double* scratch = nullptr;   // reduction partials
     hipStream_t stream = nullptr;
+    // tile counters of work-queue tile passes (one per such pass of the plan
+    // being executed; calls on one state are ordered on its stream)
+    unsigned long long* tiles = nullptr;
+    int tiles_cap = 0;
 };
 
 struct bfly_plan {

@@ -620,21 +620,34 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     const int NT = tile_threads_h(K, S);
     const unsigned NG = TS / G;
     const int WPE = tile_waves_per_eu_h(elem, K, S);
-    // One block per tile: the hardware dispatcher starts a new block as soon
-    // as one finishes, so the stage-in loads of some blocks overlap the
-    // sweeps of others on every CU.  Measured on the 30-qubit layered circuit
-    // against persistent blocks that walk the tiles with the next tile's
-    // loads issued before their sweeps: complex128 297 vs 333 ms, complex64
-    // 157 vs 176 ms (first-fit schedule 389 vs 435 ms).  BFLY_JIT_GRID=
-    // persistent builds the persistent form (BFLY_JIT_PREFETCH=0: without
-    // the next tile's loads) for measurement.
-    bool persistent = false;
-    if (const char* g = std::getenv("BFLY_JIT_GRID")) persistent = g[0] == 'p';
+    // Launch form.  Work queue (the default from 4096 tiles): a grid of the
+    // blocks the device holds at once, each taking its next tile from a
+    // counter the executing state holds, with that tile's loads issued during
+    // the current tile's first sweep (in the first sweep's register layout
+    // where the pass loads directly).  One block per tile (fewer tiles, or
+    // BFLY_JIT_GRID=tile): the hardware dispatcher starts a new block as soon
+    // as one finishes.  Persistent blocks with a fixed share of the tiles
+    // each (BFLY_JIT_GRID=persistent) wait for the slowest block.  Measured
+    // on the 30-qubit layered circuit (ms per circuit, complex128 searched /
+    // first-fit tile qubits, complex64): queue 282.5-283.1 / 389 / 157.1,
+    // one block per tile 285.3-285.5 / 410 / 157.8, fixed shares 320-323;
+    // the queue without next-tile loads (BFLY_JIT_PREFETCH=0) 291.4.
+    // BFLY_JIT_QUEUE_MIN_TILES sets the threshold (tests).
+    unsigned long long min_queue_tiles = 4096;
+    if (const char* mt = std::getenv("BFLY_JIT_QUEUE_MIN_TILES")) min_queue_tiles = std::strtoull(mt, nullptr, 10);
+    bool queue = (1ull << (P.n - K)) >= min_queue_tiles;
+    bool persistent = queue;
+    if (const char* g = std::getenv("BFLY_JIT_GRID")) {
+        if (g[0] == 't') persistent = queue = false;
+        else if (g[0] == 'p') { persistent = true; queue = false; }
+        else if (g[0] == 'q') persistent = queue = true;
+    }
     // a dispatch covers at most 2^32 - 1 work-items per dimension: beyond
     // that (2^33 amplitudes in 2^9-amplitude tiles, 2^36 in 2^12) the
     // blocks walk the tiles
-    if ((1ull << (P.n - P.k)) * static_cast<unsigned long long>(tile_threads_h(P.k, P.slots)) > 0xffffffffull)
-        persistent = true;
+    if (!persistent &&
+        (1ull << (P.n - P.k)) * static_cast<unsigned long long>(tile_threads_h(P.k, P.slots)) > 0xffffffffull)
+        persistent = queue = true;
     bool prefetch = persistent;
     if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = persistent && pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
@@ -643,6 +656,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     js.threads = NT;
     js.ntiles = ntiles;
     js.persistent = persistent;
+    js.queue = queue;
     Out o;
     auto cst = [&](double dv, float fv) {
         const size_t i = f32 ? js.cf.size() : js.cd.size();
@@ -744,10 +758,73 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // =1 takes the first and last sweep's layouts whatever their runs, =in /
     // =out only one side; BFLY_JIT_DIRECT_MIN_SWEEPS sets the pass size (1:
     // every pass).
+    // Direct member accesses: member m of a thread's group is local index
+    // jb | d(m) (disjoint bits), at global offset base + deposit(jb) +
+    // deposit(d(m)).  Members differing only in the slots on local bits 0
+    // (and 1) sit at consecutive global addresses when those local bits are
+    // global bits 0 (and 1): one vector access carries them.
+    struct MemberIo {
+        int vw;
+        int vslot[2];
+        unsigned mloc[1 << kSlots];
+    };
+    auto member_io = [&](const TileSweep& sw) {
+        MemberIo f{1, {-1, -1}, {}};
+        for (int m = 0; m < G; ++m) {
+            unsigned d = 0;
+            for (int a = 0; a < S; ++a) d |= ((static_cast<unsigned>(m) >> a) & 1u) << sw.s[a];
+            f.mloc[m] = d;
+        }
+        for (int a = 0; a < S; ++a) {
+            if (sw.s[a] == 0 && P.qpos[0] == 0) f.vslot[0] = a;
+            if (sw.s[a] == 1 && P.qpos[1] == 1) f.vslot[1] = a;
+        }
+        if (f.vslot[0] >= 0) f.vw = (f32 && f.vslot[1] >= 0) ? 4 : 2;
+        return f;
+    };
+    // loads (store = false) or stores of a thread's members dr[] / di[] at basevar + gjvar
+    auto emit_member_io = [&](const MemberIo& f, bool store, const char* dr, const char* di, const char* basevar,
+                              const char* gjvar) {
+        static const char* const kComp[4] = {"x", "y", "z", "w"};
+        const char* VT = f.vw == 1 ? T : (f32 ? (f.vw == 4 ? "float4" : "float2") : "double2");
+        auto vmember = [&](int m, int e) {   // member of element e of the vector at member m
+            int r = m;
+            if (e & 1) r |= 1 << f.vslot[0];
+            if (e & 2) r |= 1 << f.vslot[1];
+            return r;
+        };
+        for (int m = 0; m < G; ++m) {
+            if ((f.vw >= 2 && ((m >> f.vslot[0]) & 1)) || (f.vw == 4 && ((m >> f.vslot[1]) & 1))) continue;
+            const unsigned long long u = deposit_const(f.mloc[m], P.qpos, K);
+            if (f.vw == 1) {
+                if (store)
+                    o("    re[%s + %s + %lluull] = %s[%d]; im[%s + %s + %lluull] = %s[%d];\n", basevar, gjvar, u, dr, m,
+                      basevar, gjvar, u, di, m);
+                else
+                    o("    %s[%d] = re[%s + %s + %lluull]; %s[%d] = im[%s + %s + %lluull];\n", dr, m, basevar, gjvar, u,
+                      di, m, basevar, gjvar, u);
+                continue;
+            }
+            if (store) {
+                o("    { %s vr, vi;\n", VT);
+                for (int e = 0; e < f.vw; ++e)
+                    o("      vr.%s = %s[%d]; vi.%s = %s[%d];\n", kComp[e], dr, vmember(m, e), kComp[e], di, vmember(m, e));
+                o("      *reinterpret_cast<%s*>(re + %s + %s + %lluull) = vr; *reinterpret_cast<%s*>(im + %s + %s + %lluull) = vi; }\n",
+                  VT, basevar, gjvar, u, VT, basevar, gjvar, u);
+            } else {
+                o("    { const %s vr = *reinterpret_cast<const %s*>(re + %s + %s + %lluull);"
+                  " const %s vi = *reinterpret_cast<const %s*>(im + %s + %s + %lluull);\n",
+                  VT, VT, basevar, gjvar, u, VT, VT, basevar, gjvar, u);
+                for (int e = 0; e < f.vw; ++e)
+                    o("      %s[%d] = vr.%s; %s[%d] = vi.%s;\n", dr, vmember(m, e), kComp[e], di, vmember(m, e), kComp[e]);
+                o("    }\n");
+            }
+        }
+    };
     bool direct_in = false, direct_out = false;
     int min_sweeps = 6;
     if (const char* ms = std::getenv("BFLY_JIT_DIRECT_MIN_SWEEPS")) min_sweeps = std::atoi(ms);
-    if (!persistent && NT == static_cast<int>(NG) && P.nsweeps > 0 && P.nsweeps >= min_sweeps) {
+    if ((!persistent || (queue && prefetch)) && NT == static_cast<int>(NG) && P.nsweeps > 0 && P.nsweeps >= min_sweeps) {
         const TileSweep& s0 = hp.sweeps[static_cast<size_t>(P.first_sweep)];
         const TileSweep& s1 = hp.sweeps[static_cast<size_t>(P.first_sweep + P.nsweeps - 1)];
         const int need = staged_bits();
@@ -763,12 +840,23 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     o("typedef %s T;\ntypedef %s C2;\n", T, f32 ? "float2" : "double2");
     o("__device__ __forceinline__ unsigned padc(unsigned j) { return j + (j >> %d) + (j >> %d); }\n", B, 2 * B);
     o("extern \"C\" __global__ __launch_bounds__(%d, %d) void bfly_tile_pass(T* __restrict__ re, T* __restrict__ im, "
-      "const T* __restrict__ Cp) {\n",
-      NT, WPE);
+      "const T* __restrict__ Cp%s) {\n",
+      NT, WPE, queue ? ", unsigned long long* __restrict__ Q" : "");
     o("  using namespace bfly;\n");
     o("  __shared__ __attribute__((aligned(16))) C2 s_c[%u];\n", TSP);
     o("  const unsigned tid = threadIdx.x;\n");
-    o("  unsigned long long tile = blockIdx.x;\n  if (tile >= %lluull) return;\n", ntiles);
+    if (queue) {
+        // Work queue: thread 0 takes the next tile index from the pass's
+        // counter; the index reaches the block through LDS, double-buffered
+        // by iteration parity so that a take never overwrites an index some
+        // wave has not read yet (one barrier separates them).  Every block
+        // stops at the first index >= ntiles; the counter only grows.
+        o("  __shared__ unsigned long long s_tile[2];\n");
+        o("  if (tid == 0) s_tile[0] = atomicAdd(Q, 1ull);\n  __syncthreads();\n");
+        o("  unsigned long long tile = s_tile[0];\n  if (tile >= %lluull) return;\n  unsigned par = 1u;\n", ntiles);
+    } else {
+        o("  unsigned long long tile = blockIdx.x;\n  if (tile >= %lluull) return;\n", ntiles);
+    }
     // Stage-in / stage-out: each lane moves VN consecutive amplitudes per
     // 16-byte HBM access (re and im); thread tid holds local indices
     // (tid + c NT) VN + v, whose global offsets split into deposit(tid VN) and
@@ -796,6 +884,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     if (!direct_in) {
         load("base");
         if (persistent) o("  for (;;) {\n");
+        if (queue) o("  if (tid == 0) s_tile[par] = atomicAdd(Q, 1ull);\n");
         for (unsigned c = 0; c < PV; ++c) {
             if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
             o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
@@ -805,11 +894,28 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
             o("  }\n");
         }
         o("  __syncthreads();\n");
+    } else if (queue) {
+        // the first tile's members in the first sweep's layout; the loop then
+        // loads each next tile's during the current tile's first sweep
+        const TileSweep& s0 = hp.sweeps[static_cast<size_t>(P.first_sweep)];
+        int32_t tp0[kMaxTileBits];
+        thread_pos(s0, tp0);
+        o("  T nr[%d], ni[%d];\n", G, G);
+        o("  {\n    const unsigned jb = static_cast<unsigned>(%s);\n", deposit_expr("tid", tp0, 0, K - S).c_str());
+        o("    const unsigned long long gj = %s;\n", deposit_expr("jb", P.qpos, 0, K).c_str());
+        emit_member_io(member_io(s0), false, "nr", "ni", "base", "gj");
+        o("  }\n  for (;;) {\n");
+        o("  if (tid == 0) s_tile[par] = atomicAdd(Q, 1ull);\n  __syncthreads();\n");
     }
     if (!persistent) {
         // one tile per block: nothing to load ahead
+    } else if (prefetch && queue && direct_in) {
+        o("  const unsigned long long next = s_tile[par];\n  par ^= 1u;\n");
+        o("  const unsigned long long nc = next < %lluull ? next : tile;\n", ntiles);
+        o("  const unsigned long long nbase = %s;\n", tile_base("nc").c_str());
     } else if (prefetch) {
-        o("  const unsigned long long next = tile + gridDim.x;\n");
+        if (queue) o("  const unsigned long long next = s_tile[par];\n  par ^= 1u;\n");
+        else o("  const unsigned long long next = tile + gridDim.x;\n");
         // The next tile's loads are issued before the sweeps and stay in
         // flight across them.  They are unconditional (the last tile reloads
         // its own amplitudes, which are never used): a conditional load makes
@@ -819,7 +925,8 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("  const unsigned long long nbase = %s;\n", tile_base("nc").c_str());
         load("nbase");
     } else {
-        o("  const unsigned long long next = tile + gridDim.x;\n");
+        if (queue) o("  const unsigned long long next = s_tile[par];\n  par ^= 1u;\n");
+        else o("  const unsigned long long next = tile + gridDim.x;\n");
         o("  unsigned long long nbase = 0;\n");
         o("  if (next < %lluull) nbase = %s;\n", ntiles, tile_base("next").c_str());
     }
@@ -839,54 +946,21 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("    const unsigned jbs = padc(jb) * %du;\n", E);
         o("    T ar[%d], ai[%d];\n", G, G);
         unsigned moff[1 << kSlots];
-        unsigned mloc[1 << kSlots];
         for (int m = 0; m < G; ++m) {
             unsigned d = 0;
             for (int a = 0; a < S; ++a) d |= ((static_cast<unsigned>(m) >> a) & 1u) << sw.s[a];
-            mloc[m] = d;
             moff[m] = pad(d) * E;
         }
         const bool from_hbm = direct_in && s == 0, to_hbm = direct_out && s == P.nsweeps - 1;
-        // Direct member accesses: member m of the thread's group is local
-        // index jb | d(m) (disjoint bits), at global offset base + deposit(jb)
-        // + deposit(d(m)).  Members differing only in the slots on local bits
-        // 0 (and 1) sit at consecutive global addresses when those local bits
-        // are global bits 0 (and 1): one vector access carries them.
-        int vw = 1, vslot[2] = {-1, -1};
-        if (from_hbm || to_hbm) {
-            o("    const unsigned long long gj = %s;\n", deposit_expr("jb", P.qpos, 0, K).c_str());
-            for (int a = 0; a < S; ++a) {
-                if (sw.s[a] == 0 && P.qpos[0] == 0) vslot[0] = a;
-                if (sw.s[a] == 1 && P.qpos[1] == 1) vslot[1] = a;
-            }
-            if (vslot[0] >= 0) vw = (f32 && vslot[1] >= 0) ? 4 : 2;
-        }
-        const char* VT = vw == 1 ? T : (f32 ? (vw == 4 ? "float4" : "float2") : "double2");
-        static const char* const kComp[4] = {"x", "y", "z", "w"};
-        auto vmember = [&](int m, int e) {   // member of element e of the vector at member m
-            int r = m;
-            if (e & 1) r |= 1 << vslot[0];
-            if (e & 2) r |= 1 << vslot[1];
-            return r;
-        };
-        auto vhead = [&](int m) {
-            return !(vw >= 2 && ((m >> vslot[0]) & 1)) && !(vw == 4 && ((m >> vslot[1]) & 1));
-        };
-        if (from_hbm) {
-            for (int m = 0; m < G; ++m) {
-                if (!vhead(m)) continue;
-                const unsigned long long u = deposit_const(mloc[m], P.qpos, K);
-                if (vw == 1) {
-                    o("    ar[%d] = re[base + gj + %lluull]; ai[%d] = im[base + gj + %lluull];\n", m, u, m, u);
-                    continue;
-                }
-                o("    { const %s vr = *reinterpret_cast<const %s*>(re + base + gj + %lluull);"
-                  " const %s vi = *reinterpret_cast<const %s*>(im + base + gj + %lluull);\n",
-                  VT, VT, u, VT, VT, u);
-                for (int e = 0; e < vw; ++e)
-                    o("      ar[%d] = vr.%s; ai[%d] = vi.%s;\n", vmember(m, e), kComp[e], vmember(m, e), kComp[e]);
-                o("    }\n");
-            }
+        const MemberIo fio = member_io(sw);
+        if (from_hbm || to_hbm) o("    const unsigned long long gj = %s;\n", deposit_expr("jb", P.qpos, 0, K).c_str());
+        if (from_hbm && queue) {
+            // this tile's members arrived in nr / ni one tile ahead; the next
+            // tile's loads go there now and stay in flight across the sweeps
+            for (int m = 0; m < G; ++m) o("    ar[%d] = nr[%d]; ai[%d] = ni[%d];\n", m, m, m, m);
+            emit_member_io(fio, false, "nr", "ni", "nbase", "gj");
+        } else if (from_hbm) {
+            emit_member_io(fio, false, "ar", "ai", "base", "gj");
         } else {
             for (int m = 0; m < G; ++m)
                 o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
@@ -898,19 +972,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         fold_signs(ir, S);
         emit_ops(o, ir, S, cst);
         if (to_hbm) {
-            for (int m = 0; m < G; ++m) {
-                if (!vhead(m)) continue;
-                const unsigned long long u = deposit_const(mloc[m], P.qpos, K);
-                if (vw == 1) {
-                    o("    re[base + gj + %lluull] = ar[%d]; im[base + gj + %lluull] = ai[%d];\n", u, m, u, m);
-                    continue;
-                }
-                o("    { %s vr, vi;\n", VT);
-                for (int e = 0; e < vw; ++e)
-                    o("      vr.%s = ar[%d]; vi.%s = ai[%d];\n", kComp[e], vmember(m, e), kComp[e], vmember(m, e));
-                o("      *reinterpret_cast<%s*>(re + base + gj + %lluull) = vr; *reinterpret_cast<%s*>(im + base + gj + %lluull) = vi; }\n",
-                  VT, u, VT, u);
-            }
+            emit_member_io(fio, true, "ar", "ai", "base", "gj");
             o("  }\n");
         } else {
             for (int m = 0; m < G; ++m)
@@ -1002,6 +1064,8 @@ int jit_build(JitPlan& jp, const HostPlan& hp, std::string& err) {
         pass.threads = srcs.back().threads;
         pass.ntiles = srcs.back().ntiles;
         pass.persistent = srcs.back().persistent;
+        pass.queue = srcs.back().queue;
+        if (pass.queue) pass.qidx = jp.nqueue++;
         // constants: 16-byte aligned offsets into one array per plan
         const size_t elem = hp.dtype == BFLY_C64 ? 4 : 8;
         while ((jp.consts.size() % 16) != 0) jp.consts.push_back(0);
@@ -1113,9 +1177,10 @@ int jit_load(JitPlan& jp, int device, std::string& err) {
     return BFLY_OK;
 }
 
-hipError_t jit_launch(const JitPass& ps, void* re, void* im, const void* consts, hipStream_t stream) {
+hipError_t jit_launch(const JitPass& ps, void* re, void* im, const void* consts, unsigned long long* counter,
+                      hipStream_t stream) {
     void* c = const_cast<void*>(consts);
-    void* args[3] = {&re, &im, &c};
+    void* args[4] = {&re, &im, &c, &counter};
     return hipModuleLaunchKernel(ps.fn, ps.grid, 1, 1, static_cast<unsigned>(ps.threads), 1, 1, 0, stream, args,
                                  nullptr);
 }

@@ -22,6 +22,7 @@ struct JitSource {
     int threads = 0;
     unsigned long long ntiles = 0;
     bool persistent = false;       // blocks walk the tiles (else one block per tile)
+    bool queue = false;            // persistent blocks take tiles from a device counter
 };
 
 struct JitPass {
@@ -30,6 +31,8 @@ struct JitPass {
     int threads = 0;
     unsigned long long ntiles = 0;
     bool persistent = false;
+    bool queue = false;            // takes its tile counter as a fourth argument
+    int qidx = -1;                 // its counter among the state's tile counters
     unsigned grid = 0;             // blocks per launch on the loaded device
     hipFunction_t fn = nullptr;
 };
@@ -42,6 +45,7 @@ struct JitPlan {
     std::vector<JitPass> passes;
     std::vector<char> consts;      // every pass's parameters, 16-byte aligned per pass
     void* d_consts = nullptr;      // device copy (owned by the plan)
+    int nqueue = 0;                // passes with a tile counter (the executing state holds them)
 };
 
 bool jit_supported(const HostPlan& hp, const Step& st);
@@ -50,7 +54,8 @@ JitSource jit_generate(const HostPlan& hp, const Step& st);
 int jit_build(JitPlan& jp, const HostPlan& hp, std::string& err);
 // Loads the kernels on `device` (the caller made it current) and sizes their grids.
 int jit_load(JitPlan& jp, int device, std::string& err);
-hipError_t jit_launch(const JitPass& ps, void* re, void* im, const void* consts, hipStream_t stream);
+hipError_t jit_launch(const JitPass& ps, void* re, void* im, const void* consts, unsigned long long* counter,
+                      hipStream_t stream);
 int jit_cache_size();
 
 }  // namespace bfly

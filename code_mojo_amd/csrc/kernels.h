@@ -43,6 +43,7 @@ hipError_t launch_swap(int dtype, void* re, void* im, int n, int a, int b, hipSt
 hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int m, uint64_t controls,
                           const double* d_u, hipStream_t stream);
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream);
+hipError_t launch_zero_words(unsigned long long* p, int n, hipStream_t stream);
 hipError_t launch_scale(int dtype, void* re, void* im, uint64_t size, double scale, hipStream_t stream);
 // Measurement (apply_measure, state.mojo:447-553): outcome probabilities of
 // up to kOutcomeMaxBits free qubits among the amplitudes with the forced

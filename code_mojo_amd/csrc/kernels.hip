@@ -2867,6 +2867,19 @@ hipError_t launch_unitary(int dtype, void* re, void* im, int n, int target, int 
     return hipGetLastError();
 }
 
+// Clears n 64-bit words (the tile counters of work-queue passes) with a
+// kernel, so that a captured plan execution records no memset node.
+__global__ __launch_bounds__(64) void zero_words_kernel(unsigned long long* __restrict__ p, int n) {
+    const int i = static_cast<int>(blockIdx.x * 64u + threadIdx.x);
+    if (i < n) p[i] = 0ull;
+}
+
+hipError_t launch_zero_words(unsigned long long* p, int n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_reset(int dtype, void* re, void* im, uint64_t size, hipStream_t stream) {
     if (dtype == BFLY_C64)
         hipLaunchKernelGGL(reset_kernel<float>, dim3(grid_for(size)), dim3(256), 0, stream, static_cast<float*>(re),

@@ -12,6 +12,10 @@ complex64 against the complex128 oracle: L2 <= 2e-5 and fidelity >= 1 - 1e-6):
 - direct stage-in / stage-out (the first sweep loads its members from HBM
   in its own layout, the last stores them back: scalar, 2- and 4-wide vector
   accesses), forced on every pass of every sweep form;
+- the work-queue launch form (blocks take tiles from the executing state's
+  counter and load the next tile during the current one), forced on every
+  pass of every sweep form, repeated on a new state, and one plan on two
+  states at once;
 - the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place),
   BFLY_JIT_GRID=persistent (blocks walk the tiles, with or without the next
   tile's loads during the sweeps: BFLY_JIT_PREFETCH) and BFLY_JIT_DIRECT
@@ -115,6 +119,9 @@ def test_flip_heavy_circuits_vs_oracle(gpu_lib, n, tile_bits, threads):
 
 
 ALL_DIRECT = {"BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"}
+# the work-queue launch form (persistent blocks taking tiles from the state's
+# counter, next tile loaded during the current one) on every pass
+ALL_QUEUE = {"BFLY_JIT_QUEUE_MIN_TILES": "1"}
 
 
 @pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
@@ -129,12 +136,56 @@ def test_direct_stage_in_and_out_vs_oracle(gpu_lib, monkeypatch, n, tile_bits, t
     assert err <= TOL_C128, (n, tile_bits, threads, err)
 
 
-@pytest.mark.parametrize("direct", [False, True])
+@pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
+def test_work_queue_form_vs_oracle(gpu_lib, monkeypatch, n, tile_bits, threads):
+    for k, v in ALL_QUEUE.items():
+        monkeypatch.setenv(k, v)
+    ops, ud = flips_and_quarter_turns(n, 300, seed=5 * n + threads)
+    st = W.normal_state(n, 6 * n)
+    ref = oracle(n, ops, ud, st)
+    out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert info["kernels"] > 0
+    assert np.abs(out - ref).max() <= TOL_C128, (n, tile_bits, threads)
+    # the same plan again on a new state: the counters are cleared per execution
+    out2, _ = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert np.array_equal(out, out2)
+
+
+def test_one_work_queue_plan_on_two_states_at_once(gpu_lib, monkeypatch):
+    """Calls on different states may overlap: each state holds its own tile
+    counters, so one plan running on two states' streams at once gives both
+    the oracle's amplitudes."""
+    for k, v in ALL_QUEUE.items():
+        monkeypatch.setenv(k, v)
+    n = 19
+    qc = W.random_layered(n, 12, seed=31)
+    ops, ud = lower_circuit(qc)
+    sts = [W.normal_state(n, 41), W.normal_state(n, 42)]
+    gs = []
+    for st in sts:
+        g = GpuState(n, np.float64)
+        g.upload(QuantumState(n, np.float64, st.re, st.im))
+        gs.append(g)
+    p = Plan(n, np.float64, ops, ud, _lib.opts(flags=_lib.FLAG_JIT))
+    for _ in range(3):
+        for g in gs:
+            p.execute(g)      # asynchronous: the two states' streams overlap
+    for g, st in zip(gs, sts):
+        out = g.download()
+        got = out.re + 1j * out.im
+        re, im = st.re.copy(), st.im.copy()
+        for _ in range(3):
+            O.execute(n, re, im, ops, ud)
+        assert np.abs(got - (re + 1j * im)).max() <= TOL_C128
+        g.free()
+    p.free()
+
+
+@pytest.mark.parametrize("form", ["tile", "direct", "queue"])
 @pytest.mark.parametrize("tile_bits,threads", [(13, 1024), (13, 512), (12, 256)])
-def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, monkeypatch, tile_bits, threads, direct):
-    if direct:
-        for k, v in ALL_DIRECT.items():
-            monkeypatch.setenv(k, v)
+def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, monkeypatch, tile_bits, threads, form):
+    for k, v in {"tile": {}, "direct": ALL_DIRECT, "queue": {**ALL_DIRECT, **ALL_QUEUE}}[form].items():
+        monkeypatch.setenv(k, v)
     n = 19
     ops, ud = flips_and_quarter_turns(n, 300, seed=tile_bits + threads)
     st = W.normal_state(n, 5)
@@ -146,6 +197,9 @@ def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, monkeypatch, tile_b
 
 
 @pytest.mark.parametrize("switches", [{"BFLY_JIT_HOIST": "0"}, {"BFLY_JIT_GRID": "persistent"},
+                                      {"BFLY_JIT_GRID": "queue"}, {"BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "0"},
+                                      {"BFLY_JIT_GRID": "queue", "BFLY_JIT_DIRECT": "1",
+                                       "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
                                       {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "0"},
                                       {"BFLY_JIT_DIRECT": "0"}, ALL_DIRECT,
                                       {"BFLY_JIT_DIRECT": "in", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},

@@ -99,3 +99,29 @@ def test_direct_stage_in_and_out_only_in_passes_of_six_sweeps(tmp_path, monkeypa
     every = _dump_sources(tmp_path, n, dtype, {"BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
                           monkeypatch)
     assert all(all(_direct_sides(s)) for s in every)
+
+
+def test_work_queue_form_from_4096_tiles(tmp_path, monkeypatch):
+    # 2^12-amplitude tiles: 24 qubits make 4096 tiles per pass and take the
+    # work-queue form (tile counter argument, atomic take), 23 qubits do not;
+    # BFLY_JIT_GRID=tile keeps one block per tile
+    monkeypatch.setenv("BFLY_JIT_DUMP", str(tmp_path))
+
+    def sources(n, **env):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ops, ud = lower_circuit(W.random_layered(n, 4, 7))
+        p = _plan(n, np.float64, ops, ud, tile_bits=12)
+        out = [(tmp_path / f"pass{i}.hip").read_text() for i in range(p.jit_info()["kernels"])]
+        for f in tmp_path.glob("*.hip"):
+            f.unlink()
+        for k in env:
+            monkeypatch.delenv(k)
+        return out
+
+    q24 = sources(24)
+    assert q24 and all("atomicAdd(Q, 1ull)" in s and "unsigned long long* __restrict__ Q" in s for s in q24)
+    assert not any("atomicAdd" in s for s in sources(23))
+    assert not any("atomicAdd" in s for s in sources(24, BFLY_JIT_GRID="tile"))
+    forced = sources(14, BFLY_JIT_QUEUE_MIN_TILES="1")
+    assert forced and all("atomicAdd(Q, 1ull)" in s for s in forced)
