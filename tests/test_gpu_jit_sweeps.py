@@ -224,3 +224,22 @@ def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switches):
     assert info["compiled"] > 0   # the switched sources differ, so they compile anew
     assert np.abs(off - ref).max() <= TOL_C128
     assert np.abs(base - ref).max() <= TOL_C128
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_work_queue_default_is_bit_identical_to_one_block_per_tile(gpu_lib, monkeypatch, dtype):
+    """26 qubits: every pass has 2^14 (complex128) or 2^13 (complex64) tiles
+    and takes the work-queue form by default; each tile runs the same code in
+    either form, so the amplitudes equal one block per tile bit for bit."""
+    n = 26
+    qc = W.random_layered(n, 10, seed=77)
+    ops, ud = lower_circuit(qc)
+    st = W.normal_state(n, 9)
+    q, info = run(n, ops, ud, st, dtype=dtype)
+    assert info["kernels"] > 0
+    monkeypatch.setenv("BFLY_JIT_GRID", "tile")
+    t, info_t = run(n, ops, ud, st, dtype=dtype)
+    assert info_t["compiled"] > 0      # a different launch form, compiled anew
+    assert np.array_equal(q, t)
+    nrm = float(np.vdot(q, q).real)
+    assert abs(nrm - float(np.vdot(st.re + 1j * st.im, st.re + 1j * st.im).real)) <= (1e-9 if dtype == np.float64 else 1e-3)
