@@ -620,22 +620,46 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     const int NT = tile_threads_h(K, S);
     const unsigned NG = TS / G;
     const int WPE = tile_waves_per_eu_h(elem, K, S);
-    // Launch form.  Work queue (the default from 4096 tiles): a grid of the
-    // blocks the device holds at once, each taking its next tile from a
-    // counter the executing state holds, with that tile's loads issued during
-    // the current tile's first sweep (in the first sweep's register layout
-    // where the pass loads directly).  One block per tile (fewer tiles, or
+    // Split round trips (complex128): the LDS tile holds one component at a
+    // time, so a sweep boundary moves re, then im, through a buffer of half
+    // the size (three barriers instead of one: re out, re in, im out, im in)
+    // and three blocks fit a CU instead of two (the registers allow three
+    // waves per SIMD).  Needs one group of members per thread (the barriers
+    // sit inside the sweeps).  A standalone probe of the same pass shape (9
+    // sweeps of about 250 FP64 VALU per thread, 2^18 tiles): 11.96 ms with
+    // 16-byte round trips at two blocks per CU, 9.76 ms split at three, 12.14
+    // ms split at two.  Taken in passes of 6 or more sweeps: light passes are
+    // bound by their HBM stream, where the work queue's next-tile loads (whose
+    // registers do not fit three blocks per CU) pay more.  Measured on the
+    // 30-qubit layered circuit, ms per circuit (searched / first-fit tile
+    // qubits; one box): split in every pass 270.6 / 408.9, from 6 sweeps
+    // 271.5 / 391.8, from 10 sweeps 282.6 / 394.0.  BFLY_JIT_SPLIT=0 keeps
+    // 16-byte round trips, BFLY_JIT_SPLIT_MIN_SWEEPS sets the pass size.
+    int split_min_sweeps = 6;
+    if (const char* ms = std::getenv("BFLY_JIT_SPLIT_MIN_SWEEPS")) split_min_sweeps = std::atoi(ms);
+    bool split = !f32 && NT == static_cast<int>(NG) && P.nsweeps > 0 && P.nsweeps >= split_min_sweeps;
+    if (const char* sp = std::getenv("BFLY_JIT_SPLIT")) split = split && sp[0] != '0';
+    int WPE_used = split ? 3 : WPE;
+    if (const char* w = std::getenv("BFLY_JIT_WAVES_PER_SIMD")) WPE_used = std::max(1, std::atoi(w));
+    // Launch form.  One block per tile (split passes; fewer than 4096 tiles;
     // BFLY_JIT_GRID=tile): the hardware dispatcher starts a new block as soon
-    // as one finishes.  Persistent blocks with a fixed share of the tiles
-    // each (BFLY_JIT_GRID=persistent) wait for the slowest block.  Measured
-    // on the 30-qubit layered circuit (ms per circuit, complex128 searched /
-    // first-fit tile qubits, complex64): queue 282.5-283.1 / 389 / 157.1,
-    // one block per tile 285.3-285.5 / 410 / 157.8, fixed shares 320-323;
-    // the queue without next-tile loads (BFLY_JIT_PREFETCH=0) 291.4.
+    // as one finishes.  Work queue (other passes of 4096 tiles or more): a
+    // grid of the blocks the device holds at once, each taking its next tile
+    // from a counter the executing state holds, with that tile's loads issued
+    // during the current tile's first sweep (in the first sweep's register
+    // layout where the pass loads directly).  Persistent blocks with a fixed
+    // share of the tiles each (BFLY_JIT_GRID=persistent) wait for the slowest
+    // block.  Measured on the 30-qubit layered circuit (ms per circuit):
+    // 16-byte round trips: complex128 searched / first-fit tile qubits,
+    // complex64: queue 282.5-283.1 / 389 / 157.1, one block per tile
+    // 285.3-285.5 / 410 / 157.8, fixed shares 320-323, the queue without
+    // next-tile loads (BFLY_JIT_PREFETCH=0) 291.4; split complex128: one
+    // block per tile 262.0, queue without next-tile loads 273.1, with them
+    // 304.9 (their 64 registers spill at three blocks per CU).
     // BFLY_JIT_QUEUE_MIN_TILES sets the threshold (tests).
     unsigned long long min_queue_tiles = 4096;
     if (const char* mt = std::getenv("BFLY_JIT_QUEUE_MIN_TILES")) min_queue_tiles = std::strtoull(mt, nullptr, 10);
-    bool queue = (1ull << (P.n - K)) >= min_queue_tiles;
+    bool queue = !split && (1ull << (P.n - K)) >= min_queue_tiles;
     bool persistent = queue;
     if (const char* g = std::getenv("BFLY_JIT_GRID")) {
         if (g[0] == 't') persistent = queue = false;
@@ -648,7 +672,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     if (!persistent &&
         (1ull << (P.n - P.k)) * static_cast<unsigned long long>(tile_threads_h(P.k, P.slots)) > 0xffffffffull)
         persistent = queue = true;
-    bool prefetch = persistent;
+    bool prefetch = persistent && !split;
     if (const char* pf = std::getenv("BFLY_JIT_PREFETCH")) prefetch = persistent && pf[0] == '1';
     const unsigned long long ntiles = 1ull << (P.n - K);
 
@@ -667,7 +691,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
 
     int B = 5;
     if (const char* pb = std::getenv("BFLY_JIT_PAD_BITS")) B = std::atoi(pb) == 4 ? 4 : 5;
-    const int E = 2 * elem;   // bytes of a complex element in LDS
+    const int E = split ? elem : 2 * elem;   // bytes of an LDS element (one component when split)
     // LDS tile: complex-interleaved elements (re, im adjacent: one 16-byte
     // access per complex128 amplitude, 8 bytes per complex64 one), element j
     // at slot pad(j) = j + (j >> B) + (j >> 2B): one spare slot after every
@@ -841,9 +865,10 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     o("__device__ __forceinline__ unsigned padc(unsigned j) { return j + (j >> %d) + (j >> %d); }\n", B, 2 * B);
     o("extern \"C\" __global__ __launch_bounds__(%d, %d) void bfly_tile_pass(T* __restrict__ re, T* __restrict__ im, "
       "const T* __restrict__ Cp%s) {\n",
-      NT, WPE, queue ? ", unsigned long long* __restrict__ Q" : "");
+      NT, WPE_used, queue ? ", unsigned long long* __restrict__ Q" : "");
     o("  using namespace bfly;\n");
-    o("  __shared__ __attribute__((aligned(16))) C2 s_c[%u];\n", TSP);
+    if (split) o("  __shared__ __attribute__((aligned(16))) T s_c[%u];\n", TSP);
+    else o("  __shared__ __attribute__((aligned(16))) C2 s_c[%u];\n", TSP);
     o("  const unsigned tid = threadIdx.x;\n");
     if (queue) {
         // Work queue: thread 0 takes the next tile index from the pass's
@@ -878,6 +903,34 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
             if (guard) o("  }\n");
         }
     };
+    // Split round trips: one component of the staged layout (from the pr or
+    // pi registers) into the LDS tile, and a sweep's own layout
+    // (thread part, member offsets in bytes) for writing its im parts later.
+    auto emit_stage_write = [&](const char* regs) {
+        for (unsigned c = 0; c < PV; ++c) {
+            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+            o("  { const T* a = reinterpret_cast<const T*>(&%s[%u]);\n", regs, c);
+            for (int v = 0; v < VN; ++v)
+                o("    *reinterpret_cast<T*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = a[%d];\n",
+                  pad(c * NT * VN + static_cast<unsigned>(v)) * E, v);
+            o("  }\n");
+        }
+    };
+    auto sweep_layout = [&](int sidx, std::string& jbexpr, unsigned* moff_out) {
+        const TileSweep& w = hp.sweeps[static_cast<size_t>(P.first_sweep + sidx)];
+        int32_t tp[kMaxTileBits];
+        thread_pos(w, tp);
+        jbexpr = deposit_expr("tid", tp, 0, K - S);
+        for (int m = 0; m < G; ++m) {
+            unsigned d = 0;
+            for (int a = 0; a < S; ++a) d |= ((static_cast<unsigned>(m) >> a) & 1u) << w.s[a];
+            moff_out[m] = pad(d) * E;
+        }
+    };
+    // staged passes with next-tile loads: when split, the staging registers
+    // still hold the first tile's im parts until the first sweep has read
+    // them back, so the next tile's loads are issued there
+    const bool deferred_prefetch = split && persistent && prefetch && !direct_in;
     o("  unsigned long long base = %s;\n", tile_base("tile").c_str());
     // the slot of tv + c NT VN + v (disjoint bits) is pad(tv) + pad(c NT VN + v)
     o("  const unsigned stv = padc(tv) * %du;\n", E);
@@ -885,14 +938,16 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         load("base");
         if (persistent) o("  for (;;) {\n");
         if (queue) o("  if (tid == 0) s_tile[par] = atomicAdd(Q, 1ull);\n");
-        for (unsigned c = 0; c < PV; ++c) {
-            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
-            o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
-            for (int v = 0; v < VN; ++v)
-                o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = e; }\n",
-                  v, v, pad(c * NT * VN + static_cast<unsigned>(v)) * E);
-            o("  }\n");
-        }
+        if (split) emit_stage_write("pr");
+        else
+            for (unsigned c = 0; c < PV; ++c) {
+                if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+                o("  { const T* a = reinterpret_cast<const T*>(&pr[%u]); const T* b = reinterpret_cast<const T*>(&pi[%u]);\n", c, c);
+                for (int v = 0; v < VN; ++v)
+                    o("    { C2 e; e.x = a[%d]; e.y = b[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (stv + %uu)) = e; }\n",
+                      v, v, pad(c * NT * VN + static_cast<unsigned>(v)) * E);
+                o("  }\n");
+            }
         o("  __syncthreads();\n");
     } else if (queue) {
         // the first tile's members in the first sweep's layout; the loop then
@@ -923,7 +978,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         // which waits for the loads before the first sweep.
         o("  const unsigned long long nc = next < %lluull ? next : tile;\n", ntiles);
         o("  const unsigned long long nbase = %s;\n", tile_base("nc").c_str());
-        load("nbase");
+        if (!deferred_prefetch) load("nbase");
     } else {
         if (queue) o("  const unsigned long long next = s_tile[par];\n  par ^= 1u;\n");
         else o("  const unsigned long long next = tile + gridDim.x;\n");
@@ -932,6 +987,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     }
 
     // sweeps
+    if (split) o("  T ar[%d], ai[%d];\n", G, G);
     for (int s = 0; s < P.nsweeps; ++s) {
         const TileSweep& sw = hp.sweeps[static_cast<size_t>(P.first_sweep + s)];
         o("  // sweep %d: slots %d %d %d %d\n", s, sw.s[0], sw.s[1], S > 2 ? sw.s[2] : -1, S > 3 ? sw.s[3] : -1);
@@ -944,7 +1000,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         o("  {\n    unsigned zs = 0u;\n    asm volatile(\"\" : \"+s\"(zs));\n    const T* __restrict__ C = Cp + zs;\n");
         o("    const unsigned jb = static_cast<unsigned>(%s);\n", deposit_expr("tid", tpos, 0, K - S).c_str());
         o("    const unsigned jbs = padc(jb) * %du;\n", E);
-        o("    T ar[%d], ai[%d];\n", G, G);
+        if (!split) o("    T ar[%d], ai[%d];\n", G, G);
         unsigned moff[1 << kSlots];
         for (int m = 0; m < G; ++m) {
             unsigned d = 0;
@@ -961,6 +1017,27 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
             emit_member_io(fio, false, "nr", "ni", "nbase", "gj");
         } else if (from_hbm) {
             emit_member_io(fio, false, "ar", "ai", "base", "gj");
+        } else if (split) {
+            // re of this layout (written at the previous boundary), then the
+            // previous layout's im parts through the same buffer
+            for (int m = 0; m < G; ++m)
+                o("    ar[%d] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu));\n", m, moff[m]);
+            o("    __syncthreads();\n");
+            if (s == 0) {
+                emit_stage_write("pi");
+            } else {
+                std::string jp;
+                unsigned mp[1 << kSlots];
+                sweep_layout(s - 1, jp, mp);
+                o("    { const unsigned jbp = padc(static_cast<unsigned>(%s)) * %du;\n", jp.c_str(), E);
+                for (int m = 0; m < G; ++m)
+                    o("      *reinterpret_cast<T*>(reinterpret_cast<char*>(s_c) + (jbp + %uu)) = ai[%d];\n", mp[m], m);
+                o("    }\n");
+            }
+            o("    __syncthreads();\n");
+            for (int m = 0; m < G; ++m)
+                o("    ai[%d] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu));\n", m, moff[m]);
+            if (s == 0 && deferred_prefetch) load("nbase");
         } else {
             for (int m = 0; m < G; ++m)
                 o("    { const C2 e = *reinterpret_cast<const C2*>(reinterpret_cast<const char*>(s_c) + (jbs + %uu)); ar[%d] = e.x; ai[%d] = e.y; }\n",
@@ -974,6 +1051,10 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
         if (to_hbm) {
             emit_member_io(fio, true, "ar", "ai", "base", "gj");
             o("  }\n");
+        } else if (split) {
+            for (int m = 0; m < G; ++m)
+                o("    *reinterpret_cast<T*>(reinterpret_cast<char*>(s_c) + (jbs + %uu)) = ar[%d];\n", moff[m], m);
+            o("  }\n  __syncthreads();\n");
         } else {
             for (int m = 0; m < G; ++m)
                 o("    { C2 e; e.x = ar[%d]; e.y = ai[%d]; *reinterpret_cast<C2*>(reinterpret_cast<char*>(s_c) + (jbs + %uu)) = e; }\n",
@@ -983,7 +1064,41 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     }
 
     // stage out
-    if (!direct_out) {
+    if (!direct_out && split) {
+        // re from the staged layout, then the last sweep's im parts through
+        // the same buffer
+        o("  { V sr[%u];\n", PV);
+        for (unsigned c = 0; c < PV; ++c) {
+            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+            o("  { T* a = reinterpret_cast<T*>(&sr[%u]);\n", c);
+            for (int v = 0; v < VN; ++v)
+                o("    a[%d] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_c) + (stv + %uu));\n", v,
+                  pad(c * NT * VN + static_cast<unsigned>(v)) * E);
+            o("  }\n");
+        }
+        o("  __syncthreads();\n");
+        {
+            std::string jp;
+            unsigned mp[1 << kSlots];
+            sweep_layout(P.nsweeps - 1, jp, mp);
+            o("  { const unsigned jbp = padc(static_cast<unsigned>(%s)) * %du;\n", jp.c_str(), E);
+            for (int m = 0; m < G; ++m)
+                o("    *reinterpret_cast<T*>(reinterpret_cast<char*>(s_c) + (jbp + %uu)) = ai[%d];\n", mp[m], m);
+            o("  }\n");
+        }
+        o("  __syncthreads();\n");
+        for (unsigned c = 0; c < PV; ++c) {
+            const unsigned long long u = deposit_const(static_cast<unsigned long long>(c) * NT * VN, P.qpos, K);
+            if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);
+            o("  { V vi; T* b = reinterpret_cast<T*>(&vi);\n");
+            for (int v = 0; v < VN; ++v)
+                o("    b[%d] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_c) + (stv + %uu));\n", v,
+                  pad(c * NT * VN + static_cast<unsigned>(v)) * E);
+            o("    *reinterpret_cast<V*>(re + base + toff + %lluull) = sr[%u]; *reinterpret_cast<V*>(im + base + toff + %lluull) = vi; }\n",
+              u, c, u);
+        }
+        o("  }\n");
+    } else if (!direct_out) {
         for (unsigned c = 0; c < PV; ++c) {
             const unsigned long long u = deposit_const(static_cast<unsigned long long>(c) * NT * VN, P.qpos, K);
             if ((c + 1) * static_cast<unsigned>(NT) > NV) o("  if (tid + %uu < %uu)\n", c * NT, NV);

@@ -16,6 +16,9 @@ complex64 against the complex128 oracle: L2 <= 2e-5 and fidelity >= 1 - 1e-6):
   counter and load the next tile during the current one), forced on every
   pass of every sweep form, repeated on a new state, and one plan on two
   states at once;
+- split round trips (complex128 default: re and im through a one-component
+  LDS tile in turn), in every form above, and bit-identical at 26 qubits to
+  16-byte round trips;
 - the measurement switches BFLY_JIT_HOIST=0 (every diagonal action in place),
   BFLY_JIT_GRID=persistent (blocks walk the tiles, with or without the next
   tile's loads during the sweeps: BFLY_JIT_PREFETCH) and BFLY_JIT_DIRECT
@@ -121,7 +124,7 @@ def test_flip_heavy_circuits_vs_oracle(gpu_lib, n, tile_bits, threads):
 ALL_DIRECT = {"BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"}
 # the work-queue launch form (persistent blocks taking tiles from the state's
 # counter, next tile loaded during the current one) on every pass
-ALL_QUEUE = {"BFLY_JIT_QUEUE_MIN_TILES": "1"}
+ALL_QUEUE = {"BFLY_JIT_QUEUE_MIN_TILES": "1", "BFLY_JIT_SPLIT": "0"}
 
 
 @pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
@@ -130,6 +133,26 @@ def test_direct_stage_in_and_out_vs_oracle(gpu_lib, monkeypatch, n, tile_bits, t
         monkeypatch.setenv(k, v)
     ops, ud = flips_and_quarter_turns(n, 300, seed=3 * n + threads)
     st = W.normal_state(n, 4 * n)
+    out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
+    assert info["kernels"] > 0
+    err = np.abs(out - oracle(n, ops, ud, st)).max()
+    assert err <= TOL_C128, (n, tile_bits, threads, err)
+
+
+ALL_SPLIT = {"BFLY_JIT_SPLIT_MIN_SWEEPS": "1"}
+SPLIT_QUEUE = {"BFLY_JIT_SPLIT_MIN_SWEEPS": "1", "BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "1"}
+
+
+@pytest.mark.parametrize("switches", [ALL_SPLIT, SPLIT_QUEUE], ids=["split", "split_queue_prefetch"])
+@pytest.mark.parametrize("n,tile_bits,threads", SWEEP_CASES)
+def test_split_round_trips_vs_oracle(gpu_lib, monkeypatch, n, tile_bits, threads, switches):
+    """Every complex128 pass with re and im through a one-component LDS tile in
+    turn; with the work queue and next-tile loads, staged passes issue those
+    loads once the first sweep has read the staged im parts back."""
+    for k, v in switches.items():
+        monkeypatch.setenv(k, v)
+    ops, ud = flips_and_quarter_turns(n, 300, seed=11 * n + threads)
+    st = W.normal_state(n, 8 * n)
     out, info = run(n, ops, ud, st, tile_bits=tile_bits, threads=threads)
     assert info["kernels"] > 0
     err = np.abs(out - oracle(n, ops, ud, st)).max()
@@ -196,14 +219,19 @@ def test_complex64_sweep_forms_vs_complex128_oracle(gpu_lib, monkeypatch, tile_b
     assert l2 <= 2e-5 and fid >= 1 - 1e-6, (tile_bits, threads, l2, fid)
 
 
-@pytest.mark.parametrize("switches", [{"BFLY_JIT_HOIST": "0"}, {"BFLY_JIT_GRID": "persistent"},
-                                      {"BFLY_JIT_GRID": "queue"}, {"BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "0"},
-                                      {"BFLY_JIT_GRID": "queue", "BFLY_JIT_DIRECT": "1",
-                                       "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
-                                      {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "0"},
-                                      {"BFLY_JIT_DIRECT": "0"}, ALL_DIRECT,
-                                      {"BFLY_JIT_DIRECT": "in", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
-                                      {"BFLY_JIT_DIRECT": "out", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"}])
+@pytest.mark.parametrize("switches", [
+    {"BFLY_JIT_HOIST": "0"},
+    {"BFLY_JIT_GRID": "persistent"}, {"BFLY_JIT_GRID": "persistent", "BFLY_JIT_PREFETCH": "1"},
+    {"BFLY_JIT_GRID": "queue"}, {"BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "1"},
+    {"BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "1", "BFLY_JIT_DIRECT": "1", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
+    {"BFLY_JIT_DIRECT": "0"}, ALL_DIRECT,
+    {"BFLY_JIT_DIRECT": "in", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
+    {"BFLY_JIT_DIRECT": "out", "BFLY_JIT_DIRECT_MIN_SWEEPS": "1"},
+    {"BFLY_JIT_SPLIT": "0"}, {"BFLY_JIT_SPLIT": "0", "BFLY_JIT_DIRECT": "0"},
+    {"BFLY_JIT_SPLIT": "0", "BFLY_JIT_GRID": "queue"},
+    {"BFLY_JIT_SPLIT": "0", "BFLY_JIT_GRID": "queue", "BFLY_JIT_PREFETCH": "0"},
+    {"BFLY_JIT_SPLIT": "0", "BFLY_JIT_GRID": "persistent"},
+])
 def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switches):
     n = 18
     qc = W.random_layered(n, 12, seed=11)
@@ -226,20 +254,24 @@ def test_measurement_switches_give_the_same_amplitudes(gpu_lib, switches):
     assert np.abs(base - ref).max() <= TOL_C128
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-def test_work_queue_default_is_bit_identical_to_one_block_per_tile(gpu_lib, monkeypatch, dtype):
-    """26 qubits: every pass has 2^14 (complex128) or 2^13 (complex64) tiles
-    and takes the work-queue form by default; each tile runs the same code in
-    either form, so the amplitudes equal one block per tile bit for bit."""
+@pytest.mark.parametrize("dtype,other", [(np.float64, {"BFLY_JIT_SPLIT": "0"}),
+                                         (np.float64, {"BFLY_JIT_SPLIT": "0", "BFLY_JIT_GRID": "tile"}),
+                                         (np.float32, {"BFLY_JIT_GRID": "tile"})])
+def test_default_launch_forms_are_bit_identical_at_26_qubits(gpu_lib, monkeypatch, dtype, other):
+    """26 qubits (2^14 complex128 / 2^13 complex64 tiles per pass): complex128
+    passes run split round trips one block per tile by default, complex64
+    passes the work queue; every form applies the same arithmetic to each
+    tile, so the amplitudes equal those of the other forms bit for bit."""
     n = 26
     qc = W.random_layered(n, 10, seed=77)
     ops, ud = lower_circuit(qc)
     st = W.normal_state(n, 9)
     q, info = run(n, ops, ud, st, dtype=dtype)
     assert info["kernels"] > 0
-    monkeypatch.setenv("BFLY_JIT_GRID", "tile")
+    for k, v in other.items():
+        monkeypatch.setenv(k, v)
     t, info_t = run(n, ops, ud, st, dtype=dtype)
-    assert info_t["compiled"] > 0      # a different launch form, compiled anew
+    assert info_t["compiled"] > 0      # a different form, compiled anew
     assert np.array_equal(q, t)
     nrm = float(np.vdot(q, q).real)
     assert abs(nrm - float(np.vdot(st.re + 1j * st.im, st.re + 1j * st.im).real)) <= (1e-9 if dtype == np.float64 else 1e-3)

@@ -77,7 +77,7 @@ def _direct_sides(src):
     # the stage-in offsets; a direct side reads or writes the first / last
     # sweep's members at base + gj
     return ("pr[0] = *" not in src,
-            "= vr; *reinterpret_cast<V*>(im + base + toff" not in src)
+            "*reinterpret_cast<V*>(im + base + toff" not in src)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -101,17 +101,20 @@ def test_direct_stage_in_and_out_only_in_passes_of_six_sweeps(tmp_path, monkeypa
     assert all(all(_direct_sides(s)) for s in every)
 
 
-def test_work_queue_form_from_4096_tiles(tmp_path, monkeypatch):
-    # 2^12-amplitude tiles: 24 qubits make 4096 tiles per pass and take the
-    # work-queue form (tile counter argument, atomic take), 23 qubits do not;
-    # BFLY_JIT_GRID=tile keeps one block per tile
+def test_launch_forms_by_tile_count_and_width(tmp_path, monkeypatch):
+    # 2^12-amplitude tiles: complex128 passes of 6 or more sweeps move re and
+    # im through the LDS in separate round trips (a one-component tile) and
+    # run one block per tile; the other passes keep 16-byte round trips, and
+    # at 24 qubits (4096 tiles per pass) take the work-queue form (tile
+    # counter argument, atomic take), at 23 qubits not; BFLY_JIT_GRID=tile
+    # keeps one block per tile; complex64 passes of 4096 tiles take the queue
     monkeypatch.setenv("BFLY_JIT_DUMP", str(tmp_path))
 
-    def sources(n, **env):
+    def sources(n, dtype=np.float64, **env):
         for k, v in env.items():
             monkeypatch.setenv(k, v)
-        ops, ud = lower_circuit(W.random_layered(n, 4, 7))
-        p = _plan(n, np.float64, ops, ud, tile_bits=12)
+        ops, ud = lower_circuit(W.random_layered(n, 12, 7))
+        p = _plan(n, dtype, ops, ud, tile_bits=12)
         out = [(tmp_path / f"pass{i}.hip").read_text() for i in range(p.jit_info()["kernels"])]
         for f in tmp_path.glob("*.hip"):
             f.unlink()
@@ -119,9 +122,17 @@ def test_work_queue_form_from_4096_tiles(tmp_path, monkeypatch):
             monkeypatch.delenv(k)
         return out
 
-    q24 = sources(24)
-    assert q24 and all("atomicAdd(Q, 1ull)" in s and "unsigned long long* __restrict__ Q" in s for s in q24)
-    assert not any("atomicAdd" in s for s in sources(23))
-    assert not any("atomicAdd" in s for s in sources(24, BFLY_JIT_GRID="tile"))
-    forced = sources(14, BFLY_JIT_QUEUE_MIN_TILES="1")
+    d24 = sources(24)
+    heavy = [s for s in d24 if s.count("// sweep ") >= 6]
+    light = [s for s in d24 if s.count("// sweep ") < 6]
+    assert heavy and all("T s_c[" in s and "atomicAdd" not in s for s in heavy)
+    assert all("C2 s_c[" in s and "atomicAdd(Q, 1ull)" in s for s in light)
+    q24 = sources(24, BFLY_JIT_SPLIT="0")
+    assert q24 and all("atomicAdd(Q, 1ull)" in s and "unsigned long long* __restrict__ Q" in s and "C2 s_c[" in s
+                       for s in q24)
+    assert not any("atomicAdd" in s for s in sources(23, BFLY_JIT_SPLIT="0"))
+    assert not any("atomicAdd" in s for s in sources(24, BFLY_JIT_SPLIT="0", BFLY_JIT_GRID="tile"))
+    forced = sources(14, BFLY_JIT_SPLIT="0", BFLY_JIT_QUEUE_MIN_TILES="1")
     assert forced and all("atomicAdd(Q, 1ull)" in s for s in forced)
+    c64 = sources(25, np.float32)
+    assert c64 and all("atomicAdd(Q, 1ull)" in s for s in c64)
