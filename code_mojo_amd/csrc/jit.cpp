@@ -639,7 +639,10 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     if (const char* ms = std::getenv("BFLY_JIT_SPLIT_MIN_SWEEPS")) split_min_sweeps = std::atoi(ms);
     bool split = !f32 && NT == static_cast<int>(NG) && P.nsweeps > 0 && P.nsweeps >= split_min_sweeps;
     if (const char* sp = std::getenv("BFLY_JIT_SPLIT")) split = split && sp[0] != '0';
-    int WPE_used = split ? 3 : WPE;
+    // three waves per SIMD, rounded up to whole blocks (a block of NT threads
+    // puts NT / 256 waves on each SIMD)
+    const int wave_step = NT >= 256 ? NT / 256 : 1;
+    int WPE_used = split ? std::max(WPE, (3 + wave_step - 1) / wave_step * wave_step) : WPE;
     if (const char* w = std::getenv("BFLY_JIT_WAVES_PER_SIMD")) WPE_used = std::max(1, std::atoi(w));
     // Launch form.  One block per tile (split passes; fewer than 4096 tiles;
     // BFLY_JIT_GRID=tile): the hardware dispatcher starts a new block as soon
