@@ -625,6 +625,10 @@ def bench_layered(args, lib) -> dict:
     # VALU issue time of one launch: 4 cycles per wave64 instruction on one of
     # the 1024 SIMDs (256 CUs x 4) at the 2.4 GHz peak engine clock
     valu_issue_ms = valu * 4 / (1024 * 2.4e9) * 1e3 if valu else None
+    # the same at the engine clock measured under this load (GRBM counters in
+    # the pmc summary): the engine runs below its peak clock here
+    clock_ghz = read_value(f"{kname}_{n}q_layered_{args.dtype}_pmc.md", "engine_clock_ghz_under_load")
+    valu_issue_ms_clk = valu * 4 / (1024 * clock_ghz * 1e9) * 1e3 if (valu and clock_ghz) else None
     # the in-place read + write streaming ceiling measured on MI355X (a copy
     # kernel reading and writing 2 x 8 GiB in place), beside the 8 TB/s spec
     copy_gbs = read_value("hbm_copy_ceiling.md", "copy_read_write_gbs")
@@ -659,7 +663,10 @@ def bench_layered(args, lib) -> dict:
                      "frac_of_measured_copy_ceiling": round(achieved / copy_gbs, 4) if copy_gbs else None,
                      "fp64_tflops": round(fp64 / (per_launch_ms * 1e-3) / 1e12, 2) if fp64 else None,
                      "fp64_frac_of_vector_peak": round(fp64 / (per_launch_ms * 1e-3) / 78.6e12, 4) if fp64 else None,
-                     "valu_issue_frac": round(valu_issue_ms / per_launch_ms, 4) if valu_issue_ms else None},
+                     "valu_issue_frac": round(valu_issue_ms / per_launch_ms, 4) if valu_issue_ms else None,
+                     "engine_clock_ghz_under_load": clock_ghz,
+                     "valu_issue_frac_at_measured_clock": (round(valu_issue_ms_clk / per_launch_ms, 4)
+                                                           if valu_issue_ms_clk else None)},
         "cpu_baseline": None,
     }
     if args.mode == "fused" and not args.first_fit and not args.no_extras:
