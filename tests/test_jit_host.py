@@ -136,3 +136,25 @@ def test_launch_forms_by_tile_count_and_width(tmp_path, monkeypatch):
     assert forced and all("atomicAdd(Q, 1ull)" in s for s in forced)
     c64 = sources(25, np.float32)
     assert c64 and all("atomicAdd(Q, 1ull)" in s for s in c64)
+
+
+def test_split_round_trip_structure(tmp_path, monkeypatch):
+    # complex128 passes of 6 or more sweeps hold one component in LDS: each
+    # sweep boundary is re out, barrier, re in, barrier, the previous layout's
+    # im out, barrier, im in (three barriers), and the launch bound asks for
+    # three waves per SIMD; complex64 passes keep 16-byte elements
+    srcs = _dump_sources(tmp_path, 20, np.float64, {}, monkeypatch)
+    split = [s for s in srcs if "T s_c[" in s]
+    assert split
+    for s in split:
+        nsw = s.count("// sweep ")
+        assert nsw >= 6
+        assert "__launch_bounds__(256, 3)" in s
+        # one im write at the previous layout per boundary after the first sweep
+        assert s.count("const unsigned jbp = padc(") >= nsw - 1
+        assert s.count("__syncthreads()") >= 3 * (nsw - 1)
+    for s in srcs:
+        if "T s_c[" not in s:
+            assert s.count("// sweep ") < 6 and "C2 s_c[" in s
+    f32 = _dump_sources(tmp_path, 20, np.float32, {}, monkeypatch)
+    assert f32 and all("C2 s_c[" in s for s in f32)
