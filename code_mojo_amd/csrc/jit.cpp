@@ -693,7 +693,7 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     };
 
     int B = 5;
-    if (const char* pb = std::getenv("BFLY_JIT_PAD_BITS")) B = std::atoi(pb) == 4 ? 4 : 5;
+    if (const char* pb = std::getenv("BFLY_JIT_PAD_BITS")) B = std::max(4, std::min(6, std::atoi(pb)));
     const int E = split ? elem : 2 * elem;   // bytes of an LDS element (one component when split)
     // LDS tile: complex-interleaved elements (re, im adjacent: one 16-byte
     // access per complex128 amplitude, 8 bytes per complex64 one), element j
@@ -712,7 +712,9 @@ JitSource jit_generate(const HostPlan& hp, const Step& st) {
     // immediate offsets; an XOR swizzle needs one XOR per member, which the
     // compiler hoists out of the tile loop for every sweep at once (208
     // VGPRs for 13 sweeps of 16 members: spills).  BFLY_JIT_PAD_BITS=4 keeps
-    // the 16-slot rows for measurement.
+    // the 16-slot rows for measurement; 6 (64-slot rows) measured slower for
+    // the one-component round trips too (269 vs 266 ms on the 30-qubit
+    // layered circuit).
     auto pad = [&](unsigned j) { return j + (j >> B) + (j >> (2 * B)); };
     const unsigned TSP = pad(TS - 1) + 1;   // slots of the padded tile
     const int VN = f32 ? 4 : 2;   // amplitudes per 16-byte stage-in access
